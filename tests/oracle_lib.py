@@ -1,0 +1,195 @@
+"""ctypes binding of the CPU oracle (oracle/build/liborc.so).
+
+Test infrastructure: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "build", "liborc.so")
+
+CEDA, MMS = 0, 1
+RKC, RKL, DIRK = 0, 1, 2
+ST = dict(nst=0, attempts=1, nfe=2, netf=3, nstlim=4, ndomeig=5, ndee_iters=6,
+          stage_max=7, last_stages=8, nfe_dee=9, nli=10, npsetup=11, npsolve=12,
+          nni=13, ncfn=14, nlsfail=15, njtimes=16, nfe_ls=17)
+
+
+class Problem(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nx", C.c_int64), ("ny", C.c_int64), ("n", C.c_int64),
+                ("xl", C.c_double), ("xu", C.c_double), ("yl", C.c_double), ("yu", C.c_double),
+                ("dx", C.c_double), ("dy", C.c_double), ("kx", C.c_double), ("ky", C.c_double),
+                ("tf", C.c_double), ("inhomogeneous", C.c_int)] + \
+               [(f, C.POINTER(C.c_double)) for f in ("fx", "fy", "px", "py", "sx", "sy", "cx", "cy", "diag")] + \
+               [("nrhs", C.c_long)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER(Problem)
+        d, i, vp, dp = C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_double)
+        sig = {
+            "orc_problem_defaults": (None, [P, i]),
+            "orc_problem_setup": (i, [P]),
+            "orc_problem_free": (None, [P]),
+            "orc_initial": (None, [P, dp]),
+            "orc_exact": (None, [P, d, dp]),
+            "orc_rhs": (i, [P, d, dp, dp]),
+            "orc_stencil": (None, [P, dp, dp]),
+            "orc_psetup": (None, [P, d]),
+            "orc_psolve": (None, [P, dp, dp]),
+            "orc_dom_eig": (None, [P, dp, dp]),
+            "orc_seed_vector": (None, [C.c_int64, C.c_uint64, dp]),
+            "orc_create": (vp, [P, i, d, dp]),
+            "orc_free": (None, [vp]),
+            "orc_set_tolerances": (None, [vp, d, d]),
+            "orc_set_fixed_step": (None, [vp, d]),
+            "orc_set_init_step": (None, [vp, d]),
+            "orc_set_controller": (None, [vp, i]),
+            "orc_set_max_num_steps": (None, [vp, C.c_long]),
+            "orc_set_internal_eig": (None, [vp, i, i, d]),
+            "orc_set_dom_eig_frequency": (None, [vp, C.c_long]),
+            "orc_set_max_num_stages": (None, [vp, i]),
+            "orc_set_dom_eig_safety": (None, [vp, d]),
+            "orc_set_dee_preprocess": (None, [vp, i, i]),
+            "orc_set_order": (None, [vp, i]),
+            "orc_set_linear": (None, [vp, i]),
+            "orc_set_lin_solver": (None, [vp, i, d, i, i]),
+            "orc_set_ls_reltol": (None, [vp, d]),
+            "orc_evolve": (i, [vp, d, dp, dp]),
+            "orc_get_stats": (i, [vp, C.POINTER(C.c_long), i]),
+            "orc_get_current_h": (d, [vp]),
+            "orc_get_spectral_radius": (d, [vp]),
+            "orc_test_linear_solve": (i, [vp, d, d, dp, dp, d, i, C.POINTER(C.c_long)]),
+            "orc_dirk_table": (i, [i, dp, dp, dp, dp]),
+            "orc_vlinearcombination": (None, [i, dp, C.POINTER(dp), dp, C.c_int64]),
+            "orc_vwrmsnorm": (d, [dp, dp, C.c_int64]),
+            "orc_vdot": (d, [dp, dp, C.c_int64]),
+            "orc_vmaxnorm": (d, [dp, C.c_int64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleProblem:
+    """Owns an orc_problem; kwargs override the reference defaults."""
+
+    def __init__(self, kind=CEDA, **kw):
+        self.p = Problem()
+        lib().orc_problem_defaults(C.byref(self.p), kind)
+        for k, v in kw.items():
+            setattr(self.p, k, v)
+        if lib().orc_problem_setup(C.byref(self.p)) != 0:
+            raise ValueError("oracle problem setup failed")
+        self.n = self.p.n
+        self.shape = (self.p.ny, self.p.nx)
+
+    def __del__(self):
+        try:
+            lib().orc_problem_free(C.byref(self.p))
+        except Exception:
+            pass
+
+    def table(self, name, length):
+        return np.ctypeslib.as_array(getattr(self.p, name), shape=(length,)).copy()
+
+    def initial(self):
+        u = np.zeros(self.n)
+        lib().orc_initial(C.byref(self.p), ptr(u))
+        return u
+
+    def exact(self, t):
+        u = np.zeros(self.n)
+        lib().orc_exact(C.byref(self.p), t, ptr(u))
+        return u
+
+    def rhs(self, t, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        f = np.zeros(self.n)
+        lib().orc_rhs(C.byref(self.p), t, ptr(u), ptr(f))
+        return f
+
+    def psetup(self, gamma):
+        lib().orc_psetup(C.byref(self.p), gamma)
+        return np.ctypeslib.as_array(self.p.diag, shape=(self.n,)).copy()
+
+    def psolve(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.n)
+        lib().orc_psolve(C.byref(self.p), ptr(r), ptr(z))
+        return z
+
+    def dom_eig(self):
+        lr, li = C.c_double(), C.c_double()
+        lib().orc_dom_eig(C.byref(self.p), C.byref(lr), C.byref(li))
+        return lr.value, li.value
+
+
+def seed_vector(n, seed):
+    v = np.zeros(n)
+    lib().orc_seed_vector(n, seed, ptr(v))
+    return v
+
+
+class OracleIntegrator:
+    def __init__(self, prob, method, y0, t0=0.0):
+        self.prob = prob
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        self.h = lib().orc_create(C.byref(prob.p), method, t0, ptr(y0))
+        self.n = prob.n
+
+    def __del__(self):
+        try:
+            lib().orc_free(self.h)
+        except Exception:
+            pass
+
+    def __getattr__(self, name):
+        f = getattr(lib(), "orc_" + name)
+        return lambda *a: f(self.h, *a)
+
+    def evolve(self, tout):
+        y = np.zeros(self.n)
+        t = C.c_double()
+        flag = lib().orc_evolve(self.h, tout, ptr(y), C.byref(t))
+        return flag, t.value, y
+
+    def stats(self):
+        out = (C.c_long * 32)()
+        m = lib().orc_get_stats(self.h, out, 32)
+        return {k: out[v] for k, v in ST.items() if v < m}
+
+    def linear_solve(self, t, gamma, b, reltol, maxl):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n)
+        it = C.c_long()
+        ret = lib().orc_test_linear_solve(self.h, t, gamma, ptr(b), ptr(x), reltol, maxl, C.byref(it))
+        return ret, x, it.value
+
+
+def dirk_table(order):
+    A = np.zeros(16)
+    b = np.zeros(4)
+    bh = np.zeros(4)
+    c = np.zeros(4)
+    s = lib().orc_dirk_table(order, ptr(A), ptr(b), ptr(bh), ptr(c))
+    return A.reshape(4, 4)[:s, :s], b[:s], bh[:s], c[:s]

@@ -1,0 +1,376 @@
+"""Pins the CPU oracle against the golden fixtures and closed-form mathematics.
+
+Each check is chosen so that a plausible mistake in the restatement (a wrong
+face position, sign, stage coefficient, stage time, tableau entry, solver
+update) fails it; none of them retypes the oracle's own formula.
+"""
+import os
+import subprocess
+
+import numpy as np
+import numpy.polynomial.chebyshev as npc
+import numpy.polynomial.legendre as npl
+import pytest
+
+import oracle_lib as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+# ------------------------------------------------------------------ stencil
+@pytest.mark.parametrize("inhom", [0, 1])
+def test_stencil_matches_dense_reference_matrix(inhom):
+    g = gold("dense_ops.npz")
+    nx, ny, kx, ky, _ = g[f"periodic_inh{inhom}_params"]
+    P = O.OracleProblem(O.CEDA, nx=int(nx), ny=int(ny), kx=kx, ky=ky, inhomogeneous=inhom)
+    M = g[f"periodic_inh{inhom}_M"]
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        u = rng.standard_normal(P.n)
+        np.testing.assert_allclose(P.rhs(0.0, u), M @ u, rtol=1e-12, atol=1e-12 * np.abs(M).max())
+
+
+@pytest.mark.parametrize("inhom", [0, 1])
+def test_jacobi_matches_reference_diagonal(inhom):
+    g = gold("dense_ops.npz")
+    nx, ny, kx, ky, _ = g[f"periodic_inh{inhom}_params"]
+    P = O.OracleProblem(O.CEDA, nx=int(nx), ny=int(ny), kx=kx, ky=ky, inhomogeneous=inhom)
+    d = P.psetup(0.01)
+    np.testing.assert_allclose(d, g[f"periodic_inh{inhom}_jacobi"].ravel(), rtol=1e-14)
+    r = np.arange(P.n, dtype=float) - 3.0
+    np.testing.assert_array_equal(P.psolve(r), d * r)
+
+
+@pytest.mark.parametrize("nx,ny,mx,my", [(16, 12, 3, 2), (10, 14, 5, 7), (8, 8, 4, 4)])
+def test_periodic_fourier_modes_are_eigenvectors(nx, ny, mx, my):
+    kx, ky = 1.7, 0.4
+    P = O.OracleProblem(O.CEDA, nx=nx, ny=ny, kx=kx, ky=ky)
+    dx, dy = P.p.dx, P.p.dy
+    thx, thy = 2 * np.pi * mx / nx, 2 * np.pi * my / ny
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny))
+    u = np.cos(thx * i + thy * j).ravel()
+    lam = -(4 * kx / dx**2) * np.sin(thx / 2) ** 2 - (4 * ky / dy**2) * np.sin(thy / 2) ** 2
+    np.testing.assert_allclose(P.rhs(0.0, u), lam * u, atol=1e-11 * abs(lam))
+
+
+def test_periodic_homogeneous_conserves_mass():
+    P = O.OracleProblem(O.CEDA, nx=24, ny=18, kx=2.0, ky=0.3)
+    u = np.random.default_rng(3).standard_normal(P.n)
+    f = P.rhs(0.0, u)
+    assert abs(f.sum()) < 1e-10 * np.abs(f).sum()
+
+
+@pytest.mark.parametrize("p,q", [(1, 1), (3, 5), (16, 1)])
+def test_dirichlet_sine_modes_are_eigenvectors(p, q):
+    nx, ny, kx, ky = 16, 12, 1.0, 0.6
+    P = O.OracleProblem(O.MMS, nx=nx, ny=ny, kx=kx, ky=ky)
+    dx, dy = P.p.dx, P.p.dy
+    I, J = np.meshgrid(np.arange(nx) + 1, np.arange(ny) + 1)
+    u = (np.sin(p * np.pi * I / (nx + 1)) * np.sin(q * np.pi * J / (ny + 1))).ravel()
+    lam = -(4 * kx / dx**2) * np.sin(p * np.pi / (2 * (nx + 1))) ** 2 \
+          - (4 * ky / dy**2) * np.sin(q * np.pi / (2 * (ny + 1))) ** 2
+    Lu = P.rhs(0.3, u) - P.rhs(0.3, np.zeros(P.n))
+    np.testing.assert_allclose(Lu, lam * u, atol=1e-10 * abs(lam))
+
+
+def test_mms_forcing_is_consistent_with_exact_solution():
+    """rhs(t, u_exact(t)) - du_exact/dt is the O(dx^2) truncation error."""
+    errs = []
+    t = 0.37
+    for n in (16, 32, 64):
+        P = O.OracleProblem(O.MMS, nx=n, ny=n, kx=1.0, ky=0.5)
+        ue = P.exact(t)
+        # d/dt cos^2(pi t) = -pi sin(2 pi t)
+        dudt = ue / np.cos(np.pi * t) ** 2 * (-np.pi * np.sin(2 * np.pi * t))
+        errs.append(np.abs(P.rhs(t, ue) - dudt).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 1.8) and np.all(rates < 2.2), (errs, rates)
+
+
+def test_dom_eig_closed_form():
+    P = O.OracleProblem(O.CEDA, nx=40, ny=30, kx=0.7, ky=2.0)
+    lr, li = P.dom_eig()
+    assert li == 0.0
+    assert lr == -8.0 * max(0.7 / P.p.dx / P.p.dx, 2.0 / P.p.dy / P.p.dy)
+
+
+def test_mms_initial_and_exact():
+    P = O.OracleProblem(O.MMS, nx=9, ny=7)
+    x = (np.arange(9) + 1) / 10.0
+    y = (np.arange(7) + 1) / 8.0
+    u0 = (np.sin(np.pi * y)[:, None] ** 2 * np.sin(np.pi * x)[None, :] ** 2).ravel()
+    np.testing.assert_allclose(P.initial(), u0, rtol=1e-15, atol=1e-16)
+    np.testing.assert_allclose(P.exact(0.25), u0 * np.cos(np.pi * 0.25) ** 2, rtol=1e-15, atol=1e-16)
+    assert np.abs(P.exact(0.5)).max() < 1e-30
+
+
+def test_periodic_initial_condition():
+    P = O.OracleProblem(O.CEDA, nx=11, ny=9)
+    g = gold("periodic_fft.npz")
+    P2 = O.OracleProblem(O.CEDA, nx=32, ny=24)
+    np.testing.assert_allclose(P2.initial(), g["32x24_u0"].ravel(), rtol=1e-14)
+    assert P.initial().shape == (99,)
+
+
+# ------------------------------------------------------- STS recurrences
+def rkc_R(s, z):
+    eps = 2.0 / 13.0
+    w0 = 1.0 + eps / s**2
+    c = np.zeros(s + 1)
+    c[s] = 1.0
+    T = npc.chebval(w0, c)
+    T1 = npc.chebval(w0, npc.chebder(c))
+    T2 = npc.chebval(w0, npc.chebder(c, 2))
+    w1 = T1 / T2
+    b = T2 / T1**2
+    a = 1.0 - b * T
+    return a + b * npc.chebval(w0 + w1 * z, c)
+
+
+def rkl_R(s, z):
+    w1 = 4.0 / (s * s + s - 2)
+    b = (s * s + s - 2) / (2.0 * s * (s + 1))
+    c = np.zeros(s + 1)
+    c[s] = 1.0
+    return 1.0 - b + b * npl.legval(1.0 + w1 * z, c)
+
+
+@pytest.mark.parametrize("method", [O.RKC, O.RKL])
+@pytest.mark.parametrize("s", [2, 3, 7, 50, 199])
+def test_one_step_equals_stability_polynomial(method, s):
+    """One fixed STS step on an eigenmode multiplies it by R_s(h lambda)."""
+    nx = ny = 8
+    P = O.OracleProblem(O.CEDA, nx=nx, ny=ny, kx=1.0, ky=1.0)
+    rho = 1.01 * abs(P.dom_eig()[0])
+    h = (s - 0.5) ** 2 / (1.54 * rho) if method == O.RKC else (4 * s * s - 9) / (8.0 * rho)
+    for (mx, my) in [(1, 0), (2, 3), (4, 4), (3, 1)]:
+        thx, thy = 2 * np.pi * mx / nx, 2 * np.pi * my / ny
+        i, j = np.meshgrid(np.arange(nx), np.arange(ny))
+        u0 = np.cos(thx * i + thy * j).ravel()
+        lam = -(4 / P.p.dx**2) * np.sin(thx / 2) ** 2 - (4 / P.p.dy**2) * np.sin(thy / 2) ** 2
+        I = O.OracleIntegrator(P, method, u0)
+        I.set_fixed_step(h)
+        flag, t, u1 = I.evolve(h)
+        assert flag == 0 and t == h
+        assert I.stats()["last_stages"] == s
+        R = rkc_R(s, h * lam) if method == O.RKC else rkl_R(s, h * lam)
+        np.testing.assert_allclose(u1, R * u0, atol=1e-11 * max(1.0, abs(R)))
+
+
+@pytest.mark.parametrize("method", [O.RKC, O.RKL])
+def test_adaptive_matches_fft_exact_solution(method):
+    g = gold("periodic_fft.npz")
+    for key in ("32x24", "16x20"):
+        nx, ny, kx, ky = g[f"{key}_params"]
+        P = O.OracleProblem(O.CEDA, nx=int(nx), ny=int(ny), kx=kx, ky=ky)
+        I = O.OracleIntegrator(P, method, P.initial())
+        I.set_tolerances(1e-9, 1e-12)
+        I.set_max_num_steps(100000)
+        for t in (0.05, 0.1):
+            flag, tr, u = I.evolve(t)
+            assert flag == 0 and tr == t
+            ue = g[f"{key}_t{t}"].ravel()
+            err = np.abs(u - ue).max() / np.abs(ue).max()
+            assert err < 2e-7, (key, t, err)
+
+
+@pytest.mark.parametrize("method", [O.RKC, O.RKL])
+def test_fixed_step_second_order_with_time_dependent_forcing(method):
+    """Stage times enter through the MMS forcing; a wrong c_j drops the order."""
+    g = gold("mms_dst.npz")
+    nx, ny, kx, ky = g["24x20_params"]
+    ue = g["24x20_t0.3"].ravel()
+    errs = []
+    for N in (20, 40, 80):
+        P = O.OracleProblem(O.MMS, nx=int(nx), ny=int(ny), kx=kx, ky=ky)
+        I = O.OracleIntegrator(P, method, P.initial())
+        I.set_fixed_step(0.3 / N)
+        I.set_max_num_steps(100000)
+        flag, t, u = I.evolve(0.3)
+        assert flag == 0 and abs(t - 0.3) < 1e-15
+        errs.append(np.abs(u - ue).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 1.85) and np.all(rates < 2.3), (errs, rates)
+
+
+def test_adaptive_mms_matches_dst_exact_solution():
+    g = gold("mms_dst.npz")
+    nx, ny, kx, ky = g["16x16_params"]
+    for method in (O.RKC, O.RKL, O.DIRK):
+        P = O.OracleProblem(O.MMS, nx=int(nx), ny=int(ny), kx=kx, ky=ky)
+        I = O.OracleIntegrator(P, method, P.initial())
+        I.set_tolerances(1e-8, 1e-12)
+        I.set_max_num_steps(100000)
+        if method == O.DIRK:
+            I.set_lin_solver(200, 0.0, 0, 1)
+        for t in (0.1, 0.5):
+            flag, tr, u = I.evolve(t)
+            assert flag == 0
+            err = np.abs(u - g[f"16x16_t{t}"].ravel()).max()
+            assert err < 5e-6, (method, t, err)
+
+
+# ------------------------------------------------------- power iteration
+def test_power_iteration_spectral_radius():
+    nx, ny, kx, ky = 32, 24, 1.0, 0.5
+    P = O.OracleProblem(O.CEDA, nx=nx, ny=ny, kx=kx, ky=ky)
+    true = 4 * kx / P.p.dx**2 + 4 * ky / P.p.dy**2  # theta = pi mode (even nx, ny)
+    # tight: many iterations converge to the closed form from below
+    I = O.OracleIntegrator(P, O.RKC, P.initial())
+    I.set_internal_eig(1, 3000, 1e-12)
+    I.set_dom_eig_safety(1.0)
+    I.set_fixed_step(1e-4)
+    I.evolve(1e-4)
+    est = I.get_spectral_radius()
+    assert est <= true * (1 + 1e-10) and est > true * (1 - 2e-3), (est, true)
+    # the reference's (100, 0.01) settings: stops early, below the true radius
+    I2 = O.OracleIntegrator(P, O.RKC, P.initial())
+    I2.set_internal_eig(1, 100, 0.01)
+    I2.set_dom_eig_safety(1.0)
+    I2.set_fixed_step(1e-4)
+    I2.evolve(1e-4)
+    est2 = I2.get_spectral_radius()
+    assert 0.8 * true < est2 <= true * (1 + 1e-10)
+    assert I2.stats()["ndee_iters"] >= 2
+
+
+def test_seed_vector_is_splitmix64():
+    def sm(x):
+        m = (1 << 64) - 1
+        x = (x + 0x9E3779B97F4A7C15) & m
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+        return x ^ (x >> 31)
+
+    v = O.seed_vector(50, 12345)
+    ref = np.array([(sm(12345 + i) >> 11) * 2.0**-53 - 0.5 for i in range(50)])
+    np.testing.assert_array_equal(v, ref)
+    # published SplitMix64 output for seed 0 (first draw)
+    assert sm(0) == 0xE220A8397B1DCDAF
+
+
+# ------------------------------------------------------- implicit path
+@pytest.mark.parametrize("order", [2, 3])
+def test_dirk_order_conditions(order):
+    A, b, bh, c = O.dirk_table(order)
+    np.testing.assert_allclose(A.sum(axis=1), c, atol=1e-15)
+    assert np.allclose(np.triu(A, 1), 0.0)
+    assert abs(b.sum() - 1) < 1e-15 and abs(b @ c - 0.5) < 1e-15
+    assert abs(bh.sum() - 1) < 1e-15
+    if order == 3:
+        assert abs(b @ c**2 - 1 / 3) < 1e-14
+        assert abs(b @ (A @ c) - 1 / 6) < 1e-14
+        assert abs(bh @ c - 0.5) < 1e-14
+        np.testing.assert_allclose(b, A[-1], rtol=0, atol=0)  # stiffly accurate
+        assert abs((b - bh).sum()) < 1e-15
+
+
+@pytest.mark.parametrize("gamma", [1e-3, 1e-1])
+def test_pcg_solve_matches_dst_exact(gamma):
+    g = gold("dirichlet_solve.npz")
+    nx, ny, kx, ky = g["params"]
+    P = O.OracleProblem(O.MMS, nx=int(nx), ny=int(ny), kx=kx, ky=ky)
+    I = O.OracleIntegrator(P, O.DIRK, P.initial())
+    b = g[f"g{gamma}_b"].ravel()
+    ret, x, it = I.linear_solve(0.2, gamma, b, 1e-13, 2000)
+    assert ret == 0 and it > 0
+    # the Jacobian-vector product is a difference quotient (as in the reference,
+    # which sets no Jtimes, main.cpp:302-329): it limits agreement to ~1e-9
+    np.testing.assert_allclose(x, g[f"g{gamma}_x"].ravel(), rtol=0, atol=2e-8 * np.abs(x).max())
+
+
+def test_dirk3_fixed_step_third_order():
+    g = gold("mms_dst.npz")
+    nx, ny, kx, ky = g["24x20_params"]
+    ue = g["24x20_t0.3"].ravel()
+    errs = []
+    for N in (6, 12, 24):
+        P = O.OracleProblem(O.MMS, nx=int(nx), ny=int(ny), kx=kx, ky=ky)
+        I = O.OracleIntegrator(P, O.DIRK, P.initial())
+        I.set_order(3)
+        I.set_lin_solver(2000, 0.0, 0, 1)
+        I.set_ls_reltol(1e-13)
+        I.set_fixed_step(0.3 / N)
+        I.set_max_num_steps(100000)
+        flag, t, u = I.evolve(0.3)
+        assert flag == 0
+        errs.append(np.abs(u - ue).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 2.6) and np.all(rates < 3.4), (errs, rates)
+
+
+def test_dirk_nonlinear_newton_converges():
+    P = O.OracleProblem(O.CEDA, nx=20, ny=20, inhomogeneous=1)
+    I = O.OracleIntegrator(P, O.DIRK, P.initial())
+    I.set_order(3)
+    I.set_linear(0)
+    I.set_lin_solver(50, 0.0, 1, 1)
+    I.set_tolerances(1e-6, 1e-11)
+    flag, t, u = I.evolve(0.2)
+    st = I.stats()
+    assert flag == 0 and t == 0.2
+    assert st["nni"] >= 3 * st["nst"]  # >= one Newton iteration per implicit stage
+    assert st["npsetup"] >= st["nst"] // 2
+
+
+# ------------------------------------------------------- vector known answers
+def test_vector_known_answers():
+    """Constants of gkeyll_diffusion/test_nvector_gkylzero.c:97-324."""
+    n = 1000
+    a, b, c, d = 2.0, 3.0, 1.75, 2.89
+    x = np.full(n, c)
+    y = np.full(n, d)
+    z = np.zeros(n)
+    import ctypes as C
+    X = (C.POINTER(C.c_double) * 2)(O.ptr(x), O.ptr(y))
+    cc = np.array([a, b])
+    O.lib().orc_vlinearcombination(2, O.ptr(cc), X, O.ptr(z), n)
+    assert np.all(np.abs(z - (a * c + b * d)) < 1e-10)
+    assert abs(O.lib().orc_vwrmsnorm(O.ptr(np.full(n, -0.5)), O.ptr(np.full(n, 0.5)), n) - 0.25) < 1e-10
+    assert O.lib().orc_vmaxnorm(O.ptr(np.zeros(n)), n) == 0.0
+    assert abs(O.lib().orc_vdot(O.ptr(x), O.ptr(y), n) - n * c * d) < 1e-9
+
+
+# ------------------------------------------------------- driver contract
+def parse_like_harness(out):
+    """The token positions used by diffusion_2D/runtests-diffusion2d.py:36-54."""
+    st = {"FEvals": 0}
+    for line in out.split("\n"):
+        txt = line.split()
+        if "Steps" in txt:
+            st["Steps"] = int(txt[2])
+        elif "Error" in txt and "test" in txt:
+            st["Fails"] = int(txt[4])
+        elif "Maximum" in txt and "relative" in txt:
+            st["Accuracy"] = float(txt[4])
+        elif "Explicit" in txt and "RHS" in txt:
+            st["FEvals"] += int(txt[5])
+        elif "Implicit" in txt and "RHS" in txt:
+            st["FEvals"] += int(txt[5])
+        elif "LS" in txt and "RHS" in txt:
+            st["FEvals"] += int(txt[5])
+        elif "RHS" in txt and "fn" in txt:
+            st["FEvals"] += int(txt[4])
+        elif "simulation" in txt:
+            st["Runtime"] = float(txt[4])
+    return st
+
+
+@pytest.mark.parametrize("integ", ["rkc", "rkl", "dirk"])
+def test_driver_stdout_contract(integ):
+    exe = os.path.join(O.ROOT, "oracle", "build", "diffusion_2D_oracle")
+    args = [exe, "--nx", "16", "--ny", "16", "--integrator", integ, "--error", "--maxsteps", "100000",
+            "--inhomogeneous", "--atol", "1e-11", "--controller", "2", "--nout", "4", "--rtol", "1e-4"]
+    if integ == "dirk":
+        args += ["--nonlinear", "--msbp", "1", "--order", "3"]
+    else:
+        args += ["--internaleig"]
+    out = subprocess.run(args, capture_output=True, text=True, check=True).stdout
+    st = parse_like_harness(out)
+    assert st["Steps"] > 0 and st["FEvals"] > st["Steps"]
+    assert 0 < st["Accuracy"] < 1e-2
+    assert st["Runtime"] >= 0 and "Fails" in st
