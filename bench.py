@@ -1,0 +1,167 @@
+"""Benchmark of the diffusion_2D hot path on one MI355X.
+
+Workload (the GPU configuration the headline metric is quoted on): the
+manufactured 2-D heat equation u_t = kx u_xx + ky u_yy + b(x,y,t) on [0,1]^2,
+4096 x 4096 interior nodes, kx = 1, ky = 0.5, FP64, LSRKStep RKL2 with the
+spectral radius from power iteration every 25 steps, rtol 1e-6, atol 1e-10;
+fused stencil/stage kernels replayed as one HIP graph per step.  A "step" is
+one adaptive integrator step (s RKL2 stages + the embedded error estimate,
+plus the power iteration when it is due).  Inputs are resident in HBM before
+the timed region; the reported value counts every right-hand-side evaluation
+(stages, error estimate, power-iteration J v products) times the cells.
+
+Extra objects on the JSON line:
+  roofline      the fused stage kernel (the dominant kernel: >98% of the
+                step's HBM traffic), 40 algorithmic bytes per cell per launch,
+                mean launch time from HIP events on the integrator's stream
+  cpu_baseline  the CPU oracle (serial C restatement, 1 thread) timed on one
+                full adaptive step of the same workload
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "cell-RHS evals/s; time-to-solution at fixed L2 err, batched/learned vs plain"
+HBM_PEAK_GBS = 8000.0
+N_SIDE = 4096
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def hip_sync():
+    C.CDLL("libamdhip64.so").hipDeviceSynchronize()
+
+
+def read_traffic():
+    """HBM bytes per fused-stage launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    path = os.path.join(ROOT, "profiles", "fused_stage_4096_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def make_integrator(pk, ctx, fast=2):
+    G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
+    I = pk.Integrator(G, "rkl", G.initial())
+    I.ARKodeSStolerances(1e-6, 1e-10)
+    I.internal_eig(100, 0.01)
+    I.LSRKStepSetDomEigFrequency(25)
+    I.ARKodeSetMaxNumSteps(1000000)
+    I.ARKodeSetStopTime(1.0)
+    if fast:
+        assert I.LSRKStepSetDiffusionFastPath(G.h, 1 if fast > 1 else 0) == 0
+    return G, I
+
+
+def roofline(pk, ctx, G):
+    L = pk.lib()
+    vs = [G.vector() for _ in range(5)]
+    for k, v in enumerate(vs):
+        L.N_VConst(1.0 + 0.1 * k, v.h)
+    avg = C.c_double()
+    rc = L.DiffusionProblem_TimeKernel(G.h, 1, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, 50,
+                                       C.byref(avg))
+    assert rc == 0
+    avg_s = avg.value * 1e-3
+    bytes_alg = 40.0 * G.n
+    achieved = bytes_alg / avg_s / 1e9
+    rc = L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, 50, C.byref(avg))
+    stencil_gbs = 16.0 * G.n / (avg.value * 1e-3) / 1e9
+    for v in vs:
+        v.destroy()
+    return {"kernel": "k_stencil<2,false,true,true> (fused STS stage)", "bound": "hbm",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+            "bytes_per_launch_algorithmic": bytes_alg, "launch_ms": round(avg_s * 1e3, 4),
+            "rhs_stencil_GBs": round(stencil_gbs, 1)}
+
+
+def cpu_baseline():
+    """One adaptive RKL2 step of the same workload on the CPU oracle (serial)."""
+    import oracle_lib as O
+
+    P = O.OracleProblem(O.MMS, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
+    R = O.OracleIntegrator(P, O.RKL, P.initial())
+    R.set_tolerances(1e-6, 1e-10)
+    R.set_internal_eig(1, 100, 0.01)
+    R.set_dom_eig_frequency(25)
+    R.set_max_num_steps(1)
+    t0 = time.perf_counter()
+    R.evolve(1.0)
+    dt = time.perf_counter() - t0
+    st = R.stats()
+    return {"value": st["nfe"] * P.n / dt, "unit": "cell-RHS evals/s", "cores": 1, "kind": "port",
+            "sample": f"first adaptive step of the 4096^2 RKL2 workload incl. initial-step "
+                      f"estimate and power iteration: {st['nfe']} RHS evals in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--fastpath", type=int, default=2, help="0 API path, 1 fused, 2 fused+graphs")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (every box here has one MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    from hiplib import package
+
+    pk = package()
+    ctx = pk.Context()
+    G, I = make_integrator(pk, ctx, args.fastpath)
+    log(f"workload ready: {G.nx}x{G.ny} cells, warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        flag, t = I.evolve(1.0, pk.ARK_ONE_STEP)
+        assert flag >= 0, flag
+    hip_sync()
+    s0 = I.stats()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        flag, t = I.evolve(1.0, pk.ARK_ONE_STEP)
+        assert flag >= 0, flag
+    hip_sync()
+    dt = time.perf_counter() - t0
+    s1 = I.stats()
+    nfe = s1["nfe"] - s0["nfe"]
+    value = nfe * G.n / dt
+    log(f"{nfe} RHS evals in {dt:.3f} s; stages/step {nfe / args.steps:.1f}; t = {t:.4e}")
+    rl = roofline(pk, ctx, G)
+    log(f"fused stage kernel {rl['launch_ms']} ms -> {rl['achieved']} GB/s")
+    I.free()
+    G.destroy()
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, 1 thread)")
+        cpu = cpu_baseline()
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-RHS evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (manufactured-solution initial state, no dataset)",
+        "config": {"workload": "mms-dirichlet 4096x4096 interior FP64, kx=1 ky=0.5, LSRKStep RKL2, "
+                               "power-iteration spectral radius every 25 steps, rtol=1e-6 atol=1e-10",
+                   "path": ["api", "fused", "fused+hipgraph"][args.fastpath],
+                   "cells": G.n, "rhs_evals_timed": nfe, "stages_per_step": nfe / args.steps},
+        "roofline": rl, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,86 @@
+// Internal declarations shared by the HIP N_Vector, the problem kernels and
+// the integrator.  gfx950 only: 64-lane wavefronts, 256-thread workgroups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+
+#include "ceda_sundials.h"
+#include "nvector_hipnative.h"
+
+#define CEDA_HIP_CHECK(expr)                                                   \
+  do {                                                                         \
+    hipError_t e__ = (expr);                                                   \
+    if (e__ != hipSuccess) {                                                   \
+      fprintf(stderr, "HIP error %s at %s:%d: %s\n", hipGetErrorName(e__),     \
+              __FILE__, __LINE__, #expr);                                      \
+      abort();                                                                 \
+    }                                                                          \
+  } while (0)
+
+namespace ceda {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
+constexpr int kWave = 64;            // gfx950 wavefront width
+constexpr int kMaxReduceBlocks = 2048;
+constexpr int kMaxAcc = 64;          // accumulators per reduction (DotProdMulti)
+
+// One per SUNContext: the stream every vector of the context uses and the
+// reduction workspace (device partials, device result, pinned host result).
+struct Exec {
+  hipStream_t stream = nullptr;
+  double* d_partials = nullptr;      // kMaxReduceBlocks * kMaxAcc
+  double* d_result = nullptr;        // kMaxAcc
+  double* h_result = nullptr;        // pinned, kMaxAcc
+  int num_cu = 256;
+};
+
+struct NVContent {
+  sunindextype length = 0;
+  bool own_data = false;
+  double* dev = nullptr;
+  double* host = nullptr;            // pinned mirror, lazily allocated
+  Exec* exec = nullptr;
+};
+
+inline NVContent* content(N_Vector v) { return static_cast<NVContent*>(v->content); }
+inline double* dptr(N_Vector v) { return content(v)->dev; }
+inline sunindextype vlen(N_Vector v) { return content(v)->length; }
+inline hipStream_t vstream(N_Vector v) { return content(v)->exec->stream; }
+
+Exec* context_exec(SUNContext ctx);
+
+// grid for a streaming kernel handling two elements per thread per pass
+inline unsigned stream_grid(int64_t n)
+{
+  const int64_t pairs = (n + 1) / 2;
+  int64_t g = (pairs + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;  // grid-stride beyond this
+  return (unsigned)g;
+}
+
+inline unsigned reduce_grid(int64_t n)
+{
+  int64_t g = (n + 2 * kBlock - 1) / (2 * kBlock);
+  if (g < 1) g = 1;
+  if (g > kMaxReduceBlocks) g = kMaxReduceBlocks;
+  return (unsigned)g;
+}
+
+// raw-pointer helpers used by the integrator's fused STS path; identical
+// arithmetic (and, for the reduction, identical combine order) to the
+// corresponding N_Vector operations
+void raw_linear_sum(hipStream_t s, int64_t n, double a, const double* x, double b,
+                    const double* y, double* z);
+// sum_i (e_i * ewt_i)^2 with e = 0.8 yn - 0.8 ycur + 0.4h fn + 0.4h fs and
+// ewt = 1/(rtol |yn| + atol): the LSRKStep error test without materialising
+// e or ewt (same values, same reduction tree as LinearCombination + WrmsNorm)
+double raw_sts_error_wsqr(Exec* ex, int64_t n, const double* yn, const double* ycur,
+                          const double* fn, const double* fs, double h, double rtol,
+                          double atol);
+
+}  // namespace ceda

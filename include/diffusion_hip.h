@@ -1,0 +1,78 @@
+/* ---------------------------------------------------------------------------
+ * diffusion_2D problem layer on one MI355X.
+ *
+ * DiffusionProblem replaces the reference's UserData (diffusion_2D.hpp:66-211)
+ * for one GPU: the grid, coefficients and the Jacobi workspace live in HBM,
+ * and the one-rank periodic halo exchange (diffusion_2D.cpp:400-584, every
+ * neighbour is the rank itself) is folded into the stencil's index math.
+ *
+ * Entry points keep the reference's names and callback signatures:
+ *   diffusion  — ARKRhsFn, diffusion_2D.cpp:23-35 (diffusion_2D.hpp:250)
+ *   laplacian  — the stencil, diffusion.cpp:9-209 (diffusion_2D.hpp:247)
+ *   PSetup     — ARKLsPrecSetupFn, preconditioner_jacobi.cpp:9-44
+ *   PSolve     — ARKLsPrecSolveFn, preconditioner_jacobi.cpp:47-62
+ *   Initial    — initial.cpp:20-48
+ *   dom_eig    — ARKDomEigFn, main.cpp:536-550 (static there)
+ * Return convention as in the reference: 0 success, -1 unrecoverable (a HIP
+ * launch error or an argument mismatch is reported as -1, never as >0).
+ * Two problems are provided: DIFFUSION_CEDA_PERIODIC (the reference code's
+ * periodic Gaussian problem) and DIFFUSION_MMS_DIRICHLET (the manufactured
+ * problem of diffusion_2D/README.md:11-32, interior nodes only).
+ * ------------------------------------------------------------------------- */
+#ifndef DIFFUSION_HIP_H
+#define DIFFUSION_HIP_H
+
+#include "ceda_sundials.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { DIFFUSION_CEDA_PERIODIC = 0, DIFFUSION_MMS_DIRICHLET = 1 };
+
+typedef struct {
+  int kind;
+  int64_t nx, ny;           /* nodes (periodic) or interior nodes (MMS) */
+  double xl, xu, yl, yu;    /* domain (diffusion_2D.hpp:79-82 defaults) */
+  double kx, ky;            /* diffusion coefficients                    */
+  int inhomogeneous;        /* k(1+0.99 sin s) (diffusion_2D.cpp:887-897) */
+  double tf;                /* final time                                */
+} DiffusionParams;
+
+typedef struct DiffusionProblem DiffusionProblem;
+
+/* reference defaults (diffusion_2D.hpp:66-93) or README.md defaults (MMS) */
+void DiffusionParams_Default(DiffusionParams* p, int kind);
+/* allocates the HBM tables and the Jacobi diagonal; NULL on bad input */
+DiffusionProblem* DiffusionProblem_Create(const DiffusionParams* p, SUNContext ctx);
+void DiffusionProblem_Destroy(DiffusionProblem* prob);
+void DiffusionProblem_GetGrid(const DiffusionProblem* prob, int64_t* nx, int64_t* ny,
+                              double* dx, double* dy);
+long DiffusionProblem_GetNumRhsEvals(const DiffusionProblem* prob);
+/* stencil launch shape: rows per wavefront (1,2,4,8); 0 restores the default */
+int DiffusionProblem_SetStencilRows(DiffusionProblem* prob, int rows);
+
+/* [ext] times `reps` back-to-back launches of one hot-path kernel with HIP
+   events on the problem's stream: which = 0 the RHS stencil (u -> out),
+   which = 1 the fused STS stage (u = Y_{j-1}, a = Y_{j-2}, b = Y_0, c = F_0).
+   Writes the mean launch duration in milliseconds. */
+int DiffusionProblem_TimeKernel(DiffusionProblem* prob, int which, N_Vector u, N_Vector a,
+                                N_Vector b, N_Vector c, N_Vector out, int reps,
+                                double* avg_ms);
+
+int Initial(sunrealtype t, N_Vector u, void* user_data);
+int Exact(sunrealtype t, N_Vector u, void* user_data);
+int laplacian(sunrealtype t, N_Vector u, N_Vector f, void* user_data);
+int diffusion(sunrealtype t, N_Vector u, N_Vector f, void* user_data);
+int PSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok,
+           sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
+int PSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z,
+           sunrealtype gamma, sunrealtype delta, int lr, void* user_data);
+int dom_eig(sunrealtype t, N_Vector y, N_Vector fn, sunrealtype* lambdaR,
+            sunrealtype* lambdaI, void* user_data, N_Vector temp1,
+            N_Vector temp2, N_Vector temp3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -622,7 +622,9 @@ static int pcg(orc_integrator* I, double t, double gamma, const double* b, doubl
   const int64_t n = I->n;
   memset(x, 0, sizeof(double) * (size_t)n);
   vcopy(I->r, b, n);
-  const double bnorm = vwl2(I->r, I->ewt, n);
+  /* relative-tolerance mode measures the unweighted 2-norm */
+  const int rel = (I->ls_reltol > 0.0);
+  const double bnorm = rel ? sqrt(orc_vdot(I->r, I->r, n)) : vwl2(I->r, I->ewt, n);
   const double eplifac = (I->epslin > 0.0) ? I->epslin : 0.05;
   const double delta = (I->ls_reltol > 0.0) ? I->ls_reltol * bnorm
                                             : eplifac * I->nlscoef * sqrt((double)n);
@@ -638,7 +640,7 @@ static int pcg(orc_integrator* I, double t, double gamma, const double* b, doubl
     vlinsum(1.0, x, alpha, I->pv, x, n);
     vlinsum(1.0, I->r, -alpha, I->Ap, I->r, n);
     I->nli++;
-    rnorm = vwl2(I->r, I->ewt, n);
+    rnorm = rel ? sqrt(orc_vdot(I->r, I->r, n)) : vwl2(I->r, I->ewt, n);
     if (rnorm <= delta) return 0;
     psolve(I, I->r, I->zv);
     const double rznew = orc_vdot(I->r, I->zv, n);

@@ -1,0 +1,182 @@
+"""Integrator parity on the GPU: the HIP library (N_Vector path, fused stage
+kernels, fused kernels replayed as HIP graphs) against the CPU oracle on the
+same problem, method, tolerances and controller.  The bar is the final-state
+agreement of the north-star (1e-10 relative, FP64) and identical step
+statistics; the MMS cases also report the L2 error against the exact solution
+for both."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from hiplib import package
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def pk():
+    return package()
+
+
+@pytest.fixture(scope="module")
+def ctx(pk):
+    c = pk.Context()
+    yield c
+    c.close()
+
+
+def run_pair(pk, ctx, kind, method, tout, prob_kw, rtol=1e-5, atol=1e-10, controller=0,
+             fixed=0.0, internal=False, deefreq=25, fast=0, maxsteps=100000, dirk=None,
+             stagemax=200):
+    G = pk.Problem(ctx, kind, **prob_kw)
+    P = O.OracleProblem(kind, **prob_kw)
+    y0 = G.initial()
+    I = pk.Integrator(G, method, y0, ls=(dirk or {}).get("ls", "cg"),
+                      maxl=(dirk or {}).get("maxl", 20))
+    R = O.OracleIntegrator(P, {"rkc": O.RKC, "rkl": O.RKL, "dirk": O.DIRK}[method], P.initial())
+    I.ARKodeSStolerances(rtol, atol)
+    R.set_tolerances(rtol, atol)
+    I.ARKodeSetMaxNumSteps(maxsteps)
+    R.set_max_num_steps(maxsteps)
+    if method == "dirk":
+        d = dirk or {}
+        I.ARKodeSetOrder(d.get("order", 3))
+        R.set_order(d.get("order", 3))
+        if d.get("nonlinear"):
+            I.ARKodeSetNonlinear()
+            R.set_linear(0)
+        I.ARKodeSetLSetupFrequency(d.get("msbp", 0))
+        R.set_lin_solver(d.get("maxl", 20), 0.0, d.get("msbp", 0), 1)
+        if d.get("reltol"):
+            I.ARKodeSetLSRelativeTolerance(d["reltol"])
+            R.set_ls_reltol(d["reltol"])
+    else:
+        if internal:
+            I.internal_eig(100, 0.01)
+            R.set_internal_eig(1, 100, 0.01)
+        I.LSRKStepSetDomEigFrequency(deefreq)
+        R.set_dom_eig_frequency(deefreq)
+        I.LSRKStepSetMaxNumStages(stagemax)
+        R.set_max_num_stages(stagemax)
+        if fast:
+            assert I.LSRKStepSetDiffusionFastPath(G.h, 1 if fast > 1 else 0) == 0
+    if fixed > 0:
+        I.ARKodeSetFixedStep(fixed)
+        R.set_fixed_step(fixed)
+    else:
+        I.controller(controller)
+        R.set_controller(controller)
+    fg, tg = I.evolve(tout)
+    fr, tr, yr = R.evolve(tout)
+    yg = I.y.get()
+    sg, sr = I.stats(), R.stats()
+    out = dict(fg=fg, fr=fr, tg=tg, tr=tr, yg=yg, yr=yr, sg=sg, sr=sr, G=G, P=P)
+    I.free()
+    return out
+
+
+def check(o, fields=("nst", "attempts", "nfe", "netf", "nstlim", "ndomeig", "stage_max")):
+    assert o["fg"] == o["fr"], (o["fg"], o["fr"])
+    assert o["tg"] == o["tr"]
+    for f in fields:
+        assert o["sg"][f] == o["sr"][f], (f, o["sg"], o["sr"])
+    scale = np.abs(o["yr"]).max()
+    err = np.abs(o["yg"] - o["yr"]).max() / scale
+    assert err <= TOL, err
+    return err
+
+
+@pytest.mark.parametrize("method", ["rkc", "rkl"])
+@pytest.mark.parametrize("fast", [0, 1, 2])
+def test_sts_ceda_periodic(pk, ctx, method, fast):
+    o = run_pair(pk, ctx, 0, method, 0.2, dict(nx=48, ny=40), rtol=1e-5, fast=fast)
+    check(o)
+
+
+@pytest.mark.parametrize("method", ["rkc", "rkl"])
+@pytest.mark.parametrize("fast", [0, 2])
+def test_sts_ceda_inhomogeneous_internal_eig(pk, ctx, method, fast):
+    o = run_pair(pk, ctx, 0, method, 0.3, dict(nx=40, ny=36, inhomogeneous=1), rtol=1e-4,
+                 atol=1e-11, controller=2, internal=True, fast=fast)
+    check(o, fields=("nst", "attempts", "nfe", "netf", "ndomeig", "ndee_iters", "stage_max"))
+
+
+@pytest.mark.parametrize("method", ["rkc", "rkl"])
+@pytest.mark.parametrize("fast", [0, 1, 2])
+def test_sts_mms_config1(pk, ctx, method, fast):
+    """The reference's CPU-runnable configuration: MMS on a 128x128 interior
+    grid, kx=ky=1, to t=1, rtol 1e-5, atol 1e-10."""
+    o = run_pair(pk, ctx, 1, method, 1.0, dict(nx=128, ny=128), rtol=1e-5, atol=1e-10, fast=fast)
+    check(o)
+    ue = o["P"].exact(1.0)
+    eg = np.sqrt(np.mean((o["yg"] - ue) ** 2))
+    er = np.sqrt(np.mean((o["yr"] - ue) ** 2))
+    assert abs(eg - er) <= 1e-9 * er
+    assert er < 1e-3
+
+
+@pytest.mark.parametrize("fast", [0, 2])
+def test_sts_fixed_step(pk, ctx, fast):
+    o = run_pair(pk, ctx, 1, "rkl", 0.05, dict(nx=64, ny=50, kx=1.0, ky=0.5), fixed=1e-3, fast=fast)
+    check(o, fields=("nst", "nfe"))
+
+
+@pytest.mark.parametrize("controller", [1, 2, 3, 4, 5])
+def test_sts_controllers(pk, ctx, controller):
+    o = run_pair(pk, ctx, 1, "rkc", 0.3, dict(nx=32, ny=32), rtol=1e-6, controller=controller)
+    check(o)
+
+
+def test_sts_stage_limit_retries(pk, ctx):
+    o = run_pair(pk, ctx, 1, "rkl", 0.1, dict(nx=128, ny=128), rtol=1e-3, stagemax=12, fast=2)
+    check(o)
+    assert o["sg"]["nstlim"] > 0
+
+
+def test_full_size_config2_steps(pk, ctx):
+    """4096x4096 interior MMS, kx=1, ky=0.5, RKL2, power-iteration spectral
+    radius every 25 steps, rtol 1e-6: the first steps of the benchmark
+    configuration, GPU fused+graph path against the oracle."""
+    o = run_pair(pk, ctx, 1, "rkl", 1.0, dict(nx=4096, ny=4096, kx=1.0, ky=0.5), rtol=1e-6,
+                 internal=True, fast=2, maxsteps=2)
+    assert o["fg"] == -1 and o["fr"] == -1  # stopped by the step cap, as requested
+    check(o, fields=("nst", "attempts", "nfe", "ndomeig", "ndee_iters", "stage_max"))
+    assert o["sg"]["stage_max"] > 100
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("nonlinear", [False, True])
+def test_dirk_pcg_jacobi(pk, ctx, order, nonlinear):
+    o = run_pair(pk, ctx, 0, "dirk", 0.2, dict(nx=32, ny=28, inhomogeneous=1), rtol=1e-5,
+                 atol=1e-11, controller=2,
+                 dirk=dict(order=order, nonlinear=nonlinear, msbp=1, maxl=20))
+    check(o, fields=("nst", "attempts", "netf", "nni", "nli", "npsetup", "npsolve", "ncfn"))
+
+
+def test_dirk_fixed_step_reltol(pk, ctx):
+    o = run_pair(pk, ctx, 1, "dirk", 0.01, dict(nx=96, ny=96), fixed=1e-3,
+                 dirk=dict(order=3, maxl=2000, reltol=1e-8))
+    check(o, fields=("nst", "nni", "nli"))
+
+
+def test_rhs_failure_is_fatal(pk, ctx):
+    """A negative ARKRhsFn return aborts the solve with ARK_RHSFUNC_FAIL, as a
+    -1 from the reference's diffusion() does (diffusion_2D.cpp:32)."""
+    import ctypes as C
+    G = pk.Problem(ctx, pk.CEDA_PERIODIC, nx=16, ny=16)
+    y0 = G.initial()
+
+    @pk.RHSFN
+    def bad(t, y, yd, ud):
+        return -1
+
+    L = pk.lib()
+    h = L.LSRKStepCreateSTS(C.cast(bad, C.c_void_p), 0.0, y0.h, ctx.h)
+    L.LSRKStepSetDomEigFn(h, pk.fn_ptr("dom_eig"))
+    L.ARKodeSetUserData(h, G.h)
+    t = C.c_double()
+    assert L.ARKodeEvolve(h, 0.1, y0.h, C.byref(t), 1) == -8
+    hv = C.c_void_p(h)
+    L.ARKodeFree(C.byref(hv))
