@@ -11,6 +11,7 @@ main.cpp:536) and the SUNDIALS API names the reference's main.cpp calls.
 import ctypes as C
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -172,14 +173,27 @@ def header_functions():
 
 
 class Context:
+    """SUNContext.  Objects created from it register here; close() destroys
+    the live ones first (vectors, problems and integrators must be freed
+    before their context, as in SUNDIALS)."""
+
     def __init__(self):
         h = vp()
         if lib().SUNContext_Create(0, C.byref(h)) != 0:
             raise RuntimeError("SUNContext_Create failed")
         self.h = h
+        self.children = weakref.WeakSet()
+
+    def register(self, obj):
+        self.children.add(obj)
 
     def close(self):
         if self.h:
+            kids = list(self.children)
+            for kind in (Integrator, Vector, Problem):
+                for k in kids:
+                    if isinstance(k, kind):
+                        k.release()
             lib().SUNContext_Free(C.byref(self.h))
             self.h = None
 
@@ -200,6 +214,7 @@ class Vector:
             raise RuntimeError("N_VNew_HipNative failed")
         self.n = lib().N_VGetLength(self.h)
         self.owner = owner
+        ctx.register(self)
 
     @classmethod
     def from_numpy(cls, ctx, a):
@@ -223,9 +238,11 @@ class Vector:
         return Vector(self.ctx, handle=lib().N_VClone(self.h))
 
     def destroy(self):
-        if self.h and self.owner:
+        if self.h and self.owner and self.ctx.h:
             lib().N_VDestroy(self.h)
         self.h = None
+
+    release = destroy
 
     def __del__(self):
         try:
@@ -259,6 +276,7 @@ class Problem:
         lib().DiffusionProblem_GetGrid(self.h, C.byref(nx), C.byref(ny), C.byref(dx), C.byref(dy))
         self.nx, self.ny, self.dx, self.dy = nx.value, ny.value, dx.value, dy.value
         self.n = self.nx * self.ny
+        ctx.register(self)
 
     def vector(self, a=None):
         v = Vector(self.ctx, n=self.n)
@@ -298,9 +316,11 @@ class Problem:
         return lib().DiffusionProblem_SetStencilRows(self.h, rows)
 
     def destroy(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().DiffusionProblem_Destroy(self.h)
         self.h = None
+
+    release = destroy
 
     def __del__(self):
         try:
@@ -340,6 +360,7 @@ class Integrator:
             raise RuntimeError("integrator creation failed")
         L.ARKodeSetUserData(self.h, prob.h)
         self.y = y0.clone()
+        prob.ctx.register(self)
 
     def __getattr__(self, name):
         f = getattr(lib(), name)
@@ -369,8 +390,20 @@ class Integrator:
         lib().LSRKStepGetSpectralRadius(self.h, C.byref(r))
         return r.value
 
+    def release(self):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
     def free(self):
         L = lib()
+        if not self.prob.ctx.h:
+            self.h = self.LS = self.dee = self.ctrl = None
+            return
         if self.h:
             h = vp(self.h)
             L.ARKodeFree(C.byref(h))

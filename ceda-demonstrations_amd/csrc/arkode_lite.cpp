@@ -646,7 +646,9 @@ int lsrk_takestep(ArkMem* A, double* dsm)
   ss = fmax(ss, 2.0);
   if (ss >= (double)A->smax) {
     if (A->fixedstep) return ARK_MAX_STAGE_LIMIT_FAIL;
-    const double sm = (double)A->smax;
+    // largest step whose stage count is smax-1 (a step sized for smax
+    // stages can round back up to smax and never make progress)
+    const double sm = (double)A->smax - 1.0;
     const double hmax = (A->method == M_RKC) ? A->safety * (sm * sm) / (1.54 * A->rho)
                                              : A->safety * (sm * sm + sm - 2.0) / (2.0 * A->rho);
     A->eta = hmax / h;

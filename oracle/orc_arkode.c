@@ -477,7 +477,9 @@ static int lsrk_takestep(orc_integrator* I, double* dsm)
   ss = fmax(ss, 2.0);
   if (ss >= (double)I->smax) {
     if (I->fixedstep) return ORC_MAX_STAGE_LIMIT_FAIL;
-    const double sm = (double)I->smax;
+    /* largest step whose stage count is smax-1 (a step sized for smax
+       stages can round back up to smax and never make progress) */
+    const double sm = (double)I->smax - 1.0;
     const double hmax = (I->method == ORC_RKC)
                           ? I->safety * (sm * sm) / (1.54 * I->rho)
                           : I->safety * (sm * sm + sm - 2.0) / (2.0 * I->rho);

@@ -214,6 +214,46 @@ def test_adaptive_mms_matches_dst_exact_solution():
             assert err < 5e-6, (method, t, err)
 
 
+@pytest.mark.parametrize("ctrl,chaotic", [(0, False), (1, False), (2, False), (3, True),
+                                          (4, False), (5, True)])
+def test_gustafsson_controllers_are_ulp_sensitive(ctrl, chaotic):
+    """Perturb rtol by one ulp: the Soderlind controllers and ImpGus keep the
+    step sequence (state moves at round-off level); ExpGus and ImExGus change
+    it.  This fixes which controllers can be compared step for step."""
+    runs = []
+    for rt in (1e-6, np.nextafter(1e-6, 1.0)):
+        P = O.OracleProblem(O.MMS, nx=32, ny=32)
+        I = O.OracleIntegrator(P, O.RKC, P.initial())
+        I.set_tolerances(rt, 1e-10)
+        I.set_max_num_steps(100000)
+        I.set_controller(ctrl)
+        flag, t, y = I.evolve(0.3)
+        assert flag == 0
+        runs.append((I.stats()["nfe"], y))
+    d = np.abs(runs[0][1] - runs[1][1]).max()
+    if chaotic:
+        assert runs[0][0] != runs[1][0] or d > 1e-12
+    else:
+        assert runs[0][0] == runs[1][0] and d < 1e-12
+
+
+@pytest.mark.parametrize("method", [O.RKC, O.RKL])
+@pytest.mark.parametrize("smax", [3, 12, 50])
+def test_stage_limit_retry_makes_progress(method, smax):
+    """A stage-limited step is retried with the largest h whose stage count is
+    smax-1, so small stage caps still advance (a cap-sized retry step can round
+    back up to smax)."""
+    P = O.OracleProblem(O.MMS, nx=64, ny=64)
+    I = O.OracleIntegrator(P, method, P.initial())
+    I.set_tolerances(1e-3, 1e-10)
+    I.set_max_num_steps(100000)
+    I.set_max_num_stages(smax)
+    flag, t, y = I.evolve(0.05)
+    st = I.stats()
+    assert flag == 0 and t == 0.05
+    assert st["nstlim"] > 0 and st["stage_max"] <= smax - 1
+
+
 # ------------------------------------------------------- power iteration
 def test_power_iteration_spectral_radius():
     nx, ny, kx, ky = 32, 24, 1.0, 0.5
