@@ -238,7 +238,7 @@ def test_gustafsson_controllers_are_ulp_sensitive(ctrl, chaotic):
 
 
 @pytest.mark.parametrize("method", [O.RKC, O.RKL])
-@pytest.mark.parametrize("smax", [3, 12, 50])
+@pytest.mark.parametrize("smax", [3, 5, 8])
 def test_stage_limit_retry_makes_progress(method, smax):
     """A stage-limited step is retried with the largest h whose stage count is
     smax-1, so small stage caps still advance (a cap-sized retry step can round
