@@ -91,18 +91,21 @@ int orc_problem_setup(orc_problem* p)
     p->py[i] = coeff(p->ky, p->inhomogeneous, (double)i * p->dy) / dy2;
   }
   if (p->kind == ORC_MMS_DIRICHLET) {
+    /* cos^2 - sin^2 evaluated as cos(2 pi x): no sin/cos pair of one
+       argument, which a compiler may fuse into sincos() (it rounds
+       differently in rare cases, breaking bitwise reproducibility) */
     const double tpp = 2.0 * ORC_PI * ORC_PI;
     for (i = 0; i < p->nx; i++) {
       const double x = p->xl + (double)(i + 1) * p->dx;
-      const double s = sin(ORC_PI * x), c = cos(ORC_PI * x);
+      const double s = sin(ORC_PI * x);
       p->sx[i] = s * s;
-      p->cx[i] = p->kx * (-tpp * (c * c - s * s));
+      p->cx[i] = p->kx * (-tpp * cos(2.0 * ORC_PI * x));
     }
     for (i = 0; i < p->ny; i++) {
       const double y = p->yl + (double)(i + 1) * p->dy;
-      const double s = sin(ORC_PI * y), c = cos(ORC_PI * y);
+      const double s = sin(ORC_PI * y);
       p->sy[i] = s * s;
-      p->cy[i] = p->ky * (-tpp * (c * c - s * s));
+      p->cy[i] = p->ky * (-tpp * cos(2.0 * ORC_PI * y));
     }
   }
   for (i = 0; i < p->n; i++) p->diag[i] = 1.0;
@@ -147,12 +150,12 @@ void orc_exact(const orc_problem* p, double t, double* u)
 }
 
 /* time factors of the README.md:28 forcing:
-   b = T1 sx sy + T2 (kx Cx sy + ky Cy sx), T1 = -2 pi sin(pi t) cos(pi t),
-   T2 = cos^2(pi t) */
+   b = T1 sx sy + T2 (kx Cx sy + ky Cy sx), T1 = -2 pi sin(pi t) cos(pi t)
+   = -pi sin(2 pi t), T2 = cos^2(pi t) (no sin/cos pair of one argument) */
 void orc_mms_time(double t, double* T1, double* T2)
 {
-  const double s = sin(ORC_PI * t), c = cos(ORC_PI * t);
-  *T1 = -2.0 * ORC_PI * s * c;
+  const double c = cos(ORC_PI * t);
+  *T1 = -ORC_PI * sin(2.0 * ORC_PI * t);
   *T2 = c * c;
 }
 
