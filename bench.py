@@ -88,6 +88,36 @@ def roofline(pk, ctx, G):
             "rhs_stencil_GBs": round(stencil_gbs, 1)}
 
 
+def kernels_only(pk, reps):
+    """Launch the hot-path kernels alone at the workload's size (the workload
+    of the rocprofv3 counter runs): the fused STS stage, the RHS stencil for
+    each launch shape, and a plain copy (N_VScale by 1: 8 B read + 8 B written
+    per element) whose counters calibrate FETCH_SIZE/WRITE_SIZE."""
+    ctx = pk.Context()
+    G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
+    L = pk.lib()
+    vs = [G.vector() for _ in range(5)]
+    for k, v in enumerate(vs):
+        L.N_VConst(1.0 + 0.1 * k, v.h)
+    res = {}
+    avg = C.c_double()
+    for rows in (1, 2, 4, 8):
+        G.set_rows(rows)
+        L.DiffusionProblem_TimeKernel(G.h, 1, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, reps,
+                                      C.byref(avg))
+        res[f"fused_rows{rows}_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+        L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
+        res[f"rhs_rows{rows}_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+    G.set_rows(0)
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        L.N_VScale(1.0, vs[0].h, vs[4].h)
+    hip_sync()
+    res["copy_GBs"] = round(16.0 * G.n * reps / (time.perf_counter() - t0) / 1e9, 1)
+    print(json.dumps(res))
+
+
 def cpu_baseline():
     """One adaptive RKL2 step of the same workload on the CPU oracle (serial)."""
     import oracle_lib as O
@@ -114,6 +144,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--fastpath", type=int, default=2, help="0 API path, 1 fused, 2 fused+graphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
+                    help="only launch the hot-path kernels REPS times each (profiling workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one MI355X); "
@@ -122,6 +154,9 @@ def main():
     from hiplib import package
 
     pk = package()
+    if args.kernels_only:
+        kernels_only(pk, args.kernels_only)
+        return
     ctx = pk.Context()
     G, I = make_integrator(pk, ctx, args.fastpath)
     log(f"workload ready: {G.nx}x{G.ny} cells, warmup {args.warmup} steps")
