@@ -81,7 +81,10 @@ def roofline(pk, ctx, G):
     stencil_gbs = 16.0 * G.n / (avg.value * 1e-3) / 1e9
     for v in vs:
         v.destroy()
-    return {"kernel": "k_stencil<2,false,true,true> (fused STS stage)", "bound": "hbm",
+    v, r, nt = G.get_config(1)
+    kname = (f"k_stencil<2,false,true,true> rows={r}" if v == 0 else
+             f"k_march<2,false,true,true,{'true' if nt else 'false'}> rows={r}")
+    return {"kernel": kname + " (fused STS stage)", "bound": "hbm",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
             "bytes_per_launch_algorithmic": bytes_alg, "launch_ms": round(avg_s * 1e3, 4),
@@ -101,14 +104,18 @@ def kernels_only(pk, reps):
         L.N_VConst(1.0 + 0.1 * k, v.h)
     res = {}
     avg = C.c_double()
-    for rows in (1, 2, 4, 8):
-        G.set_rows(rows)
+    cands = [(0, r, 0) for r in (1, 2, 4, 8)] + [(1, r, nt) for r in (4, 8, 16, 32, 64) for nt in (0, 1)]
+    for (v, r, nt) in cands:
+        G.set_config(0, v, r, nt)
+        G.set_config(1, v, r, nt)
         L.DiffusionProblem_TimeKernel(G.h, 1, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, reps,
                                       C.byref(avg))
-        res[f"fused_rows{rows}_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+        res[f"fused_v{v}_r{r}_nt{nt}_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
         L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
-        res[f"rhs_rows{rows}_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
-    G.set_rows(0)
+        res[f"rhs_v{v}_r{r}_nt{nt}_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+    G.autotune(3)
+    res["autotuned_rhs"] = G.get_config(0)
+    res["autotuned_fused"] = G.get_config(1)
     hip_sync()
     t0 = time.perf_counter()
     for _ in range(reps):

@@ -82,6 +82,9 @@ _SIGS = {
     "DiffusionProblem_GetNumRhsEvals": (C.c_long, [vp]),
     "DiffusionProblem_SetStencilRows": (i32, [vp, i32]),
     "DiffusionProblem_TimeKernel": (i32, [vp, i32, vp, vp, vp, vp, vp, i32, dp]),
+    "DiffusionProblem_SetStencilConfig": (i32, [vp, i32, i32, i32, i32]),
+    "DiffusionProblem_GetStencilConfig": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+    "DiffusionProblem_Autotune": (i32, [vp, i32]),
     "Initial": (i32, [d, vp, vp]),
     "Exact": (i32, [d, vp, vp]),
     "laplacian": (i32, [d, vp, vp, vp]),
@@ -314,6 +317,17 @@ class Problem:
 
     def set_rows(self, rows):
         return lib().DiffusionProblem_SetStencilRows(self.h, rows)
+
+    def set_config(self, which, variant, rows, nt=0):
+        return lib().DiffusionProblem_SetStencilConfig(self.h, which, variant, rows, nt)
+
+    def get_config(self, which):
+        v, r, n = i32(), i32(), i32()
+        lib().DiffusionProblem_GetStencilConfig(self.h, which, C.byref(v), C.byref(r), C.byref(n))
+        return v.value, r.value, n.value
+
+    def autotune(self, reps=3):
+        return lib().DiffusionProblem_Autotune(self.h, reps)
 
     def destroy(self):
         if self.h and self.ctx.h:

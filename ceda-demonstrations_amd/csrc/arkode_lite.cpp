@@ -1131,6 +1131,7 @@ int LSRKStepSetDiffusionFastPath(void* m, DiffusionProblem* prob, int use_graphs
   if (prob->n != A->n || A->yn->ops->nvclone != N_VClone_HipNative) return ARK_ILL_INPUT;
   A->fast = prob;
   A->use_graphs = use_graphs ? 1 : 0;
+  if (!prob->tuned) DiffusionProblem_Autotune(prob, 3);
   if (!A->tab_ev) {
     CEDA_HIP_CHECK(hipEventCreateWithFlags(&A->tab_ev, hipEventDisableTiming));
     CEDA_HIP_CHECK(hipEventRecord(A->tab_ev, prob->exec->stream));

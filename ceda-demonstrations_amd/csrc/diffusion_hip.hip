@@ -24,6 +24,8 @@
 // bitwise identical to a serial CPU evaluation.
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "diffusion_internal.hpp"
 
@@ -166,10 +168,191 @@ __global__ __launch_bounds__(kBlock) void k_stencil(const KArgs a)
   }
 }
 
-template <int CPL, bool PER, bool MMS, bool FUSED>
-void launch_t(const KArgs& a, unsigned grid, hipStream_t s)
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ inline double2 ldv2(const double* p)
 {
-  hipLaunchKernelGGL((k_stencil<CPL, PER, MMS, FUSED>), dim3(grid), dim3(kBlock), 0, s, a);
+  if (NT) {
+    const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    return make_double2(v.x, v.y);
+  }
+  return *reinterpret_cast<const double2*>(p);
+}
+template <bool NT>
+__device__ inline void stv2(double* p, double a, double b)
+{
+  if (NT) {
+    d2v w;
+    w.x = a;
+    w.y = b;
+    __builtin_nontemporal_store(w, reinterpret_cast<d2v*>(p));
+  } else *reinterpret_cast<double2*>(p) = make_double2(a, b);
+}
+template <bool NT>
+__device__ inline double lds1(const double* p)
+{
+  return NT ? __builtin_nontemporal_load(p) : *p;
+}
+template <bool NT>
+__device__ inline void sts1(double* p, double v)
+{
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// y-marching variant: a workgroup owns 256*CPL columns x `rows` consecutive
+// rows; each lane walks down its columns keeping the south/centre/north rows
+// in registers (one new row load per output row), gets west/east neighbours
+// from the adjacent lanes by wavefront shuffles (only lanes 0 and 63 load
+// them), and prefetches the next row's operands before computing the current
+// one.  Same arithmetic, same association order as k_stencil.
+template <int CPL, bool PER, bool MMS, bool FUSED, bool NT>
+__global__ __launch_bounds__(kBlock) void k_march(const KArgs a)
+{
+  const int lane = threadIdx.x & (kWave - 1);
+  const unsigned B = gridDim.x, b = blockIdx.x;
+  const unsigned q = B >> 3, r = B & 7u, xcd = b & 7u, loc = b >> 3;
+  const unsigned tile = xcd * q + (xcd < r ? xcd : r) + loc;
+  const int tx = (int)(tile % (unsigned)a.tiles_x), ty = (int)(tile / (unsigned)a.tiles_x);
+  const int64_t nx = a.nx, ny = a.ny;
+  const int64_t i = (int64_t)tx * (kBlock * CPL) + (int64_t)threadIdx.x * CPL;
+  const bool act = i < nx;
+
+  double T1 = a.T1, T2 = a.T2;
+  double c[5] = {0, 0, 0, 0, 0};
+  const double* __restrict__ u = a.u;
+  double* __restrict__ fout = a.f;
+  const double* yjm2 = a.yjm2;
+  const double* yn = a.yn;
+  const double* fn = a.fn;
+  if (a.ent) {
+    u = a.ent->yjm1;
+    fout = a.ent->out;
+    yjm2 = a.ent->yjm2;
+    yn = a.ent->yn;
+    fn = a.ent->fn;
+    T1 = a.ent->T1;
+    T2 = a.ent->T2;
+    if (FUSED) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) c[k] = a.ent->c[k];
+    }
+  } else if (a.coef) {
+    T1 = a.coef[5];
+    T2 = a.coef[6];
+    if (FUSED) {
+#pragma unroll
+      for (int k = 0; k < 5; k++) c[k] = a.coef[k];
+    }
+  }
+  const int64_t ic = act ? i : 0;  // clamp for the table reads of idle lanes
+  const double Dw0 = a.fx[ic], De0 = a.fx[ic + 1];
+  const double De1 = (CPL == 2) ? a.fx[ic + 2] : 0.0;
+  const int64_t iw = (i > 0) ? i - 1 : (PER ? nx - 1 : -1);
+  const int64_t ie = (i + CPL < nx) ? i + CPL : (PER ? 0 : -1);
+  const bool need_w = (lane == 0);
+  const bool need_e = (lane == kWave - 1) || (i + CPL >= nx);
+  double sx0 = 0, sx1 = 0, cx0 = 0, cx1 = 0;
+  if (MMS) {
+    sx0 = a.sx[ic];
+    cx0 = a.cx[ic];
+    if (CPL == 2) {
+      sx1 = a.sx[ic + 1];
+      cx1 = a.cx[ic + 1];
+    }
+  }
+  const int64_t j0 = (int64_t)ty * a.rows;
+  const int64_t j1 = (j0 + a.rows < ny) ? j0 + a.rows : ny;
+
+  // row loader: row index wrapped (periodic) or zero outside (Dirichlet)
+  auto rowld = [&](int64_t rr, double& v0, double& v1) {
+    if (rr < 0) rr = PER ? ny - 1 : -1;
+    else if (rr >= ny) rr = PER ? 0 : -1;
+    v0 = 0.0;
+    v1 = 0.0;
+    if (act && rr >= 0) {
+      if (CPL == 2) {
+        const double2 t = *reinterpret_cast<const double2*>(u + rr * nx + i);
+        v0 = t.x;
+        v1 = t.y;
+      } else v0 = u[rr * nx + i];
+    }
+  };
+  auto edges = [&](int64_t rr, double& w, double& e) {
+    w = (act && need_w && iw >= 0) ? u[rr * nx + iw] : 0.0;
+    e = (act && need_e && ie >= 0) ? u[rr * nx + ie] : 0.0;
+  };
+  auto streams = [&](int64_t rr, double* A, double* Y, double* G) {
+    if (!FUSED || !act) return;
+    const int64_t o = rr * nx + i;
+    if (CPL == 2) {
+      const double2 x = ldv2<NT>(yjm2 + o), y = ldv2<NT>(yn + o), g = ldv2<NT>(fn + o);
+      A[0] = x.x; A[1] = x.y; Y[0] = y.x; Y[1] = y.y; G[0] = g.x; G[1] = g.y;
+    } else {
+      A[0] = lds1<NT>(yjm2 + o); Y[0] = lds1<NT>(yn + o); G[0] = lds1<NT>(fn + o);
+    }
+  };
+
+  double S0, S1, C0, C1, N0, N1, we, ee;
+  double A[2] = {0, 0}, Y[2] = {0, 0}, G[2] = {0, 0};
+  rowld(j0 - 1, S0, S1);
+  rowld(j0, C0, C1);
+  rowld(j0 + 1, N0, N1);
+  edges(j0, we, ee);
+  streams(j0, A, Y, G);
+  for (int64_t j = j0; j < j1; j++) {
+    // prefetch the next output row's operands
+    double M0 = 0, M1 = 0, we2 = 0, ee2 = 0;
+    double A2[2] = {0, 0}, Y2[2] = {0, 0}, G2[2] = {0, 0};
+    const bool more = (j + 1 < j1);
+    if (more) {
+      rowld(j + 2, M0, M1);
+      edges(j + 1, we2, ee2);
+      streams(j + 1, A2, Y2, G2);
+    }
+    double wv = __shfl_up(CPL == 2 ? C1 : C0, 1, kWave);
+    double ev = __shfl_down(C0, 1, kWave);
+    if (need_w) wv = we;
+    if (need_e) ev = ee;
+    if (act) {
+      const double Ds = a.fy[j], Dn = a.fy[j + 1];
+      const int64_t o = j * nx + i;
+      if (CPL == 2) {
+        double f0 = cell(Dw0, De0, Ds, Dn, C0, wv, C1, S0, N0);
+        double f1 = cell(De0, De1, Ds, Dn, C1, C0, ev, S1, N1);
+        if (MMS) {
+          const double syj = a.sy[j], cyj = a.cy[j];
+          f0 = f0 + forcing(T1, T2, sx0, cx0, syj, cyj);
+          f1 = f1 + forcing(T1, T2, sx1, cx1, syj, cyj);
+        }
+        if (FUSED) {
+          f0 = lc5(c, f0, A[0], Y[0], C0, G[0]);
+          f1 = lc5(c, f1, A[1], Y[1], C1, G[1]);
+        }
+        stv2<NT>(fout + o, f0, f1);
+      } else {
+        double f0 = cell(Dw0, De0, Ds, Dn, C0, wv, ev, S0, N0);
+        if (MMS) f0 = f0 + forcing(T1, T2, sx0, cx0, a.sy[j], a.cy[j]);
+        if (FUSED) f0 = lc5(c, f0, A[0], Y[0], C0, G[0]);
+        sts1<NT>(fout + o, f0);
+      }
+    }
+    S0 = C0; S1 = C1;
+    C0 = N0; C1 = N1;
+    N0 = M0; N1 = M1;
+    we = we2; ee = ee2;
+    A[0] = A2[0]; A[1] = A2[1]; Y[0] = Y2[0]; Y[1] = Y2[1]; G[0] = G2[0]; G[1] = G2[1];
+  }
+}
+
+template <int CPL, bool PER, bool MMS, bool FUSED>
+void launch_t(const KArgs& a, const StencilCfg& cfg, unsigned grid, hipStream_t s)
+{
+  if (cfg.variant == 0) hipLaunchKernelGGL((k_stencil<CPL, PER, MMS, FUSED>), dim3(grid), dim3(kBlock), 0, s, a);
+  else if (cfg.nt) hipLaunchKernelGGL((k_march<CPL, PER, MMS, FUSED, true>), dim3(grid), dim3(kBlock), 0, s, a);
+  else hipLaunchKernelGGL((k_march<CPL, PER, MMS, FUSED, false>), dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
@@ -181,7 +364,8 @@ void launch_stencil(DiffusionProblem* P, KArgs a, bool fused, hipStream_t s)
             (a.ent || (aligned16(a.u) && aligned16(a.f) &&
                        (!fused || (aligned16(a.yjm2) && aligned16(a.yn) && aligned16(a.fn)))));
   const int cpl = al ? 2 : 1;
-  const int rows = P->rows > 0 ? P->rows : 1;
+  const StencilCfg& cfg = P->cfg[fused ? 1 : 0];
+  const int rows = cfg.rows > 0 ? cfg.rows : 1;
   a.fx = P->d_fx;
   a.fy = P->d_fy;
   a.sx = P->d_sx;
@@ -191,14 +375,19 @@ void launch_stencil(DiffusionProblem* P, KArgs a, bool fused, hipStream_t s)
   a.nx = P->p.nx;
   a.ny = P->p.ny;
   a.rows = rows;
-  a.tiles_x = (int)((P->p.nx + kWave * cpl - 1) / (kWave * cpl));
-  const int64_t tiles_y = (P->p.ny + 4 * rows - 1) / (4 * rows);
+  int64_t tiles_y;
+  if (cfg.variant == 0) {
+    a.tiles_x = (int)((P->p.nx + kWave * cpl - 1) / (kWave * cpl));
+    tiles_y = (P->p.ny + 4 * rows - 1) / (4 * rows);
+  } else {
+    a.tiles_x = (int)((P->p.nx + kBlock * cpl - 1) / (kBlock * cpl));
+    tiles_y = (P->p.ny + rows - 1) / rows;
+  }
   const unsigned grid = (unsigned)(a.tiles_x * tiles_y);
   const bool per = (P->p.kind == DIFFUSION_CEDA_PERIODIC);
-  const bool mms = !per;
 #define CEDA_DISPATCH(C, PE, M)                                     \
-  if (fused) launch_t<C, PE, M, true>(a, grid, s);                  \
-  else launch_t<C, PE, M, false>(a, grid, s);
+  if (fused) launch_t<C, PE, M, true>(a, cfg, grid, s);             \
+  else launch_t<C, PE, M, false>(a, cfg, grid, s);
   if (cpl == 2) {
     if (per) { CEDA_DISPATCH(2, true, false) }
     else { CEDA_DISPATCH(2, false, true) }
@@ -207,7 +396,6 @@ void launch_stencil(DiffusionProblem* P, KArgs a, bool fused, hipStream_t s)
     else { CEDA_DISPATCH(1, false, true) }
   }
 #undef CEDA_DISPATCH
-  (void)mms;
 }
 
 // first STS stage from the table: out = c0*yn + c1*fn (N_VLinearSum order)
@@ -268,6 +456,18 @@ double* upload(const std::vector<double>& h)
     CEDA_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   return d;
 }
+
+// autotuner results, per (nx, ny, kind) for the life of the process
+struct TuneKey {
+  int64_t nx, ny;
+  int kind;
+  bool operator<(const TuneKey& o) const
+  {
+    return nx != o.nx ? nx < o.nx : (ny != o.ny ? ny < o.ny : kind < o.kind);
+  }
+};
+std::map<TuneKey, std::pair<StencilCfg, StencilCfg>> g_tune;
+std::mutex g_tune_mu;
 
 bool vec_ok(const DiffusionProblem* P, N_Vector v)
 {
@@ -421,6 +621,8 @@ DiffusionProblem* DiffusionProblem_Create(const DiffusionParams* prm, SUNContext
   CEDA_HIP_CHECK(hipHostMalloc(&P->h_tf, sizeof(double) * 8, hipHostMallocDefault));
   P->diag = N_VNew_HipNative(P->n, ctx);
   N_VConst(1.0, P->diag);
+  P->cfg[0] = StencilCfg{0, 8, 0};
+  P->cfg[1] = StencilCfg{0, 2, 0};
   return P;
 }
 
@@ -451,7 +653,98 @@ long DiffusionProblem_GetNumRhsEvals(const DiffusionProblem* P) { return P->nrhs
 int DiffusionProblem_SetStencilRows(DiffusionProblem* P, int rows)
 {
   if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8) return -1;
-  P->rows = rows;
+  for (int k = 0; k < 2; k++) {
+    P->cfg[k].variant = 0;
+    P->cfg[k].rows = rows ? rows : 1;
+    P->cfg[k].nt = 0;
+  }
+  return 0;
+}
+
+int DiffusionProblem_SetStencilConfig(DiffusionProblem* P, int which, int variant, int rows, int nt)
+{
+  if (!P || which < 0 || which > 1 || variant < 0 || variant > 1 || rows < 1 || rows > 256 ||
+      (nt != 0 && nt != 1) || (variant == 0 && nt))
+    return -1;
+  P->cfg[which].variant = variant;
+  P->cfg[which].rows = rows;
+  P->cfg[which].nt = nt;
+  return 0;
+}
+
+int DiffusionProblem_GetStencilConfig(const DiffusionProblem* P, int which, int* variant, int* rows,
+                                      int* nt)
+{
+  if (!P || which < 0 || which > 1) return -1;
+  if (variant) *variant = P->cfg[which].variant;
+  if (rows) *rows = P->cfg[which].rows;
+  if (nt) *nt = P->cfg[which].nt;
+  return 0;
+}
+
+int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
+{
+  if (!P || reps < 1) return -1;
+  const TuneKey key{P->p.nx, P->p.ny, P->p.kind};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      P->cfg[0] = it->second.first;
+      P->cfg[1] = it->second.second;
+      P->tuned = true;
+      return 0;
+    }
+  }
+  hipStream_t s = P->exec->stream;
+  double* buf[5];
+  for (int k = 0; k < 5; k++) {
+    CEDA_HIP_CHECK(hipMalloc(&buf[k], sizeof(double) * P->n));
+    CEDA_HIP_CHECK(hipMemsetAsync(buf[k], 0, sizeof(double) * P->n, s));
+  }
+  double* coef = nullptr;
+  const double hc[8] = {1.0e-9, -0.5, 0.25, 1.25, -1.0e-10, 0.0, 1.0, 0.0};
+  CEDA_HIP_CHECK(hipMalloc(&coef, sizeof(hc)));
+  CEDA_HIP_CHECK(hipMemcpyAsync(coef, hc, sizeof(hc), hipMemcpyHostToDevice, s));
+  hipEvent_t e0, e1;
+  CEDA_HIP_CHECK(hipEventCreate(&e0));
+  CEDA_HIP_CHECK(hipEventCreate(&e1));
+  std::vector<StencilCfg> cands;
+  for (int rows : {1, 2, 4, 8}) cands.push_back(StencilCfg{0, rows, 0});
+  for (int rows : {4, 8, 16, 32, 64})
+    for (int nt : {0, 1}) cands.push_back(StencilCfg{1, rows, nt});
+  for (int which = 0; which < 2; which++) {
+    float best = 1e30f;
+    StencilCfg bestc = P->cfg[which];
+    for (const StencilCfg& c : cands) {
+      if (c.variant == 1 && c.rows > P->p.ny) continue;
+      P->cfg[which] = c;
+      auto launch = [&]() {
+        if (which == 0) launch_rhs_devtime(P, coef, buf[0], buf[4], s);
+        else launch_fused_stage(P, coef, buf[0], buf[1], buf[2], buf[3], buf[4], s);
+      };
+      launch();
+      CEDA_HIP_CHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) launch();
+      CEDA_HIP_CHECK(hipEventRecord(e1, s));
+      CEDA_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      CEDA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        bestc = c;
+      }
+    }
+    P->cfg[which] = bestc;
+  }
+  CEDA_HIP_CHECK(hipEventDestroy(e0));
+  CEDA_HIP_CHECK(hipEventDestroy(e1));
+  CEDA_HIP_CHECK(hipStreamSynchronize(s));
+  for (int k = 0; k < 5; k++) CEDA_HIP_CHECK(hipFree(buf[k]));
+  CEDA_HIP_CHECK(hipFree(coef));
+  P->tuned = true;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = {P->cfg[0], P->cfg[1]};
   return 0;
 }
 

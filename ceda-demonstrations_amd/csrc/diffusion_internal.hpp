@@ -9,6 +9,16 @@
 #include "diffusion_hip.h"
 #include "nvhip_internal.hpp"
 
+// launch configuration of one stencil kernel kind (the plain RHS or the fused
+// STS stage): variant 0 = row-interleaved tiles, rows = rows per wavefront;
+// variant 1 = y-marching, rows = consecutive rows per wavefront; nt = 1 uses
+// non-temporal loads/stores for the streamed operands
+struct StencilCfg {
+  int variant = 0;
+  int rows = 1;
+  int nt = 0;
+};
+
 struct DiffusionProblem {
   DiffusionParams p;
   int64_t n = 0;
@@ -25,7 +35,8 @@ struct DiffusionProblem {
   SUNContext ctx = nullptr;
   ceda::Exec* exec = nullptr;
   long nrhs = 0;
-  int rows = 0;             // stencil rows per wavefront (0: default)
+  StencilCfg cfg[2];        // [0] RHS kernel, [1] fused stage kernel
+  bool tuned = false;
 };
 
 namespace ceda {

@@ -49,8 +49,21 @@ void DiffusionProblem_Destroy(DiffusionProblem* prob);
 void DiffusionProblem_GetGrid(const DiffusionProblem* prob, int64_t* nx, int64_t* ny,
                               double* dx, double* dy);
 long DiffusionProblem_GetNumRhsEvals(const DiffusionProblem* prob);
-/* stencil launch shape: rows per wavefront (1,2,4,8); 0 restores the default */
+/* stencil launch shape of the row-interleaved variant for both kernels:
+   rows per wavefront (1,2,4,8; 0 = 1) */
 int DiffusionProblem_SetStencilRows(DiffusionProblem* prob, int rows);
+/* launch shape of one kernel kind (which: 0 RHS stencil, 1 fused STS stage):
+   variant 0 = row-interleaved tiles (rows per wavefront), variant 1 =
+   y-marching (rows = consecutive rows per wavefront, nt = non-temporal
+   streamed operands).  Every shape computes bitwise-identical results. */
+int DiffusionProblem_SetStencilConfig(DiffusionProblem* prob, int which, int variant, int rows,
+                                      int nt);
+int DiffusionProblem_GetStencilConfig(const DiffusionProblem* prob, int which, int* variant,
+                                      int* rows, int* nt);
+/* times every candidate launch shape of both kernels on scratch buffers of
+   the problem's size (reps launches each, HIP events) and keeps the fastest;
+   results are cached per (nx, ny, kind) for the life of the process */
+int DiffusionProblem_Autotune(DiffusionProblem* prob, int reps);
 
 /* [ext] times `reps` back-to-back launches of one hot-path kernel with HIP
    events on the problem's stream: which = 0 the RHS stencil (u -> out),
