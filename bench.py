@@ -91,11 +91,12 @@ def roofline(pk, ctx, G):
             "rhs_stencil_GBs": round(stencil_gbs, 1)}
 
 
-def kernels_only(pk, reps):
+def kernels_only(pk, reps, sweep):
     """Launch the hot-path kernels alone at the workload's size (the workload
-    of the rocprofv3 counter runs): the fused STS stage, the RHS stencil for
-    each launch shape, and a plain copy (N_VScale by 1: 8 B read + 8 B written
-    per element) whose counters calibrate FETCH_SIZE/WRITE_SIZE."""
+    of the rocprofv3 counter runs): the autotuned fused STS stage and RHS
+    stencil, and a plain copy (N_VScale by 1: 8 B read + 8 B written per
+    element) whose counters calibrate FETCH_SIZE/WRITE_SIZE.  With sweep,
+    first time every candidate launch shape of both kernels."""
     ctx = pk.Context()
     G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
     L = pk.lib()
@@ -104,18 +105,23 @@ def kernels_only(pk, reps):
         L.N_VConst(1.0 + 0.1 * k, v.h)
     res = {}
     avg = C.c_double()
-    cands = [(0, r, 0) for r in (1, 2, 4, 8)] + [(1, r, nt) for r in (4, 8, 16, 32, 64) for nt in (0, 1)]
-    for (v, r, nt) in cands:
-        G.set_config(0, v, r, nt)
-        G.set_config(1, v, r, nt)
-        L.DiffusionProblem_TimeKernel(G.h, 1, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, reps,
-                                      C.byref(avg))
-        res[f"fused_v{v}_r{r}_nt{nt}_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
-        L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
-        res[f"rhs_v{v}_r{r}_nt{nt}_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+    if sweep:
+        cands = [(0, r, 0) for r in (1, 2, 4, 8)] + [(1, r, nt) for r in (4, 8, 16, 32, 64) for nt in (0, 1)]
+        for (v, r, nt) in cands:
+            G.set_config(0, v, r, nt)
+            G.set_config(1, v, r, nt)
+            L.DiffusionProblem_TimeKernel(G.h, 1, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, reps,
+                                          C.byref(avg))
+            res[f"fused_v{v}_r{r}_nt{nt}_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+            L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
+            res[f"rhs_v{v}_r{r}_nt{nt}_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
     G.autotune(3)
     res["autotuned_rhs"] = G.get_config(0)
     res["autotuned_fused"] = G.get_config(1)
+    L.DiffusionProblem_TimeKernel(G.h, 1, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, reps, C.byref(avg))
+    res["tuned_fused_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+    L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
+    res["tuned_rhs_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
     hip_sync()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -153,6 +159,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
                     help="only launch the hot-path kernels REPS times each (profiling workload)")
+    ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one MI355X); "
@@ -162,7 +169,7 @@ def main():
 
     pk = package()
     if args.kernels_only:
-        kernels_only(pk, args.kernels_only)
+        kernels_only(pk, args.kernels_only, args.sweep)
         return
     ctx = pk.Context()
     G, I = make_integrator(pk, ctx, args.fastpath)
