@@ -6,9 +6,13 @@ manufactured 2-D heat equation u_t = kx u_xx + ky u_yy + b(x,y,t) on [0,1]^2,
 spectral radius from power iteration every 25 steps, rtol 1e-6, atol 1e-10;
 fused stencil/stage kernels replayed as one HIP graph per step.  A "step" is
 one adaptive integrator step (s RKL2 stages + the embedded error estimate,
-plus the power iteration when it is due).  Inputs are resident in HBM before
-the timed region; the reported value counts every right-hand-side evaluation
-(stages, error estimate, power-iteration J v products) times the cells.
+plus the power iteration when it is due, plus any rejected attempts).  The
+solve is first advanced, untimed, to t = 0.01 — past the start-up transient of
+tiny steps into the stage-limited regime that makes up almost all of the solve
+to t = 1 — then W warm-up steps and K timed steps follow.  Inputs are resident
+in HBM before the timed region; the reported value counts every right-hand-side
+evaluation (stages, error estimate, power-iteration J v products) times the
+cells.
 
 Extra objects on the JSON line:
   roofline      the fused stage kernel (the dominant kernel: >98% of the
@@ -30,6 +34,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 METRIC = "cell-RHS evals/s; time-to-solution at fixed L2 err, batched/learned vs plain"
 HBM_PEAK_GBS = 8000.0
 N_SIDE = 4096
+T_SETUP = 0.01
 
 
 def log(msg):
@@ -154,7 +159,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--fastpath", type=int, default=2, help="0 API path, 1 fused, 2 fused+graphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
@@ -173,7 +178,10 @@ def main():
         return
     ctx = pk.Context()
     G, I = make_integrator(pk, ctx, args.fastpath)
-    log(f"workload ready: {G.nx}x{G.ny} cells, warmup {args.warmup} steps")
+    log(f"workload ready: {G.nx}x{G.ny} cells; advancing to t = {T_SETUP} (untimed)")
+    flag, t = I.evolve(T_SETUP)
+    assert flag >= 0, flag
+    log(f"setup done at t = {t}, {I.stats()['nst']} steps; warmup {args.warmup} steps")
     for _ in range(args.warmup):
         flag, t = I.evolve(1.0, pk.ARK_ONE_STEP)
         assert flag >= 0, flag
@@ -206,7 +214,9 @@ def main():
         "config": {"workload": "mms-dirichlet 4096x4096 interior FP64, kx=1 ky=0.5, LSRKStep RKL2, "
                                "power-iteration spectral radius every 25 steps, rtol=1e-6 atol=1e-10",
                    "path": ["api", "fused", "fused+hipgraph"][args.fastpath],
-                   "cells": G.n, "rhs_evals_timed": nfe, "stages_per_step": nfe / args.steps},
+                   "cells": G.n, "rhs_evals_timed": nfe, "stages_per_step": nfe / args.steps,
+                   "t_start_timed": T_SETUP, "attempts_per_step": (s1["attempts"] - s0["attempts"]) / args.steps,
+                   "error_test_fails_timed": s1["netf"] - s0["netf"]},
         "roofline": rl, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
