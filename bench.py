@@ -136,6 +136,83 @@ def kernels_only(pk, reps, sweep):
     print(json.dumps(res))
 
 
+def c4_problems(n=256, seed=20260101):
+    """The batched configuration: square grids with side uniform over
+    {64,128,256,512}, kx and ky log-uniform in [0.1, 10]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        side = int(rng.choice([64, 128, 256, 512]))
+        kx, ky = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+        out.append(dict(kind=1, nx=side, ny=side, kx=float(kx), ky=float(ky)))
+    return out
+
+
+def batch_workload(pk, nprob, tout):
+    """Batched RKC (one persistent step group per step, one launch per stage
+    index) against the same problems solved one after another with the
+    single-problem fused+graph path; absolute L2 errors against the exact
+    solution (u_exact(0.5) = 0, so a relative error is undefined there)."""
+    import numpy as np
+
+    ctx = pk.Context()
+    plist = c4_problems(nprob)
+    B = pk.Batch(ctx, plist, "rkc")
+    B.SetTolerances(1e-5, 1e-10)
+    B.SetMaxNumSteps(1000000)
+    hip_sync()
+    log(f"batched: {nprob} problems, {sum(p['nx'] * p['ny'] for p in plist)} cells")
+    t0 = time.perf_counter()
+    assert B.evolve(tout) == 0
+    hip_sync()
+    t_batch = time.perf_counter() - t0
+    groups, launches = B.groups()
+    log(f"batched done in {t_batch:.2f} s ({groups} step groups, {launches} phase launches)")
+    # one after another, each with the fused+graph single-problem path
+    probs, integs = [], []
+    for prm in plist:
+        G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=prm["nx"], ny=prm["ny"], kx=prm["kx"], ky=prm["ky"])
+        I = pk.Integrator(G, "rkc", G.initial())
+        I.ARKodeSStolerances(1e-5, 1e-10)
+        I.ARKodeSetMaxNumSteps(1000000)
+        assert I.LSRKStepSetDiffusionFastPath(G.h, 1) == 0
+        probs.append(G)
+        integs.append(I)
+    hip_sync()
+    log("one after another")
+    t0 = time.perf_counter()
+    for I in integs:
+        flag, t = I.evolve(tout)
+        assert flag == 0
+    hip_sync()
+    t_seq = time.perf_counter() - t0
+    errs, diffs, nfe_b, nfe_s, cells_rhs = [], [], 0, 0, 0
+    for k, (G, I) in enumerate(zip(probs, integs)):
+        yb = B.solution(k)
+        ys = I.y.get()
+        errs.append(float(np.sqrt(np.mean(yb ** 2))))
+        diffs.append(float(np.abs(yb - ys).max() / max(np.abs(ys).max(), 1e-300)))
+        nb, ns = B.stats(k)["nfe"], I.stats()["nfe"]
+        nfe_b += nb
+        nfe_s += ns
+        cells_rhs += nb * G.n
+    out = {"metric": "batched STS solves/s vs one after another (256 uneven problems)",
+           "n_problems": nprob, "tout": tout, "method": "RKC", "rtol": 1e-5,
+           "batched_s": t_batch, "sequential_s": t_seq,
+           "batched_solves_per_s": nprob / t_batch, "sequential_solves_per_s": nprob / t_seq,
+           "speedup": t_seq / t_batch, "step_groups": groups, "phase_launches": launches,
+           "rhs_evals_batched": nfe_b, "rhs_evals_sequential": nfe_s,
+           "batched_cell_rhs_per_s": cells_rhs / t_batch,
+           "abs_l2_error_vs_exact_max": max(errs), "abs_l2_error_vs_exact_median": float(np.median(errs)),
+           "max_rel_state_diff_batched_vs_sequential": max(diffs)}
+    for I in integs:
+        I.free()
+    B.release()
+    print(json.dumps(out))
+
+
 def cpu_baseline():
     """One adaptive RKL2 step of the same workload on the CPU oracle (serial)."""
     import oracle_lib as O
@@ -165,6 +242,10 @@ def main():
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
                     help="only launch the hot-path kernels REPS times each (profiling workload)")
     ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
+    ap.add_argument("--workload", default="sts", choices=["sts", "batch"],
+                    help="sts: the headline 4096^2 RKL2 line; batch: the 256-problem batched comparison")
+    ap.add_argument("--nprob", type=int, default=256)
+    ap.add_argument("--tout", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one MI355X); "
@@ -175,6 +256,9 @@ def main():
     pk = package()
     if args.kernels_only:
         kernels_only(pk, args.kernels_only, args.sweep)
+        return
+    if args.workload == "batch":
+        batch_workload(pk, args.nprob, args.tout)
         return
     ctx = pk.Context()
     G, I = make_integrator(pk, ctx, args.fastpath)

@@ -137,6 +137,19 @@ _SIGS = {
     "SUNAdaptController_ImExGus": (vp, [vp]),
     "SUNAdaptController_Destroy": (i32, [vp]),
     "SUNContext_GetHipStream": (vp, [vp]),
+    "DiffusionBatch_Create": (vp, [i32, C.POINTER(Params), i32, vp]),
+    "DiffusionBatch_Destroy": (None, [vp]),
+    "DiffusionBatch_SetTolerances": (i32, [vp, d, d]),
+    "DiffusionBatch_SetController": (i32, [vp, i32]),
+    "DiffusionBatch_SetMaxNumStages": (i32, [vp, i32]),
+    "DiffusionBatch_SetMaxNumSteps": (i32, [vp, C.c_long]),
+    "DiffusionBatch_SetFixedStep": (i32, [vp, d]),
+    "DiffusionBatch_SetTileRows": (i32, [vp, i32]),
+    "DiffusionBatch_Evolve": (i32, [vp, d]),
+    "DiffusionBatch_GetSolution": (i32, [vp, i32, vp]),
+    "DiffusionBatch_GetCurrentTime": (i32, [vp, i32, dp]),
+    "DiffusionBatch_GetStatsArray": (i32, [vp, i32, C.POINTER(C.c_long), i32]),
+    "DiffusionBatch_GetNumGroups": (i32, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
 }
 
 
@@ -193,7 +206,7 @@ class Context:
     def close(self):
         if self.h:
             kids = list(self.children)
-            for kind in (Integrator, Vector, Problem):
+            for kind in (Integrator, Batch, Vector, Problem):
                 for k in kids:
                     if isinstance(k, kind):
                         k.release()
@@ -339,6 +352,65 @@ class Problem:
     def __del__(self):
         try:
             self.destroy()
+        except Exception:
+            pass
+
+
+class Batch:
+    """DiffusionBatch: many problems, one step group at a time."""
+
+    def __init__(self, ctx, params_list, method="rkc"):
+        self.ctx = ctx
+        arr = (Params * len(params_list))()
+        for k, kw in enumerate(params_list):
+            p = Params()
+            lib().DiffusionParams_Default(C.byref(p), kw.get("kind", MMS_DIRICHLET))
+            for key, v in kw.items():
+                setattr(p, key, v)
+            arr[k] = p
+        self.params = arr
+        self.nprob = len(params_list)
+        self.h = lib().DiffusionBatch_Create(self.nprob, arr, RKC if method == "rkc" else RKL, ctx.h)
+        if not self.h:
+            raise ValueError("DiffusionBatch_Create rejected the parameters")
+        ctx.register(self)
+
+    def __getattr__(self, name):
+        f = getattr(lib(), "DiffusionBatch_" + name)
+        return lambda *a: f(self.h, *a)
+
+    def evolve(self, tout):
+        return lib().DiffusionBatch_Evolve(self.h, tout)
+
+    def solution(self, p):
+        n = self.params[p].nx * self.params[p].ny
+        v = Vector(self.ctx, n=n)
+        assert lib().DiffusionBatch_GetSolution(self.h, p, v.h) == 0
+        return v.get()
+
+    def time(self, p):
+        t = d()
+        lib().DiffusionBatch_GetCurrentTime(self.h, p, C.byref(t))
+        return t.value
+
+    def stats(self, p):
+        out = (C.c_long * 18)()
+        lib().DiffusionBatch_GetStatsArray(self.h, p, out, 18)
+        return dict(zip(STATS, list(out)))
+
+    def groups(self):
+        g, l = C.c_long(), C.c_long()
+        lib().DiffusionBatch_GetNumGroups(self.h, C.byref(g), C.byref(l))
+        return g.value, l.value
+
+    def release(self):
+        if self.h and self.ctx.h:
+            lib().DiffusionBatch_Destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
         except Exception:
             pass
 

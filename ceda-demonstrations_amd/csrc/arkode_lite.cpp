@@ -38,96 +38,9 @@
 #include <map>
 #include <vector>
 
-#include "arkode_hip.h"
-#include "diffusion_internal.hpp"
+#include "arkode_internal.hpp"
 
-struct SUNAdaptController_ {
-  int type;
-};
-struct SUNMatrix_ {
-  int unused;
-};
-struct SUNDomEigEstimator_ {
-  long maxiters = 100;
-  double rtol = 0.01;
-  N_Vector V = nullptr, q = nullptr, work = nullptr;
-  int started = 0;
-};
-enum { LS_PCG = 0, LS_SPGMR = 1, LS_SPFGMR = 2 };
-struct SUNLinearSolver_ {
-  int type = LS_PCG;
-  int pretype = SUN_PREC_NONE;
-  int maxl = 5;
-  int mrestart = 30;
-  N_Vector r = nullptr, p = nullptr, z = nullptr, Ap = nullptr;
-  std::vector<N_Vector> V, Z;
-};
-
-namespace {
-
-using ceda::StageEntry;
-
-enum { M_RKC = 0, M_RKL = 1, M_DIRK = 2 };
-constexpr uint64_t kDeeSeed = 20240917ull;
-constexpr double kTiny = 1.0e-10;
-constexpr double kOneMSM = 0.999999;
-constexpr double kOnePSM = 1.000001;
-constexpr int kRetry = 100;
-constexpr int kConvRecover = 101;
-
-struct ArkMem {
-  SUNContext ctx = nullptr;
-  int method = M_RKC;
-  ARKRhsFn f = nullptr;
-  void* ud = nullptr;
-  int64_t n = 0;
-  double rtol = 1.0e-5, atol = 1.0e-10;
-  double tn = 0, h = 0, h0u = 0, hfixed = 0, hold = 0, h0_used = 0;
-  double tstop = 0;
-  bool tstopset = false;
-  int fixedstep = 0, initialized = 0;
-  long mxstep = 500;
-  N_Vector yn = nullptr, ycur = nullptr, fn = nullptr, tv1 = nullptr, tv2 = nullptr,
-           ewt = nullptr;
-  int fn_is_current = 0, ewt_current = 0;
-  // controller
-  int ctype = ARK_ADAPT_PID, p_order = 2, firststeps = 0;
-  double k1 = 0.58, k2 = 0.21, k3 = 0.1, bias = 1.5, ep = 0, epp = 0, hp = 0, hpp = 0;
-  // adaptivity
-  double safety = 0.96, growth = 20.0, etamx1 = 10000.0, etamxf = 0.3, etamin = 0.1,
-         etacf = 0.25, lbound = 1.0, ubound = 1.5, eta = 1.0;
-  int small_nef = 2, maxnef = 7, maxncf = 10;
-  // LSRK
-  double rho = 0, rho_max = 0, dom_eig_safety = 1.01;
-  long dom_eig_freq = 25, dom_eig_nst = 0;
-  int dom_eig_update = 1, smax = 200;
-  ARKDomEigFn domeig = nullptr;
-  SUNDomEigEstimator dee = nullptr;
-  int dee_init_pre = 0, dee_pre = 0;
-  // fused path
-  DiffusionProblem* fast = nullptr;
-  int use_graphs = 0;
-  StageEntry* h_tab = nullptr;
-  StageEntry* d_tab = nullptr;
-  hipEvent_t tab_ev = nullptr;
-  int tab_cap = 0;
-  std::map<int, hipGraphExec_t> graphs;
-  // DIRK
-  int order = 3, stages = 0, linear = 1, msbp = 20, psetup_done = 0, maxcor = 3;
-  double A[16], b[4], bh[4], c[4];
-  N_Vector F[4] = {nullptr, nullptr, nullptr, nullptr};
-  N_Vector sdata = nullptr, zcur = nullptr, fz = nullptr, rhsv = nullptr, dz = nullptr,
-           jwork = nullptr, jv = nullptr;
-  double epslin = 0, ls_reltol = 0, gammap = 0, nlscoef = 0.1;
-  long nstlp = 0;
-  SUNLinearSolver LS = nullptr;
-  ARKLsPrecSetupFn psetup = nullptr;
-  ARKLsPrecSolveFn psolve = nullptr;
-  // statistics (same layout as the CPU restatement)
-  long nst = 0, attempts = 0, nfe = 0, netf = 0, nstlim = 0, ndomeig = 0, ndee_iters = 0,
-       stage_max = 0, last_stages = 0, nfe_dee = 0, nli = 0, npsetup = 0, npsolve = 0,
-       nni = 0, ncfn = 0, nlsfail = 0, njtimes = 0, nfe_ls = 0;
-};
+namespace ceda_ark {
 
 ArkMem* AM(void* m) { return static_cast<ArkMem*>(m); }
 
@@ -425,12 +338,6 @@ int compute_dom_eig(ArkMem* A)
 }
 
 // ------------------------------------------------------------ LSRK step
-struct StagePlan {
-  int s;
-  double mus1;                 // stage 1: Y1 = yn + (h*mus1) fn
-  std::vector<double> c;       // 5 per stage j = 2..s
-  std::vector<double> t;       // RHS time of stage j
-};
 
 void plan_stages(const ArkMem* A, int s, double h, double tn, StagePlan& P)
 {
@@ -562,14 +469,10 @@ void issue_stage_kernels(ArkMem* A, int s, hipStream_t st)
 
 // the fused path: the same arithmetic, one kernel per stage, operands and
 // coefficients from the stage table, optionally one graph launch per step
-int stages_fused(ArkMem* A, const StagePlan& SP)
+void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T)
 {
   const int s = SP.s;
   DiffusionProblem* P = A->fast;
-  hipStream_t st = P->exec->stream;
-  ensure_table(A, A->smax + 2);
-  CEDA_HIP_CHECK(hipEventSynchronize(A->tab_ev));  // previous copy consumed h_tab
-  StageEntry* T = A->h_tab;
   const bool mms = (P->p.kind == DIFFUSION_MMS_DIRICHLET);
   // roles: Y_{j-2} starts as yn itself (the API path copies yn into tv2 first;
   // reading yn directly gives the same values), scratch rotates over 3 buffers
@@ -609,6 +512,18 @@ int stages_fused(ArkMem* A, const StagePlan& SP)
   if (mms) ceda::mms_time_factors(A->tn + A->h, &ef.T1, &ef.T2);
   ef.yjm1 = N_VGetDeviceArrayPointer(A->ycur);
   ef.out = N_VGetDeviceArrayPointer(A->tv2);
+}
+
+// the fused path: the same arithmetic, one kernel per stage, operands and
+// coefficients from the stage table, optionally one graph launch per step
+int stages_fused(ArkMem* A, const StagePlan& SP)
+{
+  const int s = SP.s;
+  DiffusionProblem* P = A->fast;
+  hipStream_t st = P->exec->stream;
+  ensure_table(A, A->smax + 2);
+  CEDA_HIP_CHECK(hipEventSynchronize(A->tab_ev));  // previous copy consumed h_tab
+  fused_fill(A, SP, A->h_tab);
   CEDA_HIP_CHECK(hipMemcpyAsync(A->d_tab, A->h_tab, sizeof(StageEntry) * (s + 2),
                                 hipMemcpyHostToDevice, st));
   CEDA_HIP_CHECK(hipEventRecord(A->tab_ev, st));
@@ -633,7 +548,9 @@ int stages_fused(ArkMem* A, const StagePlan& SP)
   return 0;
 }
 
-int lsrk_takestep(ArkMem* A, double* dsm)
+// plans one LSRK attempt: spectral radius when due, stage count (or a
+// stage-limited retry, kRetry with A->eta set), F(t_n, y_n), coefficients
+int lsrk_plan(ArkMem* A, StagePlan& SP)
 {
   const double h = A->h, tn = A->tn;
   if (A->dom_eig_update) {
@@ -662,11 +579,34 @@ int lsrk_takestep(ArkMem* A, double* dsm)
     if (r) return r;
     A->fn_is_current = 1;
   }
-  StagePlan SP;
   plan_stages(A, s, h, tn, SP);
-  int r = A->fast ? stages_fused(A, SP) : stages_api(A, SP);
-  if (r) return r;
+  return 0;
+}
 
+// after the stages and the error norm of an attempt: FSAL update of F_n and
+// the spectral-radius refresh rules
+void lsrk_after(ArkMem* A, double dsm)
+{
+  if (A->fixedstep || dsm <= 1.0) {
+    if (!A->fixedstep) {
+      N_Vector t = A->fn;
+      A->fn = A->tv2;
+      A->tv2 = t;
+      A->fn_is_current = 1;
+    } else A->fn_is_current = 0;
+    if (A->dom_eig_freq > 0 && (A->nst + 1 - A->dom_eig_nst) >= A->dom_eig_freq)
+      A->dom_eig_update = 1;
+  } else if (A->dom_eig_nst != A->nst) A->dom_eig_update = 1;
+}
+
+int lsrk_takestep(ArkMem* A, double* dsm)
+{
+  const double h = A->h;
+  StagePlan SP;
+  int r = lsrk_plan(A, SP);
+  if (r) return r;
+  r = A->fast ? stages_fused(A, SP) : stages_api(A, SP);
+  if (r) return r;
   if (!A->fixedstep) {
     if (A->fast) {
       const double sq = ceda::raw_sts_error_wsqr(
@@ -681,17 +621,7 @@ int lsrk_takestep(ArkMem* A, double* dsm)
       *dsm = N_VWrmsNorm(A->tv1, A->ewt);
     }
   } else *dsm = 0.0;
-
-  if (A->fixedstep || *dsm <= 1.0) {
-    if (!A->fixedstep) {
-      N_Vector t = A->fn;
-      A->fn = A->tv2;
-      A->tv2 = t;
-      A->fn_is_current = 1;
-    } else A->fn_is_current = 0;
-    if (A->dom_eig_freq > 0 && (A->nst + 1 - A->dom_eig_nst) >= A->dom_eig_freq)
-      A->dom_eig_update = 1;
-  } else if (A->dom_eig_nst != A->nst) A->dom_eig_update = 1;
+  lsrk_after(A, *dsm);
   return 0;
 }
 
@@ -946,45 +876,48 @@ int dirk_takestep(ArkMem* A, double* dsm, int force_setup)
 }
 
 // ------------------------------------------------------------ stepping
-int take_one_step(ArkMem* A, double tout)
+void attempt_begin(ArkMem* A, double tout, AttemptState& S)
 {
-  int nef = 0, ncf = 0, force_setup = 0;
-  double dsm = 0.0;
-  int clipped;
-  for (;;) {
-    clipped = 0;
-    if (A->tn + A->h >= tout) {
-      A->h = tout - A->tn;
-      clipped = 1;
-    }
-    A->attempts++;
-    int ret = (A->method == M_DIRK) ? dirk_takestep(A, &dsm, force_setup) : lsrk_takestep(A, &dsm);
-    force_setup = 0;
-    if (ret == kRetry) {
-      A->h *= A->eta;
-      A->nstlim++;
-      continue;
-    }
-    if (ret == kConvRecover) {
-      A->ncfn++;
-      ncf++;
-      if (ncf >= A->maxncf) return ARK_CONV_FAILURE;
-      A->h *= A->etacf;
-      force_setup = 1;
-      continue;
-    }
-    if (ret < 0) return ret;
-    if (!A->fixedstep && dsm > 1.0) {
-      A->netf++;
-      nef++;
-      if (nef >= A->maxnef) return ARK_ERR_FAILURE;
-      A->h *= failure_eta(A, dsm, nef);
-      continue;
-    }
-    break;
+  S.clipped = 0;
+  if (A->tn + A->h >= tout) {
+    A->h = tout - A->tn;
+    S.clipped = 1;
   }
+  A->attempts++;
+}
+
+// 0: accepted, 1: attempt again, <0: failure
+int attempt_end(ArkMem* A, int ret, double dsm, AttemptState& S)
+{
+  S.force_setup = 0;
+  if (ret == kRetry) {
+    A->h *= A->eta;
+    A->nstlim++;
+    return 1;
+  }
+  if (ret == kConvRecover) {
+    A->ncfn++;
+    S.ncf++;
+    if (S.ncf >= A->maxncf) return ARK_CONV_FAILURE;
+    A->h *= A->etacf;
+    S.force_setup = 1;
+    return 1;
+  }
+  if (ret < 0) return ret;
+  if (!A->fixedstep && dsm > 1.0) {
+    A->netf++;
+    S.nef++;
+    if (S.nef >= A->maxnef) return ARK_ERR_FAILURE;
+    A->h *= failure_eta(A, dsm, S.nef);
+    return 1;
+  }
+  return 0;
+}
+
+int step_complete(ArkMem* A, double dsm, double tout, const AttemptState& S)
+{
   A->hold = A->h;
-  A->tn = clipped ? tout : A->tn + A->h;
+  A->tn = S.clipped ? tout : A->tn + A->h;
   N_Vector t = A->yn;
   A->yn = A->ycur;
   A->ycur = t;
@@ -999,7 +932,47 @@ int take_one_step(ArkMem* A, double tout)
   return 0;
 }
 
-}  // namespace
+int take_one_step(ArkMem* A, double tout)
+{
+  AttemptState S;
+  double dsm = 0.0;
+  for (;;) {
+    attempt_begin(A, tout, S);
+    const int ret = (A->method == M_DIRK) ? dirk_takestep(A, &dsm, S.force_setup)
+                                          : lsrk_takestep(A, &dsm);
+    const int r = attempt_end(A, ret, dsm, S);
+    if (r == 1) continue;
+    if (r < 0) return r;
+    break;
+  }
+  return step_complete(A, dsm, tout, S);
+}
+
+// first call of ARKodeEvolve: error weights, F(t0, y0) and the initial step
+int evolve_init(ArkMem* A, double tout)
+{
+  if (ewt_set(A, A->yn)) return ARK_ILL_INPUT;
+  if (A->fixedstep) A->h = A->hfixed;
+  else if (A->h0u != 0.0) A->h = A->h0u;
+  else {
+    if (!A->fn_is_current) {
+      const int r = call_rhs(A, A->tn, A->yn, A->fn);
+      if (r) return ARK_RHSFUNC_FAIL;
+      A->fn_is_current = 1;
+    }
+    double h0 = 0.0;
+    const int r = initial_step(A, tout, &h0);
+    if (r) return r;
+    A->h = h0;
+  }
+  A->h0_used = A->h;
+  A->initialized = 1;
+  return 0;
+}
+
+}  // namespace ceda_ark
+
+using namespace ceda_ark;
 
 // ===================================================================== C ABI
 extern "C" {
@@ -1259,22 +1232,8 @@ int ARKodeEvolve(void* m, sunrealtype tout, N_Vector yout, sunrealtype* tret, in
   if (A->tstopset && A->tstop < tout) tout = A->tstop;
   int ret = ARK_SUCCESS;
   if (!A->initialized) {
-    if (ewt_set(A, A->yn)) return ARK_ILL_INPUT;
-    if (A->fixedstep) A->h = A->hfixed;
-    else if (A->h0u != 0.0) A->h = A->h0u;
-    else {
-      if (!A->fn_is_current) {
-        const int r = call_rhs(A, A->tn, A->yn, A->fn);
-        if (r) return ARK_RHSFUNC_FAIL;
-        A->fn_is_current = 1;
-      }
-      double h0 = 0.0;
-      const int r = initial_step(A, tout, &h0);
-      if (r) return r;
-      A->h = h0;
-    }
-    A->h0_used = A->h;
-    A->initialized = 1;
+    const int r = evolve_init(A, tout);
+    if (r) return r;
   }
   long nstloc = 0;
   while (A->tn < tout) {

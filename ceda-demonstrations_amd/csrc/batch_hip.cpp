@@ -1,0 +1,486 @@
+// Batched super-time-stepping engine: many independent diffusion problems,
+// each with its own adaptive step and stage count, advanced in step groups.
+//
+// Every problem is an ordinary LSRKStep integrator (ArkMem) on its own
+// DiffusionProblem, and takes exactly the decisions the single-problem path
+// takes (attempt_begin / lsrk_plan / lsrk_after / attempt_end /
+// step_complete are shared).  What changes is the execution of the stage
+// loops: in a step group every active problem plans one attempt, the planned
+// problems are sorted by stage count (descending) and their tiles are listed
+// in that order, so the tiles still in their stage loop at stage index j are
+// a prefix of the list; phase j is one launch over that prefix.  A segmented
+// reduction then produces every problem's error norm (one device-to-host copy
+// per group).  Phase launches of small problems, which would each be limited
+// by launch latency when the problems are solved one after another, are
+// amortised over the whole batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "arkode_internal.hpp"
+#include "batch_hip.h"
+
+using namespace ceda_ark;
+using ceda::BatchDesc;
+using ceda::BatchErr;
+using ceda::BatchTile;
+using ceda::StageEntry;
+
+namespace {
+
+struct BProb {
+  DiffusionProblem* P = nullptr;
+  N_Vector y0 = nullptr;
+  ArkMem* A = nullptr;
+  AttemptState S;
+  long nst0 = 0;
+  bool done = false, new_step = true;
+  int err = 0;
+  int rows = 0, cpl = 1, tiles_x = 0, tiles_y = 0;
+  StagePlan SP;
+};
+
+int tiles_of(const BProb& b) { return b.tiles_x * b.tiles_y; }
+
+}  // namespace
+
+struct DiffusionBatch {
+  SUNContext ctx = nullptr;
+  ceda::Exec* ex = nullptr;
+  std::vector<BProb> probs;
+  int method = M_RKC;
+  int smax = 200, stride = 202;
+  int tile_rows = 0;
+  bool tuned = false;
+  long ngroups = 0, nlaunches = 0;
+  // device/pinned staging (grown on demand)
+  BatchDesc* d_desc = nullptr;
+  StageEntry *d_ent = nullptr, *h_ent = nullptr;
+  BatchTile *d_tiles = nullptr, *h_tiles = nullptr;
+  BatchErr *d_err = nullptr, *h_err = nullptr;
+  int *d_s = nullptr, *h_s = nullptr, *d_tb = nullptr, *h_tb = nullptr, *d_tc = nullptr,
+      *h_tc = nullptr;
+  double *d_part = nullptr, *d_dsq = nullptr, *h_dsq = nullptr;
+  int cap_slots = 0, cap_tiles = 0;
+};
+
+namespace {
+
+template <class T>
+void dev_alloc(T** d, size_t n)
+{
+  CEDA_HIP_CHECK(hipMalloc(d, sizeof(T) * (n ? n : 1)));
+}
+template <class T>
+void host_alloc(T** h, size_t n)
+{
+  CEDA_HIP_CHECK(hipHostMalloc(h, sizeof(T) * (n ? n : 1), hipHostMallocDefault));
+}
+
+void free_staging(DiffusionBatch* B)
+{
+  void* ds[] = {B->d_ent, B->d_tiles, B->d_err, B->d_s, B->d_tb, B->d_tc, B->d_part, B->d_dsq};
+  for (void* p : ds)
+    if (p) CEDA_HIP_CHECK(hipFree(p));
+  void* hs[] = {B->h_ent, B->h_tiles, B->h_err, B->h_s, B->h_tb, B->h_tc, B->h_dsq};
+  for (void* p : hs)
+    if (p) CEDA_HIP_CHECK(hipHostFree(p));
+  B->d_ent = B->h_ent = nullptr;
+  B->d_tiles = B->h_tiles = nullptr;
+  B->d_err = B->h_err = nullptr;
+  B->d_s = B->h_s = B->d_tb = B->h_tb = B->d_tc = B->h_tc = nullptr;
+  B->d_part = B->d_dsq = B->h_dsq = nullptr;
+}
+
+void ensure_staging(DiffusionBatch* B, int slots, int tiles)
+{
+  if (slots <= B->cap_slots && tiles <= B->cap_tiles) return;
+  CEDA_HIP_CHECK(hipStreamSynchronize(B->ex->stream));
+  free_staging(B);
+  B->cap_slots = std::max(slots, B->cap_slots);
+  B->cap_tiles = std::max(tiles, B->cap_tiles);
+  const size_t ns = B->cap_slots, nt = B->cap_tiles;
+  dev_alloc(&B->d_ent, ns * B->stride);
+  host_alloc(&B->h_ent, ns * B->stride);
+  memset(B->h_ent, 0, sizeof(StageEntry) * ns * B->stride);
+  dev_alloc(&B->d_tiles, nt);
+  host_alloc(&B->h_tiles, nt);
+  dev_alloc(&B->d_err, ns);
+  host_alloc(&B->h_err, ns);
+  dev_alloc(&B->d_s, ns);
+  host_alloc(&B->h_s, ns);
+  dev_alloc(&B->d_tb, ns);
+  host_alloc(&B->h_tb, ns);
+  dev_alloc(&B->d_tc, ns);
+  host_alloc(&B->h_tc, ns);
+  dev_alloc(&B->d_part, nt);
+  dev_alloc(&B->d_dsq, ns);
+  host_alloc(&B->h_dsq, ns);
+}
+
+void set_geometry(BProb& b, int rows)
+{
+  const DiffusionParams& p = b.P->p;
+  b.cpl = (p.nx % 2 == 0) ? 2 : 1;
+  b.rows = rows;
+  b.tiles_x = (int)((p.nx + 64 * b.cpl - 1) / (64 * b.cpl));
+  b.tiles_y = (int)((p.ny + 4 * rows - 1) / (4 * rows));
+}
+
+void upload_desc(DiffusionBatch* B)
+{
+  std::vector<BatchDesc> h(B->probs.size());
+  for (size_t k = 0; k < B->probs.size(); k++) {
+    const BProb& b = B->probs[k];
+    const DiffusionProblem* P = b.P;
+    h[k] = BatchDesc{P->d_fx, P->d_fy, P->d_sx, P->d_sy, P->d_cx, P->d_cy, P->p.nx, P->p.ny,
+                     P->p.kind == DIFFUSION_CEDA_PERIODIC ? 1 : 0, b.cpl, b.rows, b.tiles_x};
+  }
+  if (!B->d_desc) dev_alloc(&B->d_desc, h.size());
+  CEDA_HIP_CHECK(hipMemcpy(B->d_desc, h.data(), sizeof(BatchDesc) * h.size(), hipMemcpyHostToDevice));
+}
+
+// the tile autotuner: for every size class (nx, ny) time the fused-stage phase
+// over that class's tiles for each candidate height and keep the fastest
+void autotune_tiles(DiffusionBatch* B)
+{
+  std::map<std::pair<int64_t, int64_t>, std::vector<int>> classes;
+  for (size_t k = 0; k < B->probs.size(); k++)
+    classes[{B->probs[k].P->p.nx, B->probs[k].P->p.ny}].push_back((int)k);
+  hipStream_t s = B->ex->stream;
+  hipEvent_t e0, e1;
+  CEDA_HIP_CHECK(hipEventCreate(&e0));
+  CEDA_HIP_CHECK(hipEventCreate(&e1));
+  for (auto& cl : classes) {
+    const std::vector<int>& ks = cl.second;
+    int best_rows = 2;
+    float best = 1e30f;
+    for (int rows : {1, 2, 4, 8}) {
+      for (int k : ks) set_geometry(B->probs[k], rows);
+      upload_desc(B);
+      int nt = 0;
+      for (int k : ks) nt += tiles_of(B->probs[k]);
+      ensure_staging(B, (int)ks.size(), nt);
+      int t = 0;
+      for (size_t q = 0; q < ks.size(); q++) {
+        BProb& b = B->probs[ks[q]];
+        StageEntry& e = B->h_ent[q * B->stride + 2];
+        memset(&e, 0, sizeof(e));
+        e.yjm1 = N_VGetDeviceArrayPointer(b.A->yn);
+        e.yjm2 = N_VGetDeviceArrayPointer(b.A->yn);
+        e.yn = N_VGetDeviceArrayPointer(b.A->yn);
+        e.fn = N_VGetDeviceArrayPointer(b.A->yn);
+        e.out = N_VGetDeviceArrayPointer(b.A->tv1);  // scratch at a step boundary
+        B->h_s[q] = 3;
+        for (int ty = 0; ty < b.tiles_y; ty++)
+          for (int tx = 0; tx < b.tiles_x; tx++) B->h_tiles[t++] = BatchTile{(int)q, ks[q], tx, ty};
+      }
+      CEDA_HIP_CHECK(hipMemcpyAsync(B->d_ent, B->h_ent, sizeof(StageEntry) * ks.size() * B->stride,
+                                    hipMemcpyHostToDevice, s));
+      CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(BatchTile) * nt,
+                                    hipMemcpyHostToDevice, s));
+      CEDA_HIP_CHECK(hipMemcpyAsync(B->d_s, B->h_s, sizeof(int) * ks.size(), hipMemcpyHostToDevice, s));
+      ceda::launch_batch_phase(B->d_desc, B->d_tiles, B->d_ent, B->stride, B->d_s, 2, nt, s);
+      CEDA_HIP_CHECK(hipEventRecord(e0, s));
+      for (int r = 0; r < 3; r++)
+        ceda::launch_batch_phase(B->d_desc, B->d_tiles, B->d_ent, B->stride, B->d_s, 2, nt, s);
+      CEDA_HIP_CHECK(hipEventRecord(e1, s));
+      CEDA_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      CEDA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_rows = rows;
+      }
+    }
+    for (int k : ks) set_geometry(B->probs[k], best_rows);
+  }
+  CEDA_HIP_CHECK(hipEventDestroy(e0));
+  CEDA_HIP_CHECK(hipEventDestroy(e1));
+  upload_desc(B);
+  B->tuned = true;
+}
+
+// one step group: every active problem plans an attempt, all planned stage
+// loops run together, then every problem finishes its attempt
+int step_group(DiffusionBatch* B, double tout, bool* progress)
+{
+  *progress = false;
+  std::vector<int> planned;
+  for (size_t k = 0; k < B->probs.size(); k++) {
+    BProb& b = B->probs[k];
+    if (b.done) continue;
+    ArkMem* A = b.A;
+    if (A->tn >= tout) {
+      b.done = true;
+      continue;
+    }
+    if (b.new_step && A->mxstep > 0 && A->nst - b.nst0 >= A->mxstep) {
+      b.err = ARK_TOO_MUCH_WORK;
+      b.done = true;
+      continue;
+    }
+    if (b.new_step) {
+      b.S = AttemptState();
+      b.new_step = false;
+    }
+    for (;;) {
+      attempt_begin(A, tout, b.S);
+      const int r = lsrk_plan(A, b.SP);
+      if (r == 0) {
+        planned.push_back((int)k);
+        break;
+      }
+      const int r2 = attempt_end(A, r, 0.0, b.S);
+      if (r2 == 1) continue;
+      b.err = r2 < 0 ? r2 : r;
+      b.done = true;
+      break;
+    }
+  }
+  if (planned.empty()) return 0;
+  *progress = true;
+  std::stable_sort(planned.begin(), planned.end(), [&](int a, int c) {
+    return B->probs[a].SP.s > B->probs[c].SP.s;
+  });
+  const bool adaptive = !B->probs[planned[0]].A->fixedstep;
+  int ntiles = 0;
+  for (int k : planned) ntiles += tiles_of(B->probs[k]);
+  const int nslots = (int)planned.size();
+  ensure_staging(B, nslots, ntiles);
+  int t = 0, jmax = 0;
+  std::vector<int> tile_end(nslots);
+  for (int q = 0; q < nslots; q++) {
+    BProb& b = B->probs[planned[q]];
+    ArkMem* A = b.A;
+    StageEntry* T = B->h_ent + (size_t)q * B->stride;
+    fused_fill(A, b.SP, T);
+    B->h_s[q] = b.SP.s;
+    B->h_tb[q] = t;
+    for (int ty = 0; ty < b.tiles_y; ty++)
+      for (int tx = 0; tx < b.tiles_x; tx++) B->h_tiles[t++] = BatchTile{q, planned[q], tx, ty};
+    B->h_tc[q] = t - B->h_tb[q];
+    tile_end[q] = t;
+    B->h_err[q] = BatchErr{N_VGetDeviceArrayPointer(A->yn), N_VGetDeviceArrayPointer(A->ycur),
+                           N_VGetDeviceArrayPointer(A->fn), N_VGetDeviceArrayPointer(A->tv2),
+                           0.4 * A->h, A->rtol, A->atol, 0.0};
+    jmax = std::max(jmax, b.SP.s + (adaptive ? 1 : 0));
+  }
+  hipStream_t s = B->ex->stream;
+  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_ent, B->h_ent, sizeof(StageEntry) * nslots * B->stride,
+                                hipMemcpyHostToDevice, s));
+  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(BatchTile) * ntiles,
+                                hipMemcpyHostToDevice, s));
+  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_s, B->h_s, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
+  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_err, B->h_err, sizeof(BatchErr) * nslots, hipMemcpyHostToDevice, s));
+  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tb, B->h_tb, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
+  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tc, B->h_tc, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
+  // phase j covers the slots with s + (adaptive) >= j: a prefix of the slots
+  int active = nslots;
+  for (int j = 1; j <= jmax; j++) {
+    while (active > 0 && B->h_s[active - 1] + (adaptive ? 1 : 0) < j) active--;
+    if (active == 0) break;
+    ceda::launch_batch_phase(B->d_desc, B->d_tiles, B->d_ent, B->stride, B->d_s, j,
+                             tile_end[active - 1], s);
+    B->nlaunches++;
+  }
+  if (adaptive) {
+    ceda::launch_batch_err(B->d_desc, B->d_tiles, B->d_err, ntiles, B->d_part, B->d_tb, B->d_tc,
+                           nslots, B->d_dsq, s);
+    CEDA_HIP_CHECK(hipMemcpyAsync(B->h_dsq, B->d_dsq, sizeof(double) * nslots,
+                                  hipMemcpyDeviceToHost, s));
+  }
+  CEDA_HIP_CHECK(hipGetLastError());
+  CEDA_HIP_CHECK(hipStreamSynchronize(s));
+  B->ngroups++;
+  for (int q = 0; q < nslots; q++) {
+    BProb& b = B->probs[planned[q]];
+    ArkMem* A = b.A;
+    const long nrhs = (b.SP.s - 1) + (adaptive ? 1 : 0);
+    A->nfe += nrhs;
+    b.P->nrhs += nrhs;
+    const double dsm = adaptive ? sqrt(B->h_dsq[q] / (double)A->n) : 0.0;
+    lsrk_after(A, dsm);
+    const int r = attempt_end(A, 0, dsm, b.S);
+    if (r == 0) {
+      const int rc = step_complete(A, dsm, tout, b.S);
+      if (rc < 0) {
+        b.err = rc;
+        b.done = true;
+      }
+      b.new_step = true;
+    } else if (r < 0) {
+      b.err = r;
+      b.done = true;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+DiffusionBatch* DiffusionBatch_Create(int nprob, const DiffusionParams* params,
+                                      ARKODE_LSRKMethodType method, SUNContext ctx)
+{
+  if (nprob < 1 || !params || !ctx) return nullptr;
+  DiffusionBatch* B = new DiffusionBatch();
+  B->ctx = ctx;
+  B->ex = ceda::context_exec(ctx);
+  B->method = (method == ARKODE_LSRK_RKL_2) ? M_RKL : M_RKC;
+  B->probs.resize(nprob);
+  for (int k = 0; k < nprob; k++) {
+    BProb& b = B->probs[k];
+    b.P = DiffusionProblem_Create(&params[k], ctx);
+    if (!b.P) {
+      DiffusionBatch_Destroy(B);
+      return nullptr;
+    }
+    b.y0 = N_VNew_HipNative(b.P->n, ctx);
+    Initial(0.0, b.y0, b.P);
+    void* mem = LSRKStepCreateSTS(diffusion, 0.0, b.y0, ctx);
+    b.A = static_cast<ArkMem*>(mem);
+    LSRKStepSetSTSMethod(mem, method);
+    ARKodeSetUserData(mem, b.P);
+    LSRKStepSetDomEigFn(mem, dom_eig);
+    b.A->fast = b.P;  // error weights only when needed, as in the fused path
+    set_geometry(b, 2);
+  }
+  upload_desc(B);
+  return B;
+}
+
+void DiffusionBatch_Destroy(DiffusionBatch* B)
+{
+  if (!B) return;
+  CEDA_HIP_CHECK(hipStreamSynchronize(B->ex->stream));
+  free_staging(B);
+  if (B->d_desc) CEDA_HIP_CHECK(hipFree(B->d_desc));
+  for (BProb& b : B->probs) {
+    if (b.A) {
+      void* m = b.A;
+      ARKodeFree(&m);
+    }
+    if (b.y0) N_VDestroy(b.y0);
+    if (b.P) DiffusionProblem_Destroy(b.P);
+  }
+  delete B;
+}
+
+int DiffusionBatch_SetTolerances(DiffusionBatch* B, sunrealtype rtol, sunrealtype atol)
+{
+  if (!B) return ARK_MEM_NULL;
+  for (BProb& b : B->probs) ARKodeSStolerances(b.A, rtol, atol);
+  return 0;
+}
+
+int DiffusionBatch_SetController(DiffusionBatch* B, int controller)
+{
+  if (!B || controller < 0 || controller > 5) return ARK_ILL_INPUT;
+  for (BProb& b : B->probs) {
+    b.A->ctype = controller;
+    SUNAdaptController C = SUNAdaptController_PID(B->ctx);
+    C->type = controller;
+    ARKodeSetAdaptController(b.A, C);
+    SUNAdaptController_Destroy(C);
+  }
+  return 0;
+}
+
+int DiffusionBatch_SetMaxNumStages(DiffusionBatch* B, int smax)
+{
+  if (!B || smax < 2) return ARK_ILL_INPUT;
+  for (BProb& b : B->probs) LSRKStepSetMaxNumStages(b.A, smax);
+  B->smax = smax;
+  CEDA_HIP_CHECK(hipStreamSynchronize(B->ex->stream));
+  free_staging(B);
+  B->cap_slots = B->cap_tiles = 0;
+  B->stride = smax + 2;
+  return 0;
+}
+
+int DiffusionBatch_SetMaxNumSteps(DiffusionBatch* B, long mxsteps)
+{
+  if (!B) return ARK_MEM_NULL;
+  for (BProb& b : B->probs) ARKodeSetMaxNumSteps(b.A, mxsteps);
+  return 0;
+}
+
+int DiffusionBatch_SetFixedStep(DiffusionBatch* B, sunrealtype h)
+{
+  if (!B) return ARK_MEM_NULL;
+  for (BProb& b : B->probs) ARKodeSetFixedStep(b.A, h);
+  return 0;
+}
+
+int DiffusionBatch_SetTileRows(DiffusionBatch* B, int rows)
+{
+  if (!B || rows < 0 || rows > 16) return ARK_ILL_INPUT;
+  B->tile_rows = rows;
+  if (rows > 0) {
+    for (BProb& b : B->probs) set_geometry(b, rows);
+    upload_desc(B);
+    B->tuned = true;
+  } else B->tuned = false;
+  return 0;
+}
+
+int DiffusionBatch_Evolve(DiffusionBatch* B, sunrealtype tout)
+{
+  if (!B) return ARK_MEM_NULL;
+  for (BProb& b : B->probs) {
+    if (!b.A->initialized) {
+      const int r = evolve_init(b.A, tout);
+      if (r) return r;
+    }
+    b.nst0 = b.A->nst;
+    b.done = (b.A->tn >= tout);
+    b.new_step = true;
+    b.err = 0;
+  }
+  if (!B->tuned) autotune_tiles(B);
+  bool progress = true;
+  while (progress) {
+    const int r = step_group(B, tout, &progress);
+    if (r) return r;
+  }
+  for (BProb& b : B->probs)
+    if (b.err) return b.err;
+  return 0;
+}
+
+int DiffusionBatch_GetSolution(DiffusionBatch* B, int p, N_Vector y)
+{
+  if (!B || p < 0 || p >= (int)B->probs.size()) return ARK_ILL_INPUT;
+  if (!y || N_VGetLength(y) != B->probs[p].A->n) return ARK_ILL_INPUT;
+  N_VScale(1.0, B->probs[p].A->yn, y);
+  return 0;
+}
+
+int DiffusionBatch_GetCurrentTime(DiffusionBatch* B, int p, sunrealtype* t)
+{
+  if (!B || p < 0 || p >= (int)B->probs.size()) return ARK_ILL_INPUT;
+  *t = B->probs[p].A->tn;
+  return 0;
+}
+
+int DiffusionBatch_GetStatsArray(DiffusionBatch* B, int p, long* out, int nout)
+{
+  if (!B || p < 0 || p >= (int)B->probs.size()) return ARK_ILL_INPUT;
+  return ARKodeGetStatsArray(B->probs[p].A, out, nout);
+}
+
+int DiffusionBatch_GetNumGroups(DiffusionBatch* B, long* ngroups, long* nlaunches)
+{
+  if (!B) return ARK_MEM_NULL;
+  if (ngroups) *ngroups = B->ngroups;
+  if (nlaunches) *nlaunches = B->nlaunches;
+  return 0;
+}
+
+}  // extern "C"

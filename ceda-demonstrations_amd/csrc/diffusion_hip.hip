@@ -415,6 +415,175 @@ __global__ __launch_bounds__(kBlock) void k_stage1(int64_t n, const StageEntry* 
   if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) z[n - 1] = a0 * x[n - 1] + a1 * y[n - 1];
 }
 
+
+// ------------------------------------------------------------------ batch
+// Tile = 64*CPL columns x 4*rows rows: wavefront w of the workgroup takes rows
+// ty*4*rows + r*4 + w (r < rows), each lane CPL adjacent columns.  The same
+// cell(), forcing() and lc5() arithmetic as the single-problem kernels.
+template <int CPL, bool PER, bool MMS, int KIND>
+__device__ void batch_tile(const BatchDesc& D, const StageEntry* e, int tx, int ty)
+{
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int64_t nx = D.nx, ny = D.ny;
+  const int64_t i = (int64_t)tx * (kWave * CPL) + (int64_t)lane * CPL;
+  if (i >= nx) return;
+  const double* __restrict__ u = e->yjm1;
+  double* __restrict__ fout = e->out;
+  if (KIND == 1) {  // out = c0*yn + c1*fn
+    const double a0 = e->c[0], a1 = e->c[1];
+    for (int rr = 0; rr < D.rows; rr++) {
+      const int64_t j = (int64_t)ty * (4 * D.rows) + rr * 4 + wave;
+      if (j >= ny) break;
+      const int64_t o = j * nx + i;
+#pragma unroll
+      for (int c = 0; c < CPL; c++) fout[o + c] = a0 * e->yn[o + c] + a1 * e->fn[o + c];
+    }
+    return;
+  }
+  const double T1 = e->T1, T2 = e->T2;
+  double cc[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) cc[k] = e->c[k];
+  const double Dw0 = D.fx[i], De0 = D.fx[i + 1];
+  const double De1 = (CPL == 2) ? D.fx[i + 2] : 0.0;
+  const int64_t iw = (i > 0) ? i - 1 : (PER ? nx - 1 : -1);
+  const int64_t ie = (i + CPL < nx) ? i + CPL : (PER ? 0 : -1);
+  double sx0 = 0, sx1 = 0, cx0 = 0, cx1 = 0;
+  if (MMS) {
+    sx0 = D.sx[i];
+    cx0 = D.cx[i];
+    if (CPL == 2) {
+      sx1 = D.sx[i + 1];
+      cx1 = D.cx[i + 1];
+    }
+  }
+  for (int rr = 0; rr < D.rows; rr++) {
+    const int64_t j = (int64_t)ty * (4 * D.rows) + rr * 4 + wave;
+    if (j >= ny) break;
+    const int64_t js = (j > 0) ? j - 1 : (PER ? ny - 1 : -1);
+    const int64_t jn = (j < ny - 1) ? j + 1 : (PER ? 0 : -1);
+    const double Ds = D.fy[j], Dn = D.fy[j + 1];
+    const int64_t o = j * nx + i;
+    const double uw = (iw >= 0) ? u[j * nx + iw] : 0.0;
+    const double ue = (ie >= 0) ? u[j * nx + ie] : 0.0;
+    if (CPL == 2) {
+      const double2 C = *reinterpret_cast<const double2*>(u + o);
+      const double2 S = (js >= 0) ? *reinterpret_cast<const double2*>(u + js * nx + i) : make_double2(0.0, 0.0);
+      const double2 N = (jn >= 0) ? *reinterpret_cast<const double2*>(u + jn * nx + i) : make_double2(0.0, 0.0);
+      double f0 = cell(Dw0, De0, Ds, Dn, C.x, uw, C.y, S.x, N.x);
+      double f1 = cell(De0, De1, Ds, Dn, C.y, C.x, ue, S.y, N.y);
+      if (MMS) {
+        const double syj = D.sy[j], cyj = D.cy[j];
+        f0 = f0 + forcing(T1, T2, sx0, cx0, syj, cyj);
+        f1 = f1 + forcing(T1, T2, sx1, cx1, syj, cyj);
+      }
+      if (KIND == 2) {
+        const double2 A2 = *reinterpret_cast<const double2*>(e->yjm2 + o);
+        const double2 Y0 = *reinterpret_cast<const double2*>(e->yn + o);
+        const double2 G = *reinterpret_cast<const double2*>(e->fn + o);
+        f0 = lc5(cc, f0, A2.x, Y0.x, C.x, G.x);
+        f1 = lc5(cc, f1, A2.y, Y0.y, C.y, G.y);
+      }
+      *reinterpret_cast<double2*>(fout + o) = make_double2(f0, f1);
+    } else {
+      const double C = u[o];
+      const double S = (js >= 0) ? u[js * nx + i] : 0.0;
+      const double N = (jn >= 0) ? u[jn * nx + i] : 0.0;
+      double f0 = cell(Dw0, De0, Ds, Dn, C, uw, ue, S, N);
+      if (MMS) f0 = f0 + forcing(T1, T2, sx0, cx0, D.sy[j], D.cy[j]);
+      if (KIND == 2) f0 = lc5(cc, f0, e->yjm2[o], e->yn[o], C, e->fn[o]);
+      fout[o] = f0;
+    }
+  }
+}
+
+template <int CPL, bool PER, bool MMS>
+__device__ void batch_tile_kind(int kind, const BatchDesc& D, const StageEntry* e, int tx, int ty)
+{
+  if (kind == 1) batch_tile<CPL, PER, MMS, 1>(D, e, tx, ty);
+  else if (kind == 2) batch_tile<CPL, PER, MMS, 2>(D, e, tx, ty);
+  else batch_tile<CPL, PER, MMS, 3>(D, e, tx, ty);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_phase(const BatchDesc* __restrict__ desc,
+                                                        const BatchTile* __restrict__ tiles,
+                                                        const StageEntry* __restrict__ ent,
+                                                        int stride, const int* __restrict__ stages,
+                                                        int j)
+{
+  const BatchTile t = tiles[blockIdx.x];
+  const int s = stages[t.slot];
+  const int kind = (j == 1) ? 1 : (j <= s ? 2 : 3);
+  const BatchDesc D = desc[t.prob];
+  const StageEntry* e = ent + (int64_t)t.slot * stride + j;
+  if (D.cpl == 2) {
+    if (D.per) batch_tile_kind<2, true, false>(kind, D, e, t.tx, t.ty);
+    else batch_tile_kind<2, false, true>(kind, D, e, t.tx, t.ty);
+  } else {
+    if (D.per) batch_tile_kind<1, true, false>(kind, D, e, t.tx, t.ty);
+    else batch_tile_kind<1, false, true>(kind, D, e, t.tx, t.ty);
+  }
+}
+
+__device__ inline double sts_err_term(const BatchErr& E, int64_t o)
+{
+  const double y = E.yn[o];
+  double e = 0.8 * y;
+  e = e + -0.8 * E.ys[o];
+  e = e + E.ch * E.fn[o];
+  e = e + E.ch * E.fs[o];
+  const double w = 1.0 / (E.rtol * fabs(y) + E.atol);
+  return e * w;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_err(const BatchDesc* __restrict__ desc,
+                                                      const BatchTile* __restrict__ tiles,
+                                                      const BatchErr* __restrict__ err,
+                                                      double* __restrict__ partial)
+{
+  const BatchTile t = tiles[blockIdx.x];
+  const BatchDesc D = desc[t.prob];
+  const BatchErr E = err[t.slot];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  double acc = 0.0;
+  const int64_t i0 = (int64_t)t.tx * (kWave * D.cpl) + (int64_t)lane * D.cpl;
+  for (int rr = 0; rr < D.rows; rr++) {
+    const int64_t j = (int64_t)t.ty * (4 * D.rows) + rr * 4 + wave;
+    if (j >= D.ny) break;
+    for (int c = 0; c < D.cpl; c++) {
+      const int64_t i = i0 + c;
+      if (i < D.nx) {
+        const double p = sts_err_term(E, j * D.nx + i);
+        acc += p * p;
+      }
+    }
+  }
+  __shared__ double sh[kBlock / kWave];
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  if (lane == 0) sh[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_err_final(const double* __restrict__ partial,
+                                                            const int* __restrict__ begin,
+                                                            const int* __restrict__ count,
+                                                            double* __restrict__ dsq)
+{
+  const int slot = blockIdx.x;
+  const int b = begin[slot], c = count[slot];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < c; k += kBlock) acc += partial[b + k];
+  __shared__ double sh[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+  if (lane == 0) sh[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) dsq[slot] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
 __global__ __launch_bounds__(kBlock) void k_psetup(int64_t nx, int64_t ny, const double* ax,
                                                    const double* ay, double gamma, double* diag)
 {
@@ -498,6 +667,24 @@ void launch_fused_stage(DiffusionProblem* P, const double* coef, const double* y
   a.yn = yn;
   a.fn = fn;
   launch_stencil(P, a, true, s);
+}
+
+void launch_batch_phase(const BatchDesc* desc, const BatchTile* tiles, const StageEntry* ent,
+                        int stride, const int* stages, int j, int ntiles, hipStream_t s)
+{
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_batch_phase, dim3(ntiles), dim3(kBlock), 0, s, desc, tiles, ent, stride,
+                     stages, j);
+}
+
+void launch_batch_err(const BatchDesc* desc, const BatchTile* tiles, const BatchErr* err,
+                      int ntiles, double* partial, const int* tile_begin, const int* tile_count,
+                      int nslots, double* dsq, hipStream_t s)
+{
+  if (ntiles <= 0 || nslots <= 0) return;
+  hipLaunchKernelGGL(k_batch_err, dim3(ntiles), dim3(kBlock), 0, s, desc, tiles, err, partial);
+  hipLaunchKernelGGL(k_batch_err_final, dim3(nslots), dim3(kBlock), 0, s, partial, tile_begin,
+                     tile_count, dsq);
 }
 
 void launch_stage1_entry(DiffusionProblem* P, const StageEntry* e, hipStream_t s)

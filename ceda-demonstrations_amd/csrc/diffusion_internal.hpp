@@ -64,6 +64,30 @@ void launch_fused_entry(DiffusionProblem* P, const StageEntry* e, hipStream_t s)
 // out = F(yjm1) with the entry's time factors
 void launch_rhs_entry(DiffusionProblem* P, const StageEntry* e, hipStream_t s);
 
+// ---- batched super-time-stepping (many independent problems, one launch
+// per stage index) ----
+struct BatchDesc {           // per problem, static
+  const double *fx, *fy, *sx, *sy, *cx, *cy;
+  int64_t nx, ny;
+  int per, cpl, rows, tiles_x;
+};
+struct BatchTile {           // per tile of a step group, problems in slot order
+  int slot, prob, tx, ty;
+};
+struct BatchErr {            // per slot: operands of the embedded error norm
+  const double *yn, *ys, *fn, *fs;
+  double ch, rtol, atol, pad;
+};
+// phase j of a step group: for every tile t < ntiles, with s = stages[slot]:
+// j == 1 first stage, 2 <= j <= s fused stage, j == s + 1 the RHS at t_n + h;
+// entries of slot k start at ent + k*stride
+void launch_batch_phase(const BatchDesc* desc, const BatchTile* tiles, const StageEntry* ent,
+                        int stride, const int* stages, int j, int ntiles, hipStream_t s);
+// per-tile sums of (e*ewt)^2, then per-slot sums in tile order (deterministic)
+void launch_batch_err(const BatchDesc* desc, const BatchTile* tiles, const BatchErr* err,
+                      int ntiles, double* partial, const int* tile_begin, const int* tile_count,
+                      int nslots, double* dsq, hipStream_t s);
+
 // one STS stage of the LSRKStep recurrence, fused with the RHS evaluation:
 //   out = c0*F(yjm1) + c1*yjm2 + c2*yn + c3*yjm1 + c4*fn
 // coef points at 8 device doubles {c0..c4, T1, T2, unused}; F is the diffusion

@@ -1,0 +1,54 @@
+/* ---------------------------------------------------------------------------
+ * [ext] Batched super-time-stepping: many independent diffusion problems of
+ * uneven sizes and coefficients on one GPU, each with its own adaptive step
+ * size and stage count.
+ *
+ * There is no reference counterpart (the reference integrates one problem
+ * per run, diffusion_2D/main.cpp:431-470); each problem of a batch takes
+ * exactly the steps LSRKStepCreateSTS + ARKodeEvolve would take for it alone
+ * (same controller, stage-count and spectral-radius rules, closed-form
+ * dom_eig of main.cpp:536-550), but the stages of all problems run together:
+ * one kernel launch per stage index covers every problem still in its stage
+ * loop, one segmented reduction gives all error norms, and the host makes one
+ * device-to-host copy per step group.
+ * ------------------------------------------------------------------------- */
+#ifndef BATCH_HIP_H
+#define BATCH_HIP_H
+
+#include "arkode_hip.h"
+#include "diffusion_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct DiffusionBatch DiffusionBatch;
+
+/* nprob problems (params[k]); method ARKODE_LSRK_RKC_2 or ARKODE_LSRK_RKL_2;
+   every problem starts from Initial() at t = 0 */
+DiffusionBatch* DiffusionBatch_Create(int nprob, const DiffusionParams* params,
+                                      ARKODE_LSRKMethodType method, SUNContext ctx);
+void DiffusionBatch_Destroy(DiffusionBatch* B);
+int DiffusionBatch_SetTolerances(DiffusionBatch* B, sunrealtype rtol, sunrealtype atol);
+int DiffusionBatch_SetController(DiffusionBatch* B, int controller);    /* ARK_ADAPT_* */
+int DiffusionBatch_SetMaxNumStages(DiffusionBatch* B, int smax);
+int DiffusionBatch_SetMaxNumSteps(DiffusionBatch* B, long mxsteps);
+int DiffusionBatch_SetFixedStep(DiffusionBatch* B, sunrealtype h);
+/* tile height of the batched kernels: 4*rows rows x 64 or 128 columns per
+   workgroup (rows 1..16); 0 = choose per problem by the tile autotuner */
+int DiffusionBatch_SetTileRows(DiffusionBatch* B, int rows);
+/* advance every problem to tout; returns 0, or the first failing problem's
+   ARKODE code */
+int DiffusionBatch_Evolve(DiffusionBatch* B, sunrealtype tout);
+/* copy problem p's current state into y (a HIP-native vector of its length) */
+int DiffusionBatch_GetSolution(DiffusionBatch* B, int p, N_Vector y);
+int DiffusionBatch_GetCurrentTime(DiffusionBatch* B, int p, sunrealtype* t);
+/* same layout as ARKodeGetStatsArray */
+int DiffusionBatch_GetStatsArray(DiffusionBatch* B, int p, long* out, int nout);
+/* step groups and stage-phase launches issued so far */
+int DiffusionBatch_GetNumGroups(DiffusionBatch* B, long* ngroups, long* nlaunches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

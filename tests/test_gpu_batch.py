@@ -1,0 +1,124 @@
+"""Batched super-time-stepping on the GPU: every problem of an uneven batch
+(sizes, coefficients, kinds) must take the same steps as the CPU oracle
+integrating it alone, and reach the same state (1e-10 relative, FP64)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from hiplib import package
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pk():
+    return package()
+
+
+@pytest.fixture(scope="module")
+def ctx(pk):
+    c = pk.Context()
+    yield c
+    c.close()
+
+
+def problems(seed, n, sides, kind=1):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        side = int(rng.choice(sides))
+        kx, ky = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+        out.append(dict(kind=kind, nx=side, ny=side + int(rng.integers(0, 3)), kx=float(kx), ky=float(ky)))
+    return out
+
+
+def oracle_run(prm, method, tout, rtol, atol, fixed=0.0, controller=0, touts=None):
+    kw = {k: v for k, v in prm.items() if k != "kind"}
+    P = O.OracleProblem(prm["kind"], **kw)
+    R = O.OracleIntegrator(P, O.RKC if method == "rkc" else O.RKL, P.initial())
+    R.set_tolerances(rtol, atol)
+    R.set_max_num_steps(100000)
+    if fixed > 0:
+        R.set_fixed_step(fixed)
+    else:
+        R.set_controller(controller)
+    for tt in (touts or [tout]):
+        flag, t, y = R.evolve(tt)
+    return flag, t, y, R.stats()
+
+
+@pytest.mark.parametrize("method", ["rkc", "rkl"])
+def test_batch_matches_oracle_per_problem(pk, ctx, method):
+    plist = problems(11, 10, [16, 24, 33, 48, 64, 96])
+    B = pk.Batch(ctx, plist, method)
+    B.SetTolerances(1e-5, 1e-10)
+    B.SetMaxNumSteps(100000)
+    for tout in (0.05, 0.1):
+        assert B.evolve(tout) == 0
+    g, launches = B.groups()
+    assert g > 0 and launches > g
+    for k, prm in enumerate(plist):
+        flag, t, yr, sr = oracle_run(prm, method, 0.1, 1e-5, 1e-10, touts=[0.05, 0.1])
+        assert flag == 0 and B.time(k) == t
+        sg = B.stats(k)
+        for f in ("nst", "attempts", "nfe", "netf", "nstlim", "stage_max"):
+            assert sg[f] == sr[f], (k, f, sg[f], sr[f])
+        yg = B.solution(k)
+        assert np.abs(yg - yr).max() <= 1e-10 * np.abs(yr).max(), k
+    B.release()
+
+
+def test_batch_periodic_and_controllers(pk, ctx):
+    plist = problems(5, 6, [20, 31, 40], kind=0)
+    for controller in (1, 2):
+        B = pk.Batch(ctx, plist, "rkl")
+        B.SetTolerances(1e-4, 1e-11)
+        B.SetController(controller)
+        B.SetMaxNumSteps(100000)
+        assert B.evolve(0.2) == 0
+        for k, prm in enumerate(plist):
+            flag, t, yr, sr = oracle_run(prm, "rkl", 0.2, 1e-4, 1e-11, controller=controller)
+            assert B.stats(k)["nst"] == sr["nst"]
+            yg = B.solution(k)
+            assert np.abs(yg - yr).max() <= 1e-10 * np.abs(yr).max()
+        B.release()
+
+
+@pytest.mark.parametrize("rows", [1, 4, 8])
+def test_tile_heights_agree(pk, ctx, rows):
+    plist = problems(3, 5, [32, 64, 128])
+    ref = pk.Batch(ctx, plist, "rkc")
+    ref.SetTolerances(1e-5, 1e-10)
+    ref.SetTileRows(2)
+    assert ref.evolve(0.05) == 0
+    B = pk.Batch(ctx, plist, "rkc")
+    B.SetTolerances(1e-5, 1e-10)
+    B.SetTileRows(rows)
+    assert B.evolve(0.05) == 0
+    for k in range(len(plist)):
+        a, b = ref.solution(k), B.solution(k)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    ref.release()
+    B.release()
+
+
+def test_batch_fixed_step(pk, ctx):
+    plist = problems(8, 4, [40, 64])
+    B = pk.Batch(ctx, plist, "rkl")
+    B.SetFixedStep(2e-4)
+    B.SetMaxNumSteps(100000)
+    assert B.evolve(0.01) == 0
+    for k, prm in enumerate(plist):
+        flag, t, yr, sr = oracle_run(prm, "rkl", 0.01, 1e-5, 1e-10, fixed=2e-4)
+        assert B.stats(k)["nfe"] == sr["nfe"]
+        yg = B.solution(k)
+        assert np.abs(yg - yr).max() <= 1e-10 * np.abs(yr).max()
+    B.release()
+
+
+def test_batch_step_cap_reports_too_much_work(pk, ctx):
+    B = pk.Batch(ctx, problems(1, 3, [64]), "rkc")
+    B.SetTolerances(1e-8, 1e-12)
+    B.SetMaxNumSteps(3)
+    assert B.evolve(0.5) == -1
+    B.release()
