@@ -15,9 +15,12 @@ evaluation (stages, error estimate, power-iteration J v products) times the
 cells.
 
 Extra objects on the JSON line:
-  roofline      the fused stage kernel (the dominant kernel: >98% of the
-                step's HBM traffic), 40 algorithmic bytes per cell per launch,
-                mean launch time from HIP events on the integrator's stream
+  roofline      the dominant kernel (>97% of the step's HBM traffic): the
+                multi-stage pass (K stages per launch, 48 algorithmic bytes per
+                cell: Y_{j-1}, Y_{j-2}, Y_n, F_n read, the last two stages
+                written) when the autotuner enables it, else the single fused
+                stage (40 bytes per cell); mean launch time from HIP events on
+                the integrator's stream
   cpu_baseline  the CPU oracle (serial C restatement, 1 thread) timed on one
                 full adaptive step of the same workload
 """
@@ -45,10 +48,10 @@ def hip_sync():
     C.CDLL("libamdhip64.so").hipDeviceSynchronize()
 
 
-def read_traffic():
-    """HBM bytes per fused-stage launch from the committed rocprofv3 PMC
+def read_traffic(name):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
-    path = os.path.join(ROOT, "profiles", "fused_stage_4096_pmc.txt")
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -79,29 +82,45 @@ def roofline(pk, ctx, G):
     rc = L.DiffusionProblem_TimeKernel(G.h, 1, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, 50,
                                        C.byref(avg))
     assert rc == 0
-    avg_s = avg.value * 1e-3
-    bytes_alg = 40.0 * G.n
-    achieved = bytes_alg / avg_s / 1e9
+    stage_ms = avg.value
+    stage_gbs = 40.0 * G.n / (stage_ms * 1e-3) / 1e9
     rc = L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, 50, C.byref(avg))
     stencil_gbs = 16.0 * G.n / (avg.value * 1e-3) / 1e9
+    K, rows, nt = G.get_config(2)
+    if K >= 2:
+        rc = L.DiffusionProblem_TimeKernel(G.h, 2, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, 50,
+                                           C.byref(avg))
+        assert rc == 0
+        launch_ms = avg.value
+        bytes_alg = 48.0 * G.n
+        kname = (f"k_pass<2,false,true,{K},{'true' if nt else 'false'}> rows={rows} "
+                 f"({K} fused STS stages per launch)")
+        traffic = read_traffic(f"stage_pass_k{K}_4096_pmc.txt")
+    else:
+        launch_ms = stage_ms
+        bytes_alg = 40.0 * G.n
+        v, r, n1 = G.get_config(1)
+        kname = (f"k_stencil<2,false,true,true> rows={r}" if v == 0 else
+                 f"k_march<2,false,true,true,{'true' if n1 else 'false'}> rows={r}") + " (fused STS stage)"
+        traffic = read_traffic("fused_stage_4096_pmc.txt")
     for v in vs:
         v.destroy()
-    v, r, nt = G.get_config(1)
-    kname = (f"k_stencil<2,false,true,true> rows={r}" if v == 0 else
-             f"k_march<2,false,true,true,{'true' if nt else 'false'}> rows={r}")
-    return {"kernel": kname + " (fused STS stage)", "bound": "hbm",
+    achieved = bytes_alg / (launch_ms * 1e-3) / 1e9
+    return {"kernel": kname, "bound": "hbm",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
-            "bytes_per_launch_algorithmic": bytes_alg, "launch_ms": round(avg_s * 1e3, 4),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_launch_algorithmic": bytes_alg, "launch_ms": round(launch_ms, 4),
+            "stages_per_launch": max(K, 1),
+            "single_stage_fused_ms": round(stage_ms, 4), "single_stage_fused_GBs": round(stage_gbs, 1),
             "rhs_stencil_GBs": round(stencil_gbs, 1)}
 
 
 def kernels_only(pk, reps, sweep):
     """Launch the hot-path kernels alone at the workload's size (the workload
-    of the rocprofv3 counter runs): the autotuned fused STS stage and RHS
-    stencil, and a plain copy (N_VScale by 1: 8 B read + 8 B written per
-    element) whose counters calibrate FETCH_SIZE/WRITE_SIZE.  With sweep,
-    first time every candidate launch shape of both kernels."""
+    of the rocprofv3 counter runs): the autotuned multi-stage pass, fused
+    STS stage and RHS stencil, and a plain copy (N_VScale by 1: 8 B read +
+    8 B written per element) whose counters calibrate FETCH_SIZE/WRITE_SIZE.
+    With sweep, first time every candidate launch shape of the kernels."""
     ctx = pk.Context()
     G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
     L = pk.lib()
@@ -120,6 +139,13 @@ def kernels_only(pk, reps, sweep):
             res[f"fused_v{v}_r{r}_nt{nt}_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
             L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
             res[f"rhs_v{v}_r{r}_nt{nt}_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+        for K in (2, 3, 4):
+            for r in (8, 16, 32, 64, 128):
+                for nt in (0, 1):
+                    G.set_config(2, K, r, nt)
+                    L.DiffusionProblem_TimeKernel(G.h, 2, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h,
+                                                  reps, C.byref(avg))
+                    res[f"pass_K{K}_r{r}_nt{nt}_us_per_stage"] = round(avg.value * 1e3 / K, 2)
     G.autotune(3)
     res["autotuned_rhs"] = G.get_config(0)
     res["autotuned_fused"] = G.get_config(1)
@@ -127,6 +153,13 @@ def kernels_only(pk, reps, sweep):
     res["tuned_fused_GBs"] = round(40.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
     L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
     res["tuned_rhs_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+    res["autotuned_pass"] = G.get_config(2)
+    if G.get_config(2)[0] >= 2:
+        L.DiffusionProblem_TimeKernel(G.h, 2, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, reps,
+                                      C.byref(avg))
+        res["tuned_pass_ms"] = round(avg.value, 4)
+        res["tuned_pass_GBs"] = round(48.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
+        res["tuned_pass_us_per_stage"] = round(avg.value * 1e3 / G.get_config(2)[0], 2)
     hip_sync()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -283,7 +316,7 @@ def main():
     value = nfe * G.n / dt
     log(f"{nfe} RHS evals in {dt:.3f} s; stages/step {nfe / args.steps:.1f}; t = {t:.4e}")
     rl = roofline(pk, ctx, G)
-    log(f"fused stage kernel {rl['launch_ms']} ms -> {rl['achieved']} GB/s")
+    log(f"dominant kernel {rl['kernel']}: {rl['launch_ms']} ms -> {rl['achieved']} GB/s")
     I.free()
     G.destroy()
     cpu = None

@@ -85,6 +85,7 @@ _SIGS = {
     "DiffusionProblem_SetStencilConfig": (i32, [vp, i32, i32, i32, i32]),
     "DiffusionProblem_GetStencilConfig": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     "DiffusionProblem_Autotune": (i32, [vp, i32]),
+    "DiffusionProblem_StagePass": (i32, [vp, i32, dp, vp, vp, vp, vp, vp, vp]),
     "Initial": (i32, [d, vp, vp]),
     "Exact": (i32, [d, vp, vp]),
     "laplacian": (i32, [d, vp, vp, vp]),
@@ -145,6 +146,7 @@ _SIGS = {
     "DiffusionBatch_SetMaxNumSteps": (i32, [vp, C.c_long]),
     "DiffusionBatch_SetFixedStep": (i32, [vp, d]),
     "DiffusionBatch_SetTileRows": (i32, [vp, i32]),
+    "DiffusionBatch_SetStagesPerPass": (i32, [vp, i32]),
     "DiffusionBatch_Evolve": (i32, [vp, d]),
     "DiffusionBatch_GetSolution": (i32, [vp, i32, vp]),
     "DiffusionBatch_GetCurrentTime": (i32, [vp, i32, dp]),
@@ -333,6 +335,15 @@ class Problem:
 
     def set_config(self, which, variant, rows, nt=0):
         return lib().DiffusionProblem_SetStencilConfig(self.h, which, variant, rows, nt)
+
+    def stage_pass(self, K, coef, yjm1, yjm2, yn, fn, outkm1, outk):
+        """K fused STS stages in one multi-stage pass; coef is K rows of
+        {c0..c4, T1, T2, unused}."""
+        import numpy as np
+        cf = np.ascontiguousarray(np.asarray(coef, dtype=np.float64).reshape(-1))
+        return lib().DiffusionProblem_StagePass(
+            self.h, K, cf.ctypes.data_as(C.POINTER(C.c_double)), yjm1.h, yjm2.h, yn.h, fn.h,
+            outkm1.h if outkm1 is not None else None, outk.h)
 
     def get_config(self, which):
         v, r, n = i32(), i32(), i32()

@@ -59,6 +59,7 @@ struct ArkMem {
   long mxstep = 500;
   N_Vector yn = nullptr, ycur = nullptr, fn = nullptr, tv1 = nullptr, tv2 = nullptr,
            ewt = nullptr;
+  N_Vector tv3 = nullptr;  // fourth scratch buffer of the multi-stage fused path
   int fn_is_current = 0, ewt_current = 0;
   // controller
   int ctype = ARK_ADAPT_PID, p_order = 2, firststeps = 0;
@@ -81,7 +82,7 @@ struct ArkMem {
   StageEntry* d_tab = nullptr;
   hipEvent_t tab_ev = nullptr;
   int tab_cap = 0;
-  std::map<int, hipGraphExec_t> graphs;
+  std::map<long, hipGraphExec_t> graphs;
   // DIRK
   int order = 3, stages = 0, linear = 1, msbp = 20, psetup_done = 0, maxcor = 3;
   double A[16], b[4], bh[4], c[4];
@@ -117,7 +118,11 @@ int lsrk_plan(ArkMem* A, StagePlan& SP);
 void lsrk_after(ArkMem* A, double dsm);
 // fills table entries 1..s+1 for one s-stage attempt and assigns the buffer
 // roles exactly as the N_Vector path does (Y_s ends in ycur, F_s in tv2)
-void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T);
+// stage table of one attempt for passes of up to kp stages (kp = 1: one
+// kernel per stage)
+void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T, int kp);
+// stages per pass of a multi-stage configuration (1 when disabled)
+inline int pass_stages(const StencilCfg& c) { return c.variant >= 2 ? c.variant : 1; }
 void attempt_begin(ArkMem* A, double tout, AttemptState& S);
 int attempt_end(ArkMem* A, int ret, double dsm, AttemptState& S);
 int step_complete(ArkMem* A, double dsm, double tout, const AttemptState& S);

@@ -459,54 +459,76 @@ void ensure_table(ArkMem* A, int need)
   memset(A->h_tab, 0, sizeof(StageEntry) * need);
 }
 
-void issue_stage_kernels(ArkMem* A, int s, hipStream_t st)
+void issue_stage_kernels(ArkMem* A, int s, const StencilCfg& pc, hipStream_t st)
 {
   DiffusionProblem* P = A->fast;
+  const int kp = pass_stages(pc);
   ceda::launch_stage1_entry(P, A->d_tab + 1, st);
-  for (int j = 2; j <= s; j++) ceda::launch_fused_entry(P, A->d_tab + j, st);
+  for (int j = 2; j <= s;) {
+    const int K = std::min(kp, s - j + 1);
+    if (K == 1) ceda::launch_fused_entry(P, A->d_tab + j, st);
+    else ceda::launch_pass(P, K, pc.rows, pc.nt, A->d_tab + j, st);
+    j += K;
+  }
   if (!A->fixedstep) ceda::launch_rhs_entry(P, A->d_tab + s + 1, st);
 }
 
-// the fused path: the same arithmetic, one kernel per stage, operands and
-// coefficients from the stage table, optionally one graph launch per step
-void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T)
+// the fused path: the same arithmetic, one kernel per stage or per pass of up
+// to kp stages, operands and coefficients from the stage table, optionally
+// one graph launch per step.  A pass reads Y_{j-1}, Y_{j-2} and writes its
+// last two stages into two other buffers (never one it reads), so the
+// scratch rotates over four buffers; with kp = 1 it is the usual three-term
+// rotation.
+void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T, int kp)
 {
   const int s = SP.s;
   DiffusionProblem* P = A->fast;
   const bool mms = (P->p.kind == DIFFUSION_MMS_DIRICHLET);
-  // roles: Y_{j-2} starts as yn itself (the API path copies yn into tv2 first;
-  // reading yn directly gives the same values), scratch rotates over 3 buffers
-  N_Vector Yjm2 = A->yn, Yjm1 = A->tv1, Yj = A->ycur, spare = A->tv2;
+  N_Vector bufs[4] = {A->tv1, A->ycur, A->tv2, A->tv3};
+  // Y_{j-2} starts as yn itself (the API path copies yn into tv2 first;
+  // reading yn directly gives the same values)
+  N_Vector Yjm2 = A->yn, Yjm1 = A->tv1;
   T[1].c[0] = 1.0;
   T[1].c[1] = SP.mus1;
   T[1].yn = N_VGetDeviceArrayPointer(A->yn);
   T[1].fn = N_VGetDeviceArrayPointer(A->fn);
   T[1].out = N_VGetDeviceArrayPointer(Yjm1);
-  for (int j = 2; j <= s; j++) {
-    StageEntry& e = T[j];
-    for (int k = 0; k < 5; k++) e.c[k] = SP.c[5 * j + k];
-    e.T1 = e.T2 = 0.0;
-    if (mms) ceda::mms_time_factors(SP.t[j], &e.T1, &e.T2);
-    e.yjm1 = N_VGetDeviceArrayPointer(Yjm1);
-    e.yjm2 = N_VGetDeviceArrayPointer(Yjm2);
-    e.yn = N_VGetDeviceArrayPointer(A->yn);
-    e.fn = N_VGetDeviceArrayPointer(A->fn);
-    e.out = N_VGetDeviceArrayPointer(Yj);
-    if (j < s) {
-      N_Vector freed = (Yjm2 == A->yn) ? spare : Yjm2;
-      if (Yjm2 == A->yn) spare = nullptr;
-      Yjm2 = Yjm1; Yjm1 = Yj; Yj = freed;
+  for (int j = 2; j <= s;) {
+    const int K = std::min(std::max(kp, 1), s - j + 1);
+    N_Vector fr[3];
+    int nf = 0;
+    for (N_Vector v : bufs)
+      if (v != Yjm1 && v != Yjm2) fr[nf++] = v;
+    for (int m = 0; m < K; m++) {
+      StageEntry& e = T[j + m];
+      for (int k = 0; k < 5; k++) e.c[k] = SP.c[5 * (j + m) + k];
+      e.T1 = e.T2 = 0.0;
+      if (mms) ceda::mms_time_factors(SP.t[j + m], &e.T1, &e.T2);
+      e.yjm1 = N_VGetDeviceArrayPointer(Yjm1);
+      e.yjm2 = N_VGetDeviceArrayPointer(Yjm2);
+      e.yn = N_VGetDeviceArrayPointer(A->yn);
+      e.fn = N_VGetDeviceArrayPointer(A->fn);
+      e.out = N_VGetDeviceArrayPointer(fr[1]);
     }
+    if (K >= 2) {
+      T[j + K - 2].out = N_VGetDeviceArrayPointer(fr[0]);
+      Yjm2 = fr[0];
+      Yjm1 = fr[1];
+    } else {
+      Yjm2 = Yjm1;
+      Yjm1 = fr[1];
+    }
+    j += K;
   }
-  // Y_s is in Yj; the other two scratch buffers are free
-  N_Vector scratch[3] = {A->tv1, A->ycur, A->tv2};
-  N_Vector others[2];
+  // Y_s is in Yjm1; the other three buffers are free
+  N_Vector others[3];
   int no = 0;
-  for (N_Vector v : scratch)
-    if (v != Yj) others[no++] = v;
-  A->ycur = Yj;
+  for (N_Vector v : bufs)
+    if (v != Yjm1) others[no++] = v;
+  A->ycur = Yjm1;
   A->tv1 = others[0];
   A->tv2 = others[1];
+  A->tv3 = others[2];
   StageEntry& ef = T[s + 1];
   ef.T1 = ef.T2 = 0.0;
   if (mms) ceda::mms_time_factors(A->tn + A->h, &ef.T1, &ef.T2);
@@ -514,8 +536,6 @@ void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T)
   ef.out = N_VGetDeviceArrayPointer(A->tv2);
 }
 
-// the fused path: the same arithmetic, one kernel per stage, operands and
-// coefficients from the stage table, optionally one graph launch per step
 int stages_fused(ArkMem* A, const StagePlan& SP)
 {
   const int s = SP.s;
@@ -523,24 +543,27 @@ int stages_fused(ArkMem* A, const StagePlan& SP)
   hipStream_t st = P->exec->stream;
   ensure_table(A, A->smax + 2);
   CEDA_HIP_CHECK(hipEventSynchronize(A->tab_ev));  // previous copy consumed h_tab
-  fused_fill(A, SP, A->h_tab);
+  const StencilCfg pc = P->cfg[2];
+  fused_fill(A, SP, A->h_tab, pass_stages(pc));
   CEDA_HIP_CHECK(hipMemcpyAsync(A->d_tab, A->h_tab, sizeof(StageEntry) * (s + 2),
                                 hipMemcpyHostToDevice, st));
   CEDA_HIP_CHECK(hipEventRecord(A->tab_ev, st));
   if (A->use_graphs) {
-    auto it = A->graphs.find(s);
+    // the graph depends on s and on the pass configuration only
+    const long key = (long)s + 1024L * (pass_stages(pc) + 8L * (pc.nt + 2L * pc.rows));
+    auto it = A->graphs.find(key);
     if (it == A->graphs.end()) {
       hipGraph_t g;
       CEDA_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      issue_stage_kernels(A, s, st);
+      issue_stage_kernels(A, s, pc, st);
       CEDA_HIP_CHECK(hipStreamEndCapture(st, &g));
       hipGraphExec_t ge;
       CEDA_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
       CEDA_HIP_CHECK(hipGraphDestroy(g));
-      it = A->graphs.emplace(s, ge).first;
+      it = A->graphs.emplace(key, ge).first;
     }
     CEDA_HIP_CHECK(hipGraphLaunch(it->second, st));
-  } else issue_stage_kernels(A, s, st);
+  } else issue_stage_kernels(A, s, pc, st);
   CEDA_HIP_CHECK(hipGetLastError());
   const long nrhs = (s - 1) + (A->fixedstep ? 0 : 1);
   A->nfe += nrhs;
@@ -995,7 +1018,7 @@ void ARKodeFree(void** mem)
 {
   if (!mem || !*mem) return;
   ArkMem* A = AM(*mem);
-  N_Vector vs[] = {A->yn, A->ycur, A->fn, A->tv1, A->tv2, A->ewt, A->sdata, A->zcur, A->fz,
+  N_Vector vs[] = {A->yn, A->ycur, A->fn, A->tv1, A->tv2, A->tv3, A->ewt, A->sdata, A->zcur, A->fz,
                    A->rhsv, A->dz, A->jwork, A->jv, A->F[0], A->F[1], A->F[2], A->F[3]};
   for (N_Vector v : vs)
     if (v) N_VDestroy(v);
@@ -1104,6 +1127,7 @@ int LSRKStepSetDiffusionFastPath(void* m, DiffusionProblem* prob, int use_graphs
   if (prob->n != A->n || A->yn->ops->nvclone != N_VClone_HipNative) return ARK_ILL_INPUT;
   A->fast = prob;
   A->use_graphs = use_graphs ? 1 : 0;
+  if (!A->tv3) A->tv3 = N_VClone(A->yn);
   if (!prob->tuned) DiffusionProblem_Autotune(prob, 3);
   if (!A->tab_ev) {
     CEDA_HIP_CHECK(hipEventCreateWithFlags(&A->tab_ev, hipEventDisableTiming));

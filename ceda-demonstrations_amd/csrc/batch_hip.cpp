@@ -7,12 +7,14 @@
 // step_complete are shared).  What changes is the execution of the stage
 // loops: in a step group every active problem plans one attempt, the planned
 // problems are sorted by stage count (descending) and their tiles are listed
-// in that order, so the tiles still in their stage loop at stage index j are
-// a prefix of the list; phase j is one launch over that prefix.  A segmented
-// reduction then produces every problem's error norm (one device-to-host copy
-// per group).  Phase launches of small problems, which would each be limited
-// by launch latency when the problems are solved one after another, are
-// amortised over the whole batch.
+// in that order, so the tiles still in their stage loop at a given pass index
+// are a prefix of the list; a phase is one launch over that prefix.  Each
+// phase runs up to kpass stages of every problem in it (the multi-stage pass
+// kernel: intermediate stages stay in registers), or its first stage, or its
+// closing RHS.  A segmented reduction then produces every problem's error
+// norm (one device-to-host copy per group).  Phase launches of small
+// problems, which would each be limited by launch latency when the problems
+// are solved one after another, are amortised over the whole batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,11 +42,11 @@ struct BProb {
   long nst0 = 0;
   bool done = false, new_step = true;
   int err = 0;
-  int rows = 0, cpl = 1, tiles_x = 0, tiles_y = 0;
+  int rows = 0, cpl = 1, strips = 0, rbs = 0;
   StagePlan SP;
 };
 
-int tiles_of(const BProb& b) { return b.tiles_x * b.tiles_y; }
+int tiles_of(const BProb& b) { return b.strips * b.rbs; }
 
 }  // namespace
 
@@ -55,6 +57,7 @@ struct DiffusionBatch {
   int method = M_RKC;
   int smax = 200, stride = 202;
   int tile_rows = 0;
+  int kpass = ceda::kBatchMaxK;
   bool tuned = false;
   long ngroups = 0, nlaunches = 0;
   // device/pinned staging (grown on demand)
@@ -122,13 +125,21 @@ void ensure_staging(DiffusionBatch* B, int slots, int tiles)
   host_alloc(&B->h_dsq, ns);
 }
 
+// wavefront tiles: strips of (64 - 2L)*cpl owned columns, `rows` rows
 void set_geometry(BProb& b, int rows)
 {
   const DiffusionParams& p = b.P->p;
   b.cpl = (p.nx % 2 == 0) ? 2 : 1;
   b.rows = rows;
-  b.tiles_x = (int)((p.nx + 64 * b.cpl - 1) / (64 * b.cpl));
-  b.tiles_y = (int)((p.ny + 4 * rows - 1) / (4 * rows));
+  const int64_t ow = (int64_t)(64 - 2 * ceda::batch_halo_lanes(b.cpl)) * b.cpl;
+  b.strips = (int)((p.nx + ow - 1) / ow);
+  b.rbs = (int)((p.ny + rows - 1) / rows);
+}
+
+void list_tiles(const BProb& b, int slot, int prob, BatchTile* out, int& t)
+{
+  for (int rb = 0; rb < b.rbs; rb++)
+    for (int x = 0; x < b.strips; x++) out[t++] = BatchTile{slot, prob, x, rb};
 }
 
 void upload_desc(DiffusionBatch* B)
@@ -138,14 +149,15 @@ void upload_desc(DiffusionBatch* B)
     const BProb& b = B->probs[k];
     const DiffusionProblem* P = b.P;
     h[k] = BatchDesc{P->d_fx, P->d_fy, P->d_sx, P->d_sy, P->d_cx, P->d_cy, P->p.nx, P->p.ny,
-                     P->p.kind == DIFFUSION_CEDA_PERIODIC ? 1 : 0, b.cpl, b.rows, b.tiles_x};
+                     P->p.kind == DIFFUSION_CEDA_PERIODIC ? 1 : 0, b.cpl, b.rows, b.strips};
   }
   if (!B->d_desc) dev_alloc(&B->d_desc, h.size());
   CEDA_HIP_CHECK(hipMemcpy(B->d_desc, h.data(), sizeof(BatchDesc) * h.size(), hipMemcpyHostToDevice));
 }
 
-// the tile autotuner: for every size class (nx, ny) time the fused-stage phase
-// over that class's tiles for each candidate height and keep the fastest
+// the tile autotuner: for every size class (nx, ny) time a full-length pass
+// phase over that class's tiles for each candidate height and keep the
+// fastest
 void autotune_tiles(DiffusionBatch* B)
 {
   std::map<std::pair<int64_t, int64_t>, std::vector<int>> classes;
@@ -155,11 +167,13 @@ void autotune_tiles(DiffusionBatch* B)
   hipEvent_t e0, e1;
   CEDA_HIP_CHECK(hipEventCreate(&e0));
   CEDA_HIP_CHECK(hipEventCreate(&e1));
+  const int K = B->kpass;
   for (auto& cl : classes) {
     const std::vector<int>& ks = cl.second;
-    int best_rows = 2;
+    int best_rows = 16;
     float best = 1e30f;
-    for (int rows : {1, 2, 4, 8}) {
+    for (int rows : {4, 8, 16, 32, 64}) {
+      if (rows > 2 * cl.first.second) continue;
       for (int k : ks) set_geometry(B->probs[k], rows);
       upload_desc(B);
       int nt = 0;
@@ -168,26 +182,26 @@ void autotune_tiles(DiffusionBatch* B)
       int t = 0;
       for (size_t q = 0; q < ks.size(); q++) {
         BProb& b = B->probs[ks[q]];
-        StageEntry& e = B->h_ent[q * B->stride + 2];
-        memset(&e, 0, sizeof(e));
-        e.yjm1 = N_VGetDeviceArrayPointer(b.A->yn);
-        e.yjm2 = N_VGetDeviceArrayPointer(b.A->yn);
-        e.yn = N_VGetDeviceArrayPointer(b.A->yn);
-        e.fn = N_VGetDeviceArrayPointer(b.A->yn);
-        e.out = N_VGetDeviceArrayPointer(b.A->tv1);  // scratch at a step boundary
-        B->h_s[q] = 3;
-        for (int ty = 0; ty < b.tiles_y; ty++)
-          for (int tx = 0; tx < b.tiles_x; tx++) B->h_tiles[t++] = BatchTile{(int)q, ks[q], tx, ty};
+        // stages 2..K+1 read yn only and write tv1 (stage K) and tv2
+        // (stage K+1); scratch contents at a step boundary
+        for (int j = 2; j <= K + 1; j++) {
+          StageEntry& e = B->h_ent[q * B->stride + j];
+          memset(&e, 0, sizeof(e));
+          e.yjm1 = e.yjm2 = e.yn = e.fn = N_VGetDeviceArrayPointer(b.A->yn);
+          e.out = N_VGetDeviceArrayPointer(j == K + 1 ? b.A->tv2 : b.A->tv1);
+        }
+        B->h_s[q] = K + 1;
+        list_tiles(b, (int)q, ks[q], B->h_tiles, t);
       }
       CEDA_HIP_CHECK(hipMemcpyAsync(B->d_ent, B->h_ent, sizeof(StageEntry) * ks.size() * B->stride,
                                     hipMemcpyHostToDevice, s));
       CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(BatchTile) * nt,
                                     hipMemcpyHostToDevice, s));
       CEDA_HIP_CHECK(hipMemcpyAsync(B->d_s, B->h_s, sizeof(int) * ks.size(), hipMemcpyHostToDevice, s));
-      ceda::launch_batch_phase(B->d_desc, B->d_tiles, B->d_ent, B->stride, B->d_s, 2, nt, s);
+      ceda::launch_batch_phase(B->d_desc, B->d_tiles, nt, B->d_ent, B->stride, B->d_s, 1, K, 0, s);
       CEDA_HIP_CHECK(hipEventRecord(e0, s));
       for (int r = 0; r < 3; r++)
-        ceda::launch_batch_phase(B->d_desc, B->d_tiles, B->d_ent, B->stride, B->d_s, 2, nt, s);
+        ceda::launch_batch_phase(B->d_desc, B->d_tiles, nt, B->d_ent, B->stride, B->d_s, 1, K, 0, s);
       CEDA_HIP_CHECK(hipEventRecord(e1, s));
       CEDA_HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.0f;
@@ -252,23 +266,21 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
   for (int k : planned) ntiles += tiles_of(B->probs[k]);
   const int nslots = (int)planned.size();
   ensure_staging(B, nslots, ntiles);
-  int t = 0, jmax = 0;
+  int t = 0;
   std::vector<int> tile_end(nslots);
   for (int q = 0; q < nslots; q++) {
     BProb& b = B->probs[planned[q]];
     ArkMem* A = b.A;
     StageEntry* T = B->h_ent + (size_t)q * B->stride;
-    fused_fill(A, b.SP, T);
+    fused_fill(A, b.SP, T, B->kpass);
     B->h_s[q] = b.SP.s;
     B->h_tb[q] = t;
-    for (int ty = 0; ty < b.tiles_y; ty++)
-      for (int tx = 0; tx < b.tiles_x; tx++) B->h_tiles[t++] = BatchTile{q, planned[q], tx, ty};
+    list_tiles(b, q, planned[q], B->h_tiles, t);
     B->h_tc[q] = t - B->h_tb[q];
     tile_end[q] = t;
     B->h_err[q] = BatchErr{N_VGetDeviceArrayPointer(A->yn), N_VGetDeviceArrayPointer(A->ycur),
                            N_VGetDeviceArrayPointer(A->fn), N_VGetDeviceArrayPointer(A->tv2),
                            0.4 * A->h, A->rtol, A->atol, 0.0};
-    jmax = std::max(jmax, b.SP.s + (adaptive ? 1 : 0));
   }
   hipStream_t s = B->ex->stream;
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_ent, B->h_ent, sizeof(StageEntry) * nslots * B->stride,
@@ -279,13 +291,21 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_err, B->h_err, sizeof(BatchErr) * nslots, hipMemcpyHostToDevice, s));
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tb, B->h_tb, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tc, B->h_tc, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
-  // phase j covers the slots with s + (adaptive) >= j: a prefix of the slots
+  // phase 0: stage 1 of every slot; phase q >= 1 starts at stage
+  // j = 2 + (q-1)*kpass and covers the slots with s >= j (their next pass) or,
+  // adaptive, s >= j - kpass (their closing RHS): a prefix of the slots
+  const int kp = B->kpass;
+  ceda::launch_batch_phase(B->d_desc, B->d_tiles, ntiles, B->d_ent, B->stride, B->d_s, 0, kp,
+                           adaptive ? 1 : 0, s);
+  B->nlaunches++;
   int active = nslots;
-  for (int j = 1; j <= jmax; j++) {
-    while (active > 0 && B->h_s[active - 1] + (adaptive ? 1 : 0) < j) active--;
+  for (int q = 1;; q++) {
+    const int j = 2 + (q - 1) * kp;
+    const int need = adaptive ? j - kp : j;
+    while (active > 0 && B->h_s[active - 1] < need) active--;
     if (active == 0) break;
-    ceda::launch_batch_phase(B->d_desc, B->d_tiles, B->d_ent, B->stride, B->d_s, j,
-                             tile_end[active - 1], s);
+    ceda::launch_batch_phase(B->d_desc, B->d_tiles, tile_end[active - 1], B->d_ent, B->stride,
+                             B->d_s, q, kp, adaptive ? 1 : 0, s);
     B->nlaunches++;
   }
   if (adaptive) {
@@ -349,7 +369,8 @@ DiffusionBatch* DiffusionBatch_Create(int nprob, const DiffusionParams* params,
     ARKodeSetUserData(mem, b.P);
     LSRKStepSetDomEigFn(mem, dom_eig);
     b.A->fast = b.P;  // error weights only when needed, as in the fused path
-    set_geometry(b, 2);
+    b.A->tv3 = N_VClone(b.A->yn);
+    set_geometry(b, 16);
   }
   upload_desc(B);
   return B;
@@ -420,13 +441,21 @@ int DiffusionBatch_SetFixedStep(DiffusionBatch* B, sunrealtype h)
 
 int DiffusionBatch_SetTileRows(DiffusionBatch* B, int rows)
 {
-  if (!B || rows < 0 || rows > 16) return ARK_ILL_INPUT;
+  if (!B || rows < 0 || rows > 256) return ARK_ILL_INPUT;
   B->tile_rows = rows;
   if (rows > 0) {
     for (BProb& b : B->probs) set_geometry(b, rows);
     upload_desc(B);
     B->tuned = true;
   } else B->tuned = false;
+  return 0;
+}
+
+int DiffusionBatch_SetStagesPerPass(DiffusionBatch* B, int kpass)
+{
+  if (!B || kpass < 1 || kpass > ceda::kBatchMaxK) return ARK_ILL_INPUT;
+  B->kpass = kpass;
+  if (!B->tile_rows) B->tuned = false;
   return 0;
 }
 

@@ -22,12 +22,14 @@
 //   t = -((Dw+De)+(Ds+Dn))*uc + Dw*uw + De*ue + Ds*us + Dn*un ; f = 0 + t
 // and the library is built without FMA contraction, which makes the result
 // bitwise identical to a serial CPU evaluation.
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
 
 #include "diffusion_internal.hpp"
+#include "stencil_ops.hpp"
 
 using namespace ceda;
 
@@ -46,33 +48,6 @@ struct KArgs {
   int64_t nx, ny;
   int tiles_x, rows;
 };
-
-__device__ inline double cell(double Dw, double De, double Ds, double Dn, double uc,
-                              double uw, double ue, double us, double un)
-{
-  double t = -((Dw + De) + (Ds + Dn)) * uc;
-  t = t + Dw * uw;
-  t = t + De * ue;
-  t = t + Ds * us;
-  t = t + Dn * un;
-  return 0.0 + t;
-}
-
-__device__ inline double forcing(double T1, double T2, double sxi, double cxi, double syj,
-                                 double cyj)
-{
-  return (T1 * sxi) * syj + T2 * (cxi * syj + cyj * sxi);
-}
-
-__device__ inline double lc5(const double* c, double F, double a, double b, double u, double g)
-{
-  double acc = c[0] * F;
-  acc = acc + c[1] * a;
-  acc = acc + c[2] * b;
-  acc = acc + c[3] * u;
-  acc = acc + c[4] * g;
-  return acc;
-}
 
 // FUSED = false: f = F(u).  FUSED = true: f = stage combination with F(u).
 template <int CPL, bool PER, bool MMS, bool FUSED>
@@ -168,39 +143,6 @@ __global__ __launch_bounds__(kBlock) void k_stencil(const KArgs a)
   }
 }
 
-
-typedef double d2v __attribute__((ext_vector_type(2)));
-
-template <bool NT>
-__device__ inline double2 ldv2(const double* p)
-{
-  if (NT) {
-    const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
-    return make_double2(v.x, v.y);
-  }
-  return *reinterpret_cast<const double2*>(p);
-}
-template <bool NT>
-__device__ inline void stv2(double* p, double a, double b)
-{
-  if (NT) {
-    d2v w;
-    w.x = a;
-    w.y = b;
-    __builtin_nontemporal_store(w, reinterpret_cast<d2v*>(p));
-  } else *reinterpret_cast<double2*>(p) = make_double2(a, b);
-}
-template <bool NT>
-__device__ inline double lds1(const double* p)
-{
-  return NT ? __builtin_nontemporal_load(p) : *p;
-}
-template <bool NT>
-__device__ inline void sts1(double* p, double v)
-{
-  if (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
 
 // y-marching variant: a workgroup owns 256*CPL columns x `rows` consecutive
 // rows; each lane walks down its columns keeping the south/centre/north rows
@@ -416,174 +358,6 @@ __global__ __launch_bounds__(kBlock) void k_stage1(int64_t n, const StageEntry* 
 }
 
 
-// ------------------------------------------------------------------ batch
-// Tile = 64*CPL columns x 4*rows rows: wavefront w of the workgroup takes rows
-// ty*4*rows + r*4 + w (r < rows), each lane CPL adjacent columns.  The same
-// cell(), forcing() and lc5() arithmetic as the single-problem kernels.
-template <int CPL, bool PER, bool MMS, int KIND>
-__device__ void batch_tile(const BatchDesc& D, const StageEntry* e, int tx, int ty)
-{
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  const int64_t nx = D.nx, ny = D.ny;
-  const int64_t i = (int64_t)tx * (kWave * CPL) + (int64_t)lane * CPL;
-  if (i >= nx) return;
-  const double* __restrict__ u = e->yjm1;
-  double* __restrict__ fout = e->out;
-  if (KIND == 1) {  // out = c0*yn + c1*fn
-    const double a0 = e->c[0], a1 = e->c[1];
-    for (int rr = 0; rr < D.rows; rr++) {
-      const int64_t j = (int64_t)ty * (4 * D.rows) + rr * 4 + wave;
-      if (j >= ny) break;
-      const int64_t o = j * nx + i;
-#pragma unroll
-      for (int c = 0; c < CPL; c++) fout[o + c] = a0 * e->yn[o + c] + a1 * e->fn[o + c];
-    }
-    return;
-  }
-  const double T1 = e->T1, T2 = e->T2;
-  double cc[5];
-#pragma unroll
-  for (int k = 0; k < 5; k++) cc[k] = e->c[k];
-  const double Dw0 = D.fx[i], De0 = D.fx[i + 1];
-  const double De1 = (CPL == 2) ? D.fx[i + 2] : 0.0;
-  const int64_t iw = (i > 0) ? i - 1 : (PER ? nx - 1 : -1);
-  const int64_t ie = (i + CPL < nx) ? i + CPL : (PER ? 0 : -1);
-  double sx0 = 0, sx1 = 0, cx0 = 0, cx1 = 0;
-  if (MMS) {
-    sx0 = D.sx[i];
-    cx0 = D.cx[i];
-    if (CPL == 2) {
-      sx1 = D.sx[i + 1];
-      cx1 = D.cx[i + 1];
-    }
-  }
-  for (int rr = 0; rr < D.rows; rr++) {
-    const int64_t j = (int64_t)ty * (4 * D.rows) + rr * 4 + wave;
-    if (j >= ny) break;
-    const int64_t js = (j > 0) ? j - 1 : (PER ? ny - 1 : -1);
-    const int64_t jn = (j < ny - 1) ? j + 1 : (PER ? 0 : -1);
-    const double Ds = D.fy[j], Dn = D.fy[j + 1];
-    const int64_t o = j * nx + i;
-    const double uw = (iw >= 0) ? u[j * nx + iw] : 0.0;
-    const double ue = (ie >= 0) ? u[j * nx + ie] : 0.0;
-    if (CPL == 2) {
-      const double2 C = *reinterpret_cast<const double2*>(u + o);
-      const double2 S = (js >= 0) ? *reinterpret_cast<const double2*>(u + js * nx + i) : make_double2(0.0, 0.0);
-      const double2 N = (jn >= 0) ? *reinterpret_cast<const double2*>(u + jn * nx + i) : make_double2(0.0, 0.0);
-      double f0 = cell(Dw0, De0, Ds, Dn, C.x, uw, C.y, S.x, N.x);
-      double f1 = cell(De0, De1, Ds, Dn, C.y, C.x, ue, S.y, N.y);
-      if (MMS) {
-        const double syj = D.sy[j], cyj = D.cy[j];
-        f0 = f0 + forcing(T1, T2, sx0, cx0, syj, cyj);
-        f1 = f1 + forcing(T1, T2, sx1, cx1, syj, cyj);
-      }
-      if (KIND == 2) {
-        const double2 A2 = *reinterpret_cast<const double2*>(e->yjm2 + o);
-        const double2 Y0 = *reinterpret_cast<const double2*>(e->yn + o);
-        const double2 G = *reinterpret_cast<const double2*>(e->fn + o);
-        f0 = lc5(cc, f0, A2.x, Y0.x, C.x, G.x);
-        f1 = lc5(cc, f1, A2.y, Y0.y, C.y, G.y);
-      }
-      *reinterpret_cast<double2*>(fout + o) = make_double2(f0, f1);
-    } else {
-      const double C = u[o];
-      const double S = (js >= 0) ? u[js * nx + i] : 0.0;
-      const double N = (jn >= 0) ? u[jn * nx + i] : 0.0;
-      double f0 = cell(Dw0, De0, Ds, Dn, C, uw, ue, S, N);
-      if (MMS) f0 = f0 + forcing(T1, T2, sx0, cx0, D.sy[j], D.cy[j]);
-      if (KIND == 2) f0 = lc5(cc, f0, e->yjm2[o], e->yn[o], C, e->fn[o]);
-      fout[o] = f0;
-    }
-  }
-}
-
-template <int CPL, bool PER, bool MMS>
-__device__ void batch_tile_kind(int kind, const BatchDesc& D, const StageEntry* e, int tx, int ty)
-{
-  if (kind == 1) batch_tile<CPL, PER, MMS, 1>(D, e, tx, ty);
-  else if (kind == 2) batch_tile<CPL, PER, MMS, 2>(D, e, tx, ty);
-  else batch_tile<CPL, PER, MMS, 3>(D, e, tx, ty);
-}
-
-__global__ __launch_bounds__(kBlock) void k_batch_phase(const BatchDesc* __restrict__ desc,
-                                                        const BatchTile* __restrict__ tiles,
-                                                        const StageEntry* __restrict__ ent,
-                                                        int stride, const int* __restrict__ stages,
-                                                        int j)
-{
-  const BatchTile t = tiles[blockIdx.x];
-  const int s = stages[t.slot];
-  const int kind = (j == 1) ? 1 : (j <= s ? 2 : 3);
-  const BatchDesc D = desc[t.prob];
-  const StageEntry* e = ent + (int64_t)t.slot * stride + j;
-  if (D.cpl == 2) {
-    if (D.per) batch_tile_kind<2, true, false>(kind, D, e, t.tx, t.ty);
-    else batch_tile_kind<2, false, true>(kind, D, e, t.tx, t.ty);
-  } else {
-    if (D.per) batch_tile_kind<1, true, false>(kind, D, e, t.tx, t.ty);
-    else batch_tile_kind<1, false, true>(kind, D, e, t.tx, t.ty);
-  }
-}
-
-__device__ inline double sts_err_term(const BatchErr& E, int64_t o)
-{
-  const double y = E.yn[o];
-  double e = 0.8 * y;
-  e = e + -0.8 * E.ys[o];
-  e = e + E.ch * E.fn[o];
-  e = e + E.ch * E.fs[o];
-  const double w = 1.0 / (E.rtol * fabs(y) + E.atol);
-  return e * w;
-}
-
-__global__ __launch_bounds__(kBlock) void k_batch_err(const BatchDesc* __restrict__ desc,
-                                                      const BatchTile* __restrict__ tiles,
-                                                      const BatchErr* __restrict__ err,
-                                                      double* __restrict__ partial)
-{
-  const BatchTile t = tiles[blockIdx.x];
-  const BatchDesc D = desc[t.prob];
-  const BatchErr E = err[t.slot];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-  double acc = 0.0;
-  const int64_t i0 = (int64_t)t.tx * (kWave * D.cpl) + (int64_t)lane * D.cpl;
-  for (int rr = 0; rr < D.rows; rr++) {
-    const int64_t j = (int64_t)t.ty * (4 * D.rows) + rr * 4 + wave;
-    if (j >= D.ny) break;
-    for (int c = 0; c < D.cpl; c++) {
-      const int64_t i = i0 + c;
-      if (i < D.nx) {
-        const double p = sts_err_term(E, j * D.nx + i);
-        acc += p * p;
-      }
-    }
-  }
-  __shared__ double sh[kBlock / kWave];
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-  if (lane == 0) sh[wave] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-}
-
-__global__ __launch_bounds__(kBlock) void k_batch_err_final(const double* __restrict__ partial,
-                                                            const int* __restrict__ begin,
-                                                            const int* __restrict__ count,
-                                                            double* __restrict__ dsq)
-{
-  const int slot = blockIdx.x;
-  const int b = begin[slot], c = count[slot];
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < c; k += kBlock) acc += partial[b + k];
-  __shared__ double sh[kBlock / kWave];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-  if (lane == 0) sh[wave] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) dsq[slot] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-}
-
 __global__ __launch_bounds__(kBlock) void k_psetup(int64_t nx, int64_t ny, const double* ax,
                                                    const double* ay, double gamma, double* diag)
 {
@@ -635,7 +409,7 @@ struct TuneKey {
     return nx != o.nx ? nx < o.nx : (ny != o.ny ? ny < o.ny : kind < o.kind);
   }
 };
-std::map<TuneKey, std::pair<StencilCfg, StencilCfg>> g_tune;
+std::map<TuneKey, std::array<StencilCfg, 3>> g_tune;
 std::mutex g_tune_mu;
 
 bool vec_ok(const DiffusionProblem* P, N_Vector v)
@@ -667,24 +441,6 @@ void launch_fused_stage(DiffusionProblem* P, const double* coef, const double* y
   a.yn = yn;
   a.fn = fn;
   launch_stencil(P, a, true, s);
-}
-
-void launch_batch_phase(const BatchDesc* desc, const BatchTile* tiles, const StageEntry* ent,
-                        int stride, const int* stages, int j, int ntiles, hipStream_t s)
-{
-  if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_batch_phase, dim3(ntiles), dim3(kBlock), 0, s, desc, tiles, ent, stride,
-                     stages, j);
-}
-
-void launch_batch_err(const BatchDesc* desc, const BatchTile* tiles, const BatchErr* err,
-                      int ntiles, double* partial, const int* tile_begin, const int* tile_count,
-                      int nslots, double* dsq, hipStream_t s)
-{
-  if (ntiles <= 0 || nslots <= 0) return;
-  hipLaunchKernelGGL(k_batch_err, dim3(ntiles), dim3(kBlock), 0, s, desc, tiles, err, partial);
-  hipLaunchKernelGGL(k_batch_err_final, dim3(nslots), dim3(kBlock), 0, s, partial, tile_begin,
-                     tile_count, dsq);
 }
 
 void launch_stage1_entry(DiffusionProblem* P, const StageEntry* e, hipStream_t s)
@@ -810,6 +566,7 @@ DiffusionProblem* DiffusionProblem_Create(const DiffusionParams* prm, SUNContext
   N_VConst(1.0, P->diag);
   P->cfg[0] = StencilCfg{0, 8, 0};
   P->cfg[1] = StencilCfg{0, 2, 0};
+  P->cfg[2] = StencilCfg{0, 32, 0};
   return P;
 }
 
@@ -850,9 +607,9 @@ int DiffusionProblem_SetStencilRows(DiffusionProblem* P, int rows)
 
 int DiffusionProblem_SetStencilConfig(DiffusionProblem* P, int which, int variant, int rows, int nt)
 {
-  if (!P || which < 0 || which > 1 || variant < 0 || variant > 1 || rows < 1 || rows > 256 ||
-      (nt != 0 && nt != 1) || (variant == 0 && nt))
-    return -1;
+  if (!P || which < 0 || which > 2 || rows < 1 || rows > 1024 || (nt != 0 && nt != 1)) return -1;
+  if (which < 2 && (variant < 0 || variant > 1 || rows > 256 || (variant == 0 && nt))) return -1;
+  if (which == 2 && (variant < 0 || variant > 4)) return -1;
   P->cfg[which].variant = variant;
   P->cfg[which].rows = rows;
   P->cfg[which].nt = nt;
@@ -862,12 +619,43 @@ int DiffusionProblem_SetStencilConfig(DiffusionProblem* P, int which, int varian
 int DiffusionProblem_GetStencilConfig(const DiffusionProblem* P, int which, int* variant, int* rows,
                                       int* nt)
 {
-  if (!P || which < 0 || which > 1) return -1;
+  if (!P || which < 0 || which > 2) return -1;
   if (variant) *variant = P->cfg[which].variant;
   if (rows) *rows = P->cfg[which].rows;
   if (nt) *nt = P->cfg[which].nt;
   return 0;
 }
+
+}  // extern "C"
+
+namespace {
+// device stage table of a K-stage timing pass: the timing coefficients,
+// inputs ops[0..3] = Y_{j-1}, Y_{j-2}, Y_n, F_n; stage K-1 -> ops[4],
+// stage K -> ops[5]
+StageEntry* pass_table(double* const* ops, int K, hipStream_t s)
+{
+  StageEntry h[4];
+  memset(h, 0, sizeof(h));
+  const double hc[5] = {1.0e-9, -0.5, 0.25, 1.25, -1.0e-10};
+  for (int m = 0; m < 4; m++) {
+    for (int k = 0; k < 5; k++) h[m].c[k] = hc[k];
+    h[m].T2 = 1.0;
+    h[m].yjm1 = ops[0];
+    h[m].yjm2 = ops[1];
+    h[m].yn = ops[2];
+    h[m].fn = ops[3];
+    h[m].out = ops[5];
+  }
+  if (K >= 2) h[K - 2].out = ops[4];
+  StageEntry* d = nullptr;
+  CEDA_HIP_CHECK(hipMalloc(&d, sizeof(h)));
+  CEDA_HIP_CHECK(hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, s));
+  CEDA_HIP_CHECK(hipStreamSynchronize(s));
+  return d;
+}
+}  // namespace
+
+extern "C" {
 
 int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
 {
@@ -877,15 +665,14 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
-      P->cfg[0] = it->second.first;
-      P->cfg[1] = it->second.second;
+      for (int k = 0; k < 3; k++) P->cfg[k] = it->second[k];
       P->tuned = true;
       return 0;
     }
   }
   hipStream_t s = P->exec->stream;
-  double* buf[5];
-  for (int k = 0; k < 5; k++) {
+  double* buf[6];
+  for (int k = 0; k < 6; k++) {
     CEDA_HIP_CHECK(hipMalloc(&buf[k], sizeof(double) * P->n));
     CEDA_HIP_CHECK(hipMemsetAsync(buf[k], 0, sizeof(double) * P->n, s));
   }
@@ -900,6 +687,7 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
   for (int rows : {1, 2, 4, 8}) cands.push_back(StencilCfg{0, rows, 0});
   for (int rows : {4, 8, 16, 32, 64})
     for (int nt : {0, 1}) cands.push_back(StencilCfg{1, rows, nt});
+  float best_stage = 1e30f;
   for (int which = 0; which < 2; which++) {
     float best = 1e30f;
     StencilCfg bestc = P->cfg[which];
@@ -923,15 +711,41 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
       }
     }
     P->cfg[which] = bestc;
+    if (which == 1) best_stage = best;
   }
+  // multi-stage passes: kept only when a stage costs less than the best
+  // single-stage kernel (per stage: pass time / K)
+  StencilCfg bestp{0, 32, 0};
+  float bestp_ms = best_stage * 0.97f;
+  for (int K : {2, 3, 4}) {
+    StageEntry* dtab = pass_table(buf, K, s);
+    for (int rows : {8, 16, 32, 64, 128})
+      for (int nt : {0, 1}) {
+        if (rows > 2 * P->p.ny) continue;
+        launch_pass(P, K, rows, nt, dtab, s);
+        CEDA_HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) launch_pass(P, K, rows, nt, dtab, s);
+        CEDA_HIP_CHECK(hipEventRecord(e1, s));
+        CEDA_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        CEDA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / K < bestp_ms) {
+          bestp_ms = ms / K;
+          bestp = StencilCfg{K, rows, nt};
+        }
+      }
+    CEDA_HIP_CHECK(hipStreamSynchronize(s));
+    CEDA_HIP_CHECK(hipFree(dtab));
+  }
+  P->cfg[2] = bestp;
   CEDA_HIP_CHECK(hipEventDestroy(e0));
   CEDA_HIP_CHECK(hipEventDestroy(e1));
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
-  for (int k = 0; k < 5; k++) CEDA_HIP_CHECK(hipFree(buf[k]));
+  for (int k = 0; k < 6; k++) CEDA_HIP_CHECK(hipFree(buf[k]));
   CEDA_HIP_CHECK(hipFree(coef));
   P->tuned = true;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tune[key] = {P->cfg[0], P->cfg[1]};
+  g_tune[key] = {P->cfg[0], P->cfg[1], P->cfg[2]};
   return 0;
 }
 
@@ -939,8 +753,8 @@ int DiffusionProblem_TimeKernel(DiffusionProblem* P, int which, N_Vector u, N_Ve
                                 N_Vector b, N_Vector c, N_Vector out, int reps, double* avg_ms)
 {
   if (!P || reps < 1 || !avg_ms || !vec_ok(P, u) || !vec_ok(P, out)) return -1;
-  if (which == 1 && (!vec_ok(P, a) || !vec_ok(P, b) || !vec_ok(P, c))) return -1;
-  if (which != 0 && which != 1) return -1;
+  if (which >= 1 && (!vec_ok(P, a) || !vec_ok(P, b) || !vec_ok(P, c))) return -1;
+  if (which < 0 || which > 2 || (which == 2 && P->cfg[2].variant < 1)) return -1;
   hipStream_t s = P->exec->stream;
   double* coef = nullptr;
   // stage coefficients of a mid-loop RKL2 stage; the values do not change
@@ -951,9 +765,19 @@ int DiffusionProblem_TimeKernel(DiffusionProblem* P, int which, N_Vector u, N_Ve
   hipEvent_t e0, e1;
   CEDA_HIP_CHECK(hipEventCreate(&e0));
   CEDA_HIP_CHECK(hipEventCreate(&e1));
+  // a pass writes two stages: the next-to-last goes to a scratch vector
+  double* mid = nullptr;
+  StageEntry* dtab = nullptr;
+  if (which == 2) {
+    CEDA_HIP_CHECK(hipMalloc(&mid, sizeof(double) * P->n));
+    CEDA_HIP_CHECK(hipMemset(mid, 0, sizeof(double) * P->n));
+    double* ops[6] = {dptr(u), dptr(a), dptr(b), dptr(c), mid, dptr(out)};
+    dtab = pass_table(ops, P->cfg[2].variant, s);
+  }
   auto launch = [&]() {
     if (which == 0) launch_rhs_devtime(P, coef, dptr(u), dptr(out), s);
-    else launch_fused_stage(P, coef, dptr(u), dptr(a), dptr(b), dptr(c), dptr(out), s);
+    else if (which == 1) launch_fused_stage(P, coef, dptr(u), dptr(a), dptr(b), dptr(c), dptr(out), s);
+    else launch_pass(P, P->cfg[2].variant, P->cfg[2].rows, P->cfg[2].nt, dtab, s);
   };
   launch();  // warm
   CEDA_HIP_CHECK(hipEventRecord(e0, s));
@@ -963,10 +787,52 @@ int DiffusionProblem_TimeKernel(DiffusionProblem* P, int which, N_Vector u, N_Ve
   float ms = 0.0f;
   CEDA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   *avg_ms = (double)ms / reps;
+  if (mid) {
+    CEDA_HIP_CHECK(hipStreamSynchronize(s));
+    CEDA_HIP_CHECK(hipFree(mid));
+    CEDA_HIP_CHECK(hipFree(dtab));
+  }
   CEDA_HIP_CHECK(hipEventDestroy(e0));
   CEDA_HIP_CHECK(hipEventDestroy(e1));
   CEDA_HIP_CHECK(hipFree(coef));
   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int DiffusionProblem_StagePass(DiffusionProblem* P, int K, const double* coef, N_Vector yjm1,
+                               N_Vector yjm2, N_Vector yn, N_Vector fn, N_Vector outkm1,
+                               N_Vector outk)
+{
+  if (!P || K < 1 || K > 4 || !coef || !vec_ok(P, yjm1) || !vec_ok(P, yjm2) || !vec_ok(P, yn) ||
+      !vec_ok(P, fn) || !vec_ok(P, outk) || (K >= 2 && !vec_ok(P, outkm1)))
+    return -1;
+  const double* ins[4] = {dptr(yjm1), dptr(yjm2), dptr(yn), dptr(fn)};
+  double* outs[2] = {K >= 2 ? dptr(outkm1) : nullptr, dptr(outk)};
+  if (outs[0] == outs[1]) return -1;
+  for (const double* x : ins)
+    for (double* o : outs)
+      if (o && o == x) return -1;
+  hipStream_t s = P->exec->stream;
+  StageEntry h[4];
+  memset(h, 0, sizeof(h));
+  for (int m = 0; m < K; m++) {
+    for (int k = 0; k < 5; k++) h[m].c[k] = coef[8 * m + k];
+    h[m].T1 = coef[8 * m + 5];
+    h[m].T2 = coef[8 * m + 6];
+    h[m].yjm1 = ins[0];
+    h[m].yjm2 = ins[1];
+    h[m].yn = ins[2];
+    h[m].fn = ins[3];
+    h[m].out = outs[1];
+  }
+  if (K >= 2) h[K - 2].out = outs[0];
+  StageEntry* d = nullptr;
+  CEDA_HIP_CHECK(hipMalloc(&d, sizeof(h)));
+  CEDA_HIP_CHECK(hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, s));
+  launch_pass(P, K, P->cfg[2].rows, P->cfg[2].nt, d, s);
+  CEDA_HIP_CHECK(hipGetLastError());
+  CEDA_HIP_CHECK(hipStreamSynchronize(s));
+  CEDA_HIP_CHECK(hipFree(d));
+  return 0;
 }
 
 int Initial(sunrealtype, N_Vector u, void* user_data)

@@ -35,7 +35,9 @@ struct DiffusionProblem {
   SUNContext ctx = nullptr;
   ceda::Exec* exec = nullptr;
   long nrhs = 0;
-  StencilCfg cfg[2];        // [0] RHS kernel, [1] fused stage kernel
+  // [0] RHS kernel, [1] fused stage kernel, [2] multi-stage pass (variant =
+  // stages per pass K, 0 = single-stage kernels only; rows; nt)
+  StencilCfg cfg[3];
   bool tuned = false;
 };
 
@@ -64,25 +66,38 @@ void launch_fused_entry(DiffusionProblem* P, const StageEntry* e, hipStream_t s)
 // out = F(yjm1) with the entry's time factors
 void launch_rhs_entry(DiffusionProblem* P, const StageEntry* e, hipStream_t s);
 
+// K consecutive fused stages in one sweep (temporal blocking): stages read
+// e[0].yjm1, e[0].yjm2, e[0].yn, e[0].fn; stage K-1 is written to e[K-2].out
+// and stage K to e[K-1].out (K = 1..4).  rows = output rows per wavefront
+// tile, nt = non-temporal streams.  Bitwise equal to K single-stage launches.
+void launch_pass(DiffusionProblem* P, int K, int rows, int nt, const StageEntry* e, hipStream_t s);
+// halo lanes of a K-stage pass (each side of a wavefront strip)
+int pass_halo_lanes(int K, int cpl);
+
 // ---- batched super-time-stepping (many independent problems, one launch
-// per stage index) ----
-struct BatchDesc {           // per problem, static
+// per pass index) ----
+constexpr int kBatchMaxK = 3;  // stages per pass the batch tile geometry covers
+struct BatchDesc {             // per problem, static
   const double *fx, *fy, *sx, *sy, *cx, *cy;
   int64_t nx, ny;
-  int per, cpl, rows, tiles_x;
+  int per, cpl, rows, strips;
 };
-struct BatchTile {           // per tile of a step group, problems in slot order
-  int slot, prob, tx, ty;
+struct BatchTile {             // one wavefront tile, problems in slot order
+  int slot, prob, strip, rb;
 };
-struct BatchErr {            // per slot: operands of the embedded error norm
+struct BatchErr {              // per slot: operands of the embedded error norm
   const double *yn, *ys, *fn, *fs;
   double ch, rtol, atol, pad;
 };
-// phase j of a step group: for every tile t < ntiles, with s = stages[slot]:
-// j == 1 first stage, 2 <= j <= s fused stage, j == s + 1 the RHS at t_n + h;
-// entries of slot k start at ent + k*stride
-void launch_batch_phase(const BatchDesc* desc, const BatchTile* tiles, const StageEntry* ent,
-                        int stride, const int* stages, int j, int ntiles, hipStream_t s);
+int batch_halo_lanes(int cpl);
+// phase 0: stage 1 of every slot; phase q >= 1: stages j = 2+(q-1)*kmax ..
+// j+kmax-1 (clipped at s = stages[slot]) of the slots with j <= s, and the RHS
+// at t_n + h (adaptive) of the slots whose last pass was phase q-1; entries of
+// slot k start at ent + k*stride; tiles [0, ntiles) must cover exactly those
+// slots (a prefix of the slots when they are sorted by s, descending)
+void launch_batch_phase(const BatchDesc* desc, const BatchTile* tiles, int ntiles,
+                        const StageEntry* ent, int stride, const int* stages, int phase, int kmax,
+                        int adaptive, hipStream_t s);
 // per-tile sums of (e*ewt)^2, then per-slot sums in tile order (deterministic)
 void launch_batch_err(const BatchDesc* desc, const BatchTile* tiles, const BatchErr* err,
                       int ntiles, double* partial, const int* tile_begin, const int* tile_count,

@@ -34,9 +34,13 @@ int DiffusionBatch_SetController(DiffusionBatch* B, int controller);    /* ARK_A
 int DiffusionBatch_SetMaxNumStages(DiffusionBatch* B, int smax);
 int DiffusionBatch_SetMaxNumSteps(DiffusionBatch* B, long mxsteps);
 int DiffusionBatch_SetFixedStep(DiffusionBatch* B, sunrealtype h);
-/* tile height of the batched kernels: 4*rows rows x 64 or 128 columns per
-   workgroup (rows 1..16); 0 = choose per problem by the tile autotuner */
+/* tile height of the batched kernels: each wavefront tile is `rows` rows of a
+   124-column strip (62 for odd nx); rows 1..256, 0 = choose per size class by
+   the tile autotuner */
 int DiffusionBatch_SetTileRows(DiffusionBatch* B, int rows);
+/* stages per pass of the batched multi-stage kernel (1..3, default 3): each
+   phase launch advances every active problem by up to this many stages */
+int DiffusionBatch_SetStagesPerPass(DiffusionBatch* B, int kpass);
 /* advance every problem to tout; returns 0, or the first failing problem's
    ARKODE code */
 int DiffusionBatch_Evolve(DiffusionBatch* B, sunrealtype tout);

@@ -55,23 +55,39 @@ int DiffusionProblem_SetStencilRows(DiffusionProblem* prob, int rows);
 /* launch shape of one kernel kind (which: 0 RHS stencil, 1 fused STS stage):
    variant 0 = row-interleaved tiles (rows per wavefront), variant 1 =
    y-marching (rows = consecutive rows per wavefront, nt = non-temporal
-   streamed operands).  Every shape computes bitwise-identical results. */
+   streamed operands).  which = 2 is the multi-stage pass of the fused path:
+   variant = stages per pass K (2..4; 0 = one kernel per stage), rows = output
+   rows per wavefront tile (1..1024), nt as above.  Every shape computes
+   bitwise-identical results. */
 int DiffusionProblem_SetStencilConfig(DiffusionProblem* prob, int which, int variant, int rows,
                                       int nt);
 int DiffusionProblem_GetStencilConfig(const DiffusionProblem* prob, int which, int* variant,
                                       int* rows, int* nt);
-/* times every candidate launch shape of both kernels on scratch buffers of
-   the problem's size (reps launches each, HIP events) and keeps the fastest;
-   results are cached per (nx, ny, kind) for the life of the process */
+/* times every candidate launch shape of the three kernels on scratch buffers
+   of the problem's size (reps launches each, HIP events) and keeps the
+   fastest; multi-stage passes are enabled only when their time per stage
+   beats the best single-stage kernel.  Results are cached per (nx, ny, kind)
+   for the life of the process */
 int DiffusionProblem_Autotune(DiffusionProblem* prob, int reps);
 
 /* [ext] times `reps` back-to-back launches of one hot-path kernel with HIP
    events on the problem's stream: which = 0 the RHS stencil (u -> out),
-   which = 1 the fused STS stage (u = Y_{j-1}, a = Y_{j-2}, b = Y_0, c = F_0).
-   Writes the mean launch duration in milliseconds. */
+   which = 1 the fused STS stage (u = Y_{j-1}, a = Y_{j-2}, b = Y_0, c = F_0),
+   which = 2 the configured multi-stage pass (same operands; the last stage
+   goes to out).  Writes the mean launch duration in milliseconds. */
 int DiffusionProblem_TimeKernel(DiffusionProblem* prob, int which, N_Vector u, N_Vector a,
                                 N_Vector b, N_Vector c, N_Vector out, int reps,
                                 double* avg_ms);
+/* [ext] K consecutive STS stages (K = 1..4) in one pass of the multi-stage
+   kernel, with the configured rows/nt of which = 2: stage m (0-based) uses
+   coef[8m .. 8m+4] as {c0..c4} and coef[8m+5], coef[8m+6] as the MMS time
+   factors (T1, T2) of its RHS; reads yjm1 = Y_{j-1}, yjm2 = Y_{j-2}, yn, fn;
+   writes stage K-1 to outkm1 (K >= 2; ignored for K = 1) and stage K to
+   outk.  The outputs must not alias an input.  Bitwise equal to K fused
+   single-stage launches.  Returns 0, or -1 for invalid arguments. */
+int DiffusionProblem_StagePass(DiffusionProblem* prob, int K, const double* coef, N_Vector yjm1,
+                               N_Vector yjm2, N_Vector yn, N_Vector fn, N_Vector outkm1,
+                               N_Vector outk);
 
 int Initial(sunrealtype t, N_Vector u, void* user_data);
 int Exact(sunrealtype t, N_Vector u, void* user_data);
