@@ -172,7 +172,7 @@ void autotune_tiles(DiffusionBatch* B)
     const std::vector<int>& ks = cl.second;
     int best_rows = 16;
     float best = 1e30f;
-    for (int rows : {4, 8, 16, 32, 64}) {
+    for (int rows : {4, 8, 16, 24, 32, 48, 64}) {
       if (rows > 2 * cl.first.second) continue;
       for (int k : ks) set_geometry(B->probs[k], rows);
       upload_desc(B);

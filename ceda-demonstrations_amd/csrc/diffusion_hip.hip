@@ -717,9 +717,12 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
   // single-stage kernel (per stage: pass time / K)
   StencilCfg bestp{0, 32, 0};
   float bestp_ms = best_stage * 0.97f;
+  // heights between 16 and 64 matter: the K-1 halo rows above and below a
+  // tile are re-read (mostly from HBM), while tall tiles leave too few
+  // wavefronts or a partial last round of them
   for (int K : {2, 3, 4}) {
     StageEntry* dtab = pass_table(buf, K, s);
-    for (int rows : {8, 16, 32, 64, 128})
+    for (int rows : {8, 16, 24, 32, 40, 48, 64, 96, 128})
       for (int nt : {0, 1}) {
         if (rows > 2 * P->p.ny) continue;
         launch_pass(P, K, rows, nt, dtab, s);
