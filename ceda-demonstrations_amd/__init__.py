@@ -124,6 +124,21 @@ _SIGS = {
     "ARKodeSetEpsLin": (i32, [vp, d]),
     "ARKodeSetLinear": (i32, [vp, i32]),
     "ARKodeSetNonlinear": (i32, [vp]),
+    "ARKodeSetLSFallbackPCG": (i32, [vp, i32, vp, vp]),
+    "ARKodeGetNumLSFallbacks": (i32, [vp, C.POINTER(C.c_long)]),
+    "LearnedPrec_Create": (vp, [i32, C.POINTER(i32), vp]),
+    "LearnedPrec_Destroy": (None, [vp]),
+    "LearnedPrec_NumParams": (i32, [vp]),
+    "LearnedPrec_SetParams": (i32, [vp, C.POINTER(C.c_float)]),
+    "LearnedPrec_GetParams": (i32, [vp, C.POINTER(C.c_float)]),
+    "LearnedPrec_InitIdentity": (i32, [vp, d, C.c_ulong]),
+    "LearnedPrec_SetOutputScale": (i32, [vp, d]),
+    "LearnedPrec_GetOutputScale": (d, [vp]),
+    "LearnedPrec_Apply": (i32, [vp, i32, i32, i32, vp, vp, i32]),
+    "LearnedPrec_LossGrad": (i32, [vp, i32, i32, i32, d, vp, vp, dp, C.POINTER(C.c_float)]),
+    "LearnedPrec_MakeTrainingSet": (i32, [i32, i32, d, d, C.c_ulong, vp, vp, C.POINTER(i32), vp]),
+    "LearnedPrec_Train": (i32, [vp, i32, i32, d, vp, vp, i32, i32, d, C.c_ulong, dp]),
+    "DiffusionProblem_SetLearnedPrec": (i32, [vp, vp]),
     "ARKodeSetLSRelativeTolerance": (i32, [vp, d]),
     "SUNLinSol_PCG": (vp, [vp, i32, i32, vp]),
     "SUNLinSol_SPGMR": (vp, [vp, i32, i32, vp]),
@@ -208,7 +223,7 @@ class Context:
     def close(self):
         if self.h:
             kids = list(self.children)
-            for kind in (Integrator, Batch, Vector, Problem):
+            for kind in (Integrator, Batch, LearnedPrec, Vector, Problem):
                 for k in kids:
                     if isinstance(k, kind):
                         k.release()
@@ -430,6 +445,11 @@ def fn_ptr(name):
     return C.cast(getattr(lib(), name), vp)
 
 
+def dptr_of(v):
+    """Device address of a Vector's data."""
+    return C.cast(lib().N_VGetDeviceArrayPointer(v.h), vp)
+
+
 class Integrator:
     """LSRKStep (rkc/rkl) or implicit ARKStep (dirk) over a Problem."""
 
@@ -451,7 +471,11 @@ class Integrator:
             make = {"cg": L.SUNLinSol_PCG, "gmres": L.SUNLinSol_SPGMR, "fgmres": L.SUNLinSol_SPFGMR}[ls]
             self.LS = make(y0.h, pre, maxl, prob.ctx.h)
             L.ARKodeSetLinearSolver(self.h, self.LS, None)
-            if precond:
+            if precond == "learned":
+                # network on the problem (DiffusionProblem_SetLearnedPrec), Jacobi-PCG fallback
+                L.ARKodeSetPreconditioner(self.h, fn_ptr("LearnedPSetup"), fn_ptr("LearnedPSolve"))
+                L.ARKodeSetLSFallbackPCG(self.h, 100000, fn_ptr("PSolve"), prob.h)
+            elif precond:
                 L.ARKodeSetPreconditioner(self.h, fn_ptr("PSetup"), fn_ptr("PSolve"))
         if not self.h:
             raise RuntimeError("integrator creation failed")
@@ -481,6 +505,11 @@ class Integrator:
         out = (C.c_long * 18)()
         lib().ARKodeGetStatsArray(self.h, out, 18)
         return dict(zip(STATS, list(out)))
+
+    def fallbacks(self):
+        n = C.c_long()
+        lib().ARKodeGetNumLSFallbacks(self.h, C.byref(n))
+        return n.value
 
     def spectral_radius(self):
         r = d()
@@ -515,3 +544,84 @@ class Integrator:
         if self.ctrl:
             L.SUNAdaptController_Destroy(self.ctrl)
             self.ctrl = None
+
+
+class LearnedPrec:
+    """The learned preconditioner network (learned_prec.h): 3x3 conv layers
+    with the given dilations, 16 hidden channels, BF16 on the matrix cores."""
+
+    def __init__(self, ctx, dil=(1, 1, 1, 1)):
+        d = (i32 * len(dil))(*dil)
+        self.ctx = ctx
+        self.dil = tuple(dil)
+        self.h = lib().LearnedPrec_Create(len(dil), d, ctx.h)
+        if not self.h:
+            raise ValueError("bad network shape")
+        ctx.register(self)
+
+    @property
+    def nparams(self):
+        return lib().LearnedPrec_NumParams(self.h)
+
+    def params(self):
+        import numpy as np
+        p = np.zeros(self.nparams, dtype=np.float32)
+        lib().LearnedPrec_GetParams(self.h, p.ctypes.data_as(C.POINTER(C.c_float)))
+        return p
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        return lib().LearnedPrec_SetParams(self.h, p.ctypes.data_as(C.POINTER(C.c_float)))
+
+    def init_identity(self, noise=1e-2, seed=0):
+        return lib().LearnedPrec_InitIdentity(self.h, noise, seed)
+
+    def set_output_scale(self, s):
+        return lib().LearnedPrec_SetOutputScale(self.h, s)
+
+    def apply(self, nb, nx, ny, vin, vout, addto=0):
+        return lib().LearnedPrec_Apply(self.h, nb, nx, ny, dptr_of(vin), dptr_of(vout), addto)
+
+    def loss_grad(self, nb, nx, ny, ratio, r, e, grad=True):
+        import numpy as np
+        loss = C.c_double()
+        g = np.zeros(self.nparams, dtype=np.float32) if grad else None
+        rc = lib().LearnedPrec_LossGrad(self.h, nb, nx, ny, ratio, dptr_of(r), dptr_of(e), C.byref(loss),
+                                        g.ctypes.data_as(C.POINTER(C.c_float)) if grad else None)
+        assert rc == 0
+        return loss.value, g
+
+    def train(self, nsamp, n, ratio, r, e, steps, batch, lr, seed=0):
+        import numpy as np
+        hist = np.zeros(steps)
+        rc = lib().LearnedPrec_Train(self.h, nsamp, n, ratio, dptr_of(r), dptr_of(e), steps, batch, lr, seed,
+                                     hist.ctypes.data_as(C.POINTER(C.c_double)))
+        assert rc == 0
+        return hist
+
+    def attach(self, prob):
+        return lib().DiffusionProblem_SetLearnedPrec(prob.h, self.h)
+
+    def release(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib().LearnedPrec_Destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+def training_set(ctx, nsamp, n, ratio, reltol=1e-12, seed=1234):
+    """(r, e) pairs on the GPU: two Vectors of nsamp*n*n values and the largest
+    Jacobi-PCG iteration count."""
+    r = Vector(ctx, n=nsamp * n * n)
+    e = Vector(ctx, n=nsamp * n * n)
+    it = i32()
+    rc = lib().LearnedPrec_MakeTrainingSet(nsamp, n, ratio, reltol, seed, dptr_of(r), dptr_of(e),
+                                          C.byref(it), ctx.h)
+    return r, e, it.value, rc

@@ -90,6 +90,11 @@ struct ArkMem {
   N_Vector sdata = nullptr, zcur = nullptr, fz = nullptr, rhsv = nullptr, dz = nullptr,
            jwork = nullptr, jv = nullptr;
   double epslin = 0, ls_reltol = 0, gammap = 0, nlscoef = 0.1;
+  // fallback Jacobi-PCG of a GMRES solve that misses its tolerance
+  SUNLinearSolver fbLS = nullptr;
+  ARKLsPrecSolveFn fb_psolve = nullptr;
+  void* fb_ud = nullptr;
+  long nls_fallback = 0;
   long nstlp = 0;
   SUNLinearSolver LS = nullptr;
   ARKLsPrecSetupFn psetup = nullptr;

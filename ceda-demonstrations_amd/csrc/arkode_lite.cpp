@@ -673,10 +673,19 @@ double lnorm(ArkMem* A, N_Vector r)
   return N_VWL2Norm(r, A->ewt);
 }
 
-// (I - gamma J) x = b; 0 converged, 1 reduced, 2 failed, <0 fatal
-int pcg(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
+// (I - gamma J) x = b by PCG on solver S with preconditioner solve ps/ud
+// (ps = nullptr: none); 0 converged, 1 reduced, 2 failed, <0 fatal
+int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double t, double gamma,
+             N_Vector b, N_Vector x)
 {
-  SUNLinearSolver S = A->LS;
+  auto prec = [&](double delta, N_Vector r, N_Vector z) -> int {
+    if (ps) {
+      A->npsolve++;
+      return ps(t, A->zcur, A->fz, r, z, gamma, delta, 1, ud);
+    }
+    N_VScale(1.0, r, z);
+    return 0;
+  };
   N_VConst(0.0, x);
   N_VScale(1.0, b, S->r);
   const double bnorm = lnorm(A, S->r);
@@ -684,7 +693,7 @@ int pcg(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
   const double delta = (A->ls_reltol > 0.0) ? A->ls_reltol * bnorm
                                             : eplifac * A->nlscoef * sqrt((double)A->n);
   if (bnorm <= delta) return 0;
-  if (psolve(A, t, gamma, delta, S->r, S->z)) return -1;
+  if (prec(delta, S->r, S->z)) return -1;
   N_VScale(1.0, S->z, S->p);
   double rz = N_VDotProd(S->r, S->z);
   double rnorm = bnorm;
@@ -699,13 +708,19 @@ int pcg(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
     A->nli++;
     rnorm = lnorm(A, S->r);
     if (rnorm <= delta) return 0;
-    if (psolve(A, t, gamma, delta, S->r, S->z)) return -1;
+    if (prec(delta, S->r, S->z)) return -1;
     const double rznew = N_VDotProd(S->r, S->z);
     const double beta = rznew / rz;
     rz = rznew;
     N_VLinearSum(1.0, S->z, beta, S->p, S->p);
   }
   return (rnorm < bnorm) ? 1 : 2;
+}
+
+int pcg(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
+{
+  const bool pre = A->psolve && A->LS->pretype != SUN_PREC_NONE;
+  return pcg_with(A, A->LS, pre ? A->psolve : nullptr, A->ud, t, gamma, b, x);
 }
 
 // restarted flexible GMRES, right preconditioning, classical Gram-Schmidt
@@ -800,10 +815,17 @@ int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
   return (resid < (A->ls_reltol > 0.0 ? bnorm : delta2 * bnorm / delta)) ? 1 : 2;
 }
 
+// the configured Krylov solve; a GMRES solve that stops short of its
+// tolerance at its iteration cap is repeated by the fallback PCG when set
 int lsolve(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
 {
   if (A->LS->type == LS_PCG) return pcg(A, t, gamma, b, x);
-  return fgmres(A, t, gamma, b, x);
+  const int r = fgmres(A, t, gamma, b, x);
+  if (r > 0 && A->fbLS) {
+    A->nls_fallback++;
+    return pcg_with(A, A->fbLS, A->fb_psolve, A->fb_ud, t, gamma, b, x);
+  }
+  return r;
 }
 
 int newton(ArkMem* A, double t, double gamma, int force_setup)
@@ -1029,6 +1051,7 @@ void ARKodeFree(void** mem)
   }
   for (auto& g : A->graphs) CEDA_HIP_CHECK(hipGraphExecDestroy(g.second));
   if (A->tab_ev) CEDA_HIP_CHECK(hipEventDestroy(A->tab_ev));
+  if (A->fbLS) SUNLinSolFree(A->fbLS);
   delete A;
   *mem = nullptr;
 }
@@ -1413,6 +1436,30 @@ int ARKodeSetNonlinear(void* m)
 {
   if (!m) return ARK_MEM_NULL;
   AM(m)->linear = 0;
+  return 0;
+}
+
+int ARKodeSetLSFallbackPCG(void* m, int maxl, ARKLsPrecSolveFn psolve, void* user_data)
+{
+  if (!m) return ARK_MEM_NULL;
+  ArkMem* A = AM(m);
+  if (A->method != M_DIRK || maxl < 0) return ARK_ILL_INPUT;
+  if (A->fbLS) {
+    SUNLinSolFree(A->fbLS);
+    A->fbLS = nullptr;
+  }
+  if (maxl > 0) {
+    A->fbLS = SUNLinSol_PCG(A->yn, psolve ? SUN_PREC_LEFT : SUN_PREC_NONE, maxl, A->ctx);
+    A->fb_psolve = psolve;
+    A->fb_ud = user_data;
+  }
+  return 0;
+}
+
+int ARKodeGetNumLSFallbacks(void* m, long* nfallbacks)
+{
+  if (!m || !nfallbacks) return ARK_MEM_NULL;
+  *nfallbacks = AM(m)->nls_fallback;
   return 0;
 }
 

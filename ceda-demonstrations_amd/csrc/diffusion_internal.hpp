@@ -19,6 +19,8 @@ struct StencilCfg {
   int nt = 0;
 };
 
+struct LearnedPrec;
+
 struct DiffusionProblem {
   DiffusionParams p;
   int64_t n = 0;
@@ -38,6 +40,7 @@ struct DiffusionProblem {
   // [0] RHS kernel, [1] fused stage kernel, [2] multi-stage pass (variant =
   // stages per pass K, 0 = single-stage kernels only; rows; nt)
   StencilCfg cfg[3];
+  LearnedPrec* learned = nullptr;  // network of LearnedPSolve (not owned)
   bool tuned = false;
 };
 

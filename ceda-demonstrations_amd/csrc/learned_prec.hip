@@ -1,0 +1,986 @@
+// Learned preconditioner on gfx950: a 3x3 convolutional network with BF16
+// weights/activations and FP32 accumulation on the matrix cores, its
+// backward pass, the Adam trainer, the training-set generator (batched
+// Jacobi-PCG) and the ARKLsPrec callbacks.
+//
+// Convolution as implicit GEMM on v_mfma_f32_16x16x16bf16_1k.  For a block
+// of 16 consecutive pixels of one row, D[co][px] = sum_k A[co][k] B[k][px]
+// with k running over (tap, input channel): for a 16 -> 16 layer one MFMA per
+// tap (K = the 16 input channels of the shifted pixel), for the 1 -> 16 input
+// layer one MFMA whose K index is the tap (9 of 16 used).  Operand layout of
+// the instruction (lane l, g = l/16, n = l%16): A holds A[n][4g..4g+3], B
+// holds B[4g..4g+3][n], D register i holds D[4g+i][n].  So the weights are the
+// A operand (packed once per layer, loaded once per wavefront), the B operand
+// is 4 channels of pixel n read as one 8-byte load from the NHWC activation
+// (16 lanes cover 512 contiguous bytes), and each lane's D registers are 4
+// consecutive channels of one pixel: one 8-byte store.  Activations are NHWC
+// with 16 BF16 channels (32 bytes per pixel); the network input and output
+// are FP64 images.
+//
+// Backward.  The input gradient of a layer is the same convolution with the
+// taps flipped and the channel roles exchanged (a separately packed A
+// operand); the ReLU derivative of the layer below is applied on store (the
+// stored activation is positive exactly where the pre-activation was).  The
+// weight gradient dW[co][ci][t] = sum_p dY[p][co] x[p+off_t][ci] is a GEMM
+// with K = pixels: each wavefront accumulates 9 tap tiles over its pixel
+// blocks, and the per-wavefront partials are summed in a fixed order
+// (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "diffusion_internal.hpp"
+#include "learned_prec.h"
+
+using ceda::kBlock;
+using ceda::kWave;
+
+namespace {
+
+constexpr int kC = 16;        // hidden channels
+constexpr int kMaxL = 8;      // layers
+constexpr int kWavesPB = kBlock / kWave;
+constexpr int kPW = 9 * 256 + 16;  // weight-gradient partial of one wavefront
+constexpr int kWGradWaves = 1024;
+
+typedef short bfx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short f2bf(float f)
+{
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+__device__ inline unsigned short d2bf(double x) { return f2bf((float)x); }
+
+
+// ------------------------------------------------------------ forward conv
+struct ConvArgs {
+  int nb, nx, ny, dil;
+  const void* in;             // NHWC16 BF16, or an FP64 image (CIN = 1)
+  void* out;                  // NHWC16 BF16, or an FP64 image (COUT = 1)
+  const unsigned short* apack;
+  const float* bias;          // nullptr: none
+  const double* inv_scale;    // CIN = 1: per-image input multiplier (nullptr: 1)
+  const double* scale;        // COUT = 1: per-image output multiplier (nullptr: 1)
+  double out_mul;             // COUT = 1: constant output multiplier
+  const unsigned short* mask; // NHWC16: outputs zeroed where the mask is not > 0
+  int addto;                  // COUT = 1: out += result
+};
+
+template <int CIN, int COUT, bool RELU>
+__global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
+{
+  const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, n = lane & 15;
+  constexpr int NT = (CIN == kC) ? 9 : 1;
+  bfx4 A[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) A[t] = reinterpret_cast<const bfx4*>(a.apack)[t * kWave + lane];
+  float bv[4] = {0.f, 0.f, 0.f, 0.f};
+  if (a.bias) {
+    if (COUT == kC) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) bv[i] = a.bias[4 * g + i];
+    } else bv[0] = a.bias[0];
+  }
+  const int nx = a.nx, ny = a.ny, d = a.dil;
+  const int bpr = (nx + 15) >> 4;
+  const int64_t nblk = (int64_t)a.nb * ny * bpr;
+  const int64_t nwaves = (int64_t)gridDim.x * kWavesPB;
+  for (int64_t blk = (int64_t)blockIdx.x * kWavesPB + (threadIdx.x >> 6); blk < nblk; blk += nwaves) {
+    const int bx = (int)(blk % bpr);
+    const int64_t rowi = blk / bpr;
+    const int y = (int)(rowi % ny);
+    const int64_t b = rowi / ny;
+    const int x = bx * 16 + n;
+    fx4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (CIN == kC) {
+      const unsigned short* in = static_cast<const unsigned short*>(a.in);
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
+        bfx4 B = {0, 0, 0, 0};
+        if (x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny)
+          B = *reinterpret_cast<const bfx4*>(in + (((b * ny + ys) * nx + xs) * kC + 4 * g));
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[t], B, acc, 0, 0, 0);
+      }
+    } else {
+      const double* in = static_cast<const double*>(a.in);
+      const double is = a.inv_scale ? a.inv_scale[b] : 1.0;
+      bfx4 B;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int t = 4 * g + j;
+        unsigned short v = 0;
+        if (t < 9 && x < nx) {
+          const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
+          if (xs >= 0 && xs < nx && ys >= 0 && ys < ny) v = d2bf(in[(b * ny + ys) * nx + xs] * is);
+        }
+        B[j] = (short)v;
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[0], B, acc, 0, 0, 0);
+    }
+    if (x >= nx) continue;
+    const int64_t pix = (b * ny + y) * nx + x;
+    if (COUT == kC) {
+      bfx4 o;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        float v = acc[i] + bv[i];
+        if (RELU) v = fmaxf(v, 0.0f);
+        if (a.mask && !(bf2f(a.mask[pix * kC + 4 * g + i]) > 0.0f)) v = 0.0f;
+        o[i] = (short)f2bf(v);
+      }
+      *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + pix * kC + 4 * g) = o;
+    } else if (g == 0) {
+      const double v = (double)(acc[0] + bv[0]) * a.out_mul * (a.scale ? a.scale[b] : 1.0);
+      double* out = static_cast<double*>(a.out);
+      out[pix] = a.addto ? out[pix] + v : v;
+    }
+  }
+}
+
+// ---------------------------------------------------------- weight gradient
+struct WGradArgs {
+  int nb, nx, ny, dil;
+  const void* x;              // layer input: NHWC16 BF16, or FP64 image (CIN = 1)
+  const double* inv_scale;    // CIN = 1
+  const void* dy;             // gradient of the pre-activation: NHWC16, or FP64 (COUT = 1)
+  float* partial;             // [waves][kPW]
+};
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(kBlock) void k_wgrad(const WGradArgs a)
+{
+  const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, n = lane & 15;
+  constexpr int NT = (CIN == kC) ? 9 : 1;
+  fx4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = fx4{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.0f;
+  const int nx = a.nx, ny = a.ny, d = a.dil;
+  const int bpr = (nx + 15) >> 4;
+  const int64_t nblk = (int64_t)a.nb * ny * bpr;
+  const int wave = blockIdx.x * kWavesPB + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * kWavesPB;
+  for (int64_t blk = wave; blk < nblk; blk += nwaves) {
+    const int bx = (int)(blk % bpr);
+    const int64_t rowi = blk / bpr;
+    const int y = (int)(rowi % ny);
+    const int64_t b = rowi / ny;
+    // A[m = co = n][k = pixel 4g+j]
+    bfx4 Aop;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int px = bx * 16 + 4 * g + j;
+      unsigned short v = 0;
+      if (px < nx) {
+        const int64_t pix = (b * ny + y) * nx + px;
+        if (COUT == kC) v = static_cast<const unsigned short*>(a.dy)[pix * kC + n];
+        else if (n == 0) v = d2bf(static_cast<const double*>(a.dy)[pix]);
+      }
+      Aop[j] = (short)v;
+      bsum += bf2f(v);
+    }
+    if (CIN == kC) {
+      const unsigned short* xin = static_cast<const unsigned short*>(a.x);
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        bfx4 Bop;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int px = bx * 16 + 4 * g + j;
+          const int xs = px + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
+          unsigned short v = 0;
+          if (px < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny)
+            v = xin[((b * ny + ys) * nx + xs) * kC + n];
+          Bop[j] = (short)v;
+        }
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop, Bop, acc[t], 0, 0, 0);
+      }
+    } else {
+      // B[k = pixel][n = tap]
+      const double* xin = static_cast<const double*>(a.x);
+      const double is = a.inv_scale ? a.inv_scale[b] : 1.0;
+      bfx4 Bop;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int px = bx * 16 + 4 * g + j;
+        unsigned short v = 0;
+        if (n < 9 && px < nx) {
+          const int xs = px + (n % 3 - 1) * d, ys = y + (n / 3 - 1) * d;
+          if (xs >= 0 && xs < nx && ys >= 0 && ys < ny) v = d2bf(xin[(b * ny + ys) * nx + xs] * is);
+        }
+        Bop[j] = (short)v;
+      }
+      acc[0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop, Bop, acc[0], 0, 0, 0);
+    }
+  }
+  float* P = a.partial + (int64_t)wave * kPW;
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) P[t * 256 + (4 * g + i) * 16 + n] = acc[t][i];
+  bsum += __shfl_xor(bsum, 16, kWave);
+  bsum += __shfl_xor(bsum, 32, kWave);
+  if (g == 0) P[9 * 256 + n] = bsum;
+}
+
+// sum of the per-wavefront partials in wavefront order, into the parameter
+// layout W[cout][cin][3][3], b[cout]
+__global__ __launch_bounds__(kBlock) void k_wreduce(const float* __restrict__ partial, int nwaves,
+                                                    int cin, int cout, float* __restrict__ gw,
+                                                    float* __restrict__ gb)
+{
+  const int nw = cout * cin * 9;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nw + cout) return;
+  int off;
+  if (k < nw) {
+    const int co = k / (cin * 9), ci = (k / 9) % cin, t = k % 9;
+    // partial tile element (m, n) of tap tile T
+    if (cin == kC) off = t * 256 + co * 16 + ci;  // m = co, n = ci
+    else off = co * 16 + t;                         // single tile, n = tap
+  } else off = 9 * 256 + (k - nw);
+  float s = 0.0f;
+  for (int w = 0; w < nwaves; w++) s += partial[(int64_t)w * kPW + off];
+  if (k < nw) gw[k] = s;
+  else gb[k - nw] = s;
+}
+
+// ------------------------------------------------- packing of the A operands
+// forward: A[t][lane]: m = co = lane%16, k = 4g+j -> ci (16 -> *) or tap (1 -> *)
+// backward (input gradient of layer l): m = ci, k = co (hidden) or tap (cout 1),
+// weights of the flipped tap 8 - t
+__global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned short* __restrict__ afw,
+                       unsigned short* __restrict__ abw)
+{
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // t*256 + lane*4 + j
+  if (idx >= 9 * 256) return;
+  const int t = idx / 256, lane = (idx / 4) % 64, j = idx % 4;
+  const int g = lane / 16, n = lane % 16, k = 4 * g + j;
+  auto W = [&](int co, int ci, int tap) -> float {
+    return (co < cout && ci < cin && tap < 9) ? w[(co * cin + ci) * 9 + tap] : 0.0f;
+  };
+  // forward
+  if (cin == kC) afw[idx] = f2bf(W(n, k, t));
+  else if (t == 0) afw[idx] = f2bf(W(n, 0, k));
+  // backward
+  if (abw) {
+    if (cout == kC) abw[idx] = f2bf(W(k, n, 8 - t));
+    else if (t == 0) abw[idx] = f2bf(k < 9 ? W(0, n, 8 - k) : 0.0f);
+  }
+}
+
+
+// ------------------------------------------------------------- small kernels
+// per-image sums of squares (partials: P blocks per image)
+__global__ __launch_bounds__(kBlock) void k_sumsq(int64_t npix, int P, const double* __restrict__ x,
+                                                  double* __restrict__ part)
+{
+  const int b = blockIdx.y, p = blockIdx.x;
+  const double* xb = x + (int64_t)b * npix;
+  double s = 0.0;
+  for (int64_t i = (int64_t)p * kBlock + threadIdx.x; i < npix; i += (int64_t)P * kBlock) s += xb[i] * xb[i];
+  __shared__ double sh[kWavesPB];
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(int64_t)b * P + p] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+// s = RMS, inv = 1/s (an all-zero image keeps s = 1)
+__global__ void k_rms_final(int nb, int P, int64_t npix, const double* part, double* s, double* inv)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double t = 0.0;
+  for (int p = 0; p < P; p++) t += part[(int64_t)b * P + p];
+  double v = sqrt(t / (double)npix);
+  if (!(v > 0.0)) v = 1.0;
+  s[b] = v;
+  inv[b] = 1.0 / v;
+}
+
+// y = (I - ratio*L) x on nb stacked nx*ny Dirichlet grids (unit spacing)
+__global__ __launch_bounds__(kBlock) void k_applyA(int nb, int nx, int ny, double ratio,
+                                                   const double* __restrict__ x, double* __restrict__ y)
+{
+  const int64_t n = (int64_t)nb * nx * ny;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
+    const int i = (int)(k % nx);
+    const int j = (int)((k / nx) % ny);
+    const double c = x[k];
+    const double w = i > 0 ? x[k - 1] : 0.0, e = i < nx - 1 ? x[k + 1] : 0.0;
+    const double s = j > 0 ? x[k - nx] : 0.0, no = j < ny - 1 ? x[k + nx] : 0.0;
+    y[k] = c - ratio * ((((w + e) + s) + no) - 4.0 * c);
+  }
+}
+
+// d = u - e*inv_s (u: network output in normalised units)
+__global__ __launch_bounds__(kBlock) void k_diff(int64_t npix, int nb, const double* __restrict__ u,
+                                                 const double* __restrict__ e, const double* __restrict__ inv,
+                                                 double* __restrict__ dd)
+{
+  const int64_t n = npix * nb;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+    dd[k] = u[k] - e[k] * inv[k / npix];
+}
+
+// per-image partial dot products d.Ad and e.r
+__global__ __launch_bounds__(kBlock) void k_dot2(int64_t npix, int P, const double* __restrict__ a1,
+                                                 const double* __restrict__ b1, const double* __restrict__ a2,
+                                                 const double* __restrict__ b2, double* __restrict__ part)
+{
+  const int b = blockIdx.y, p = blockIdx.x;
+  const int64_t o = (int64_t)b * npix;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = (int64_t)p * kBlock + threadIdx.x; i < npix; i += (int64_t)P * kBlock) {
+    s1 += a1[o + i] * b1[o + i];
+    s2 += a2[o + i] * b2[o + i];
+  }
+  __shared__ double sh[2][kWavesPB];
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    s1 += __shfl_xor(s1, off, kWave);
+    s2 += __shfl_xor(s2, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    sh[0][threadIdx.x / kWave] = s1;
+    sh[1][threadIdx.x / kWave] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((int64_t)b * P + p) * 2] = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
+    part[((int64_t)b * P + p) * 2 + 1] = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
+  }
+}
+// loss_b = (d.Ad) / (e.r * inv_s^2); coef_b = 2*out_mul / (nb * e.r * inv_s^2)
+__global__ void k_loss_final(int nb, int P, const double* part, const double* inv, double out_mul,
+                             double* loss_b, double* coef)
+{
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int p = 0; p < P; p++) {
+    s1 += part[((int64_t)b * P + p) * 2];
+    s2 += part[((int64_t)b * P + p) * 2 + 1];
+  }
+  const double er = s2 * inv[b] * inv[b];
+  loss_b[b] = s1 / er;
+  coef[b] = 2.0 * out_mul / ((double)nb * er);
+}
+__global__ __launch_bounds__(kBlock) void k_scale_img(int64_t npix, int nb, const double* __restrict__ c,
+                                                      const double* __restrict__ x, double* __restrict__ y)
+{
+  const int64_t n = npix * nb;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+    y[k] = c[k / npix] * x[k];
+}
+__global__ void k_mean(int nb, const double* v, double* out)
+{
+  if (blockIdx.x || threadIdx.x) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += v[b];
+  *out = s / (double)nb;
+}
+
+// Adam on the FP32 parameters
+__global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float g = gr[k];
+  m[k] = b1 * m[k] + (1.0f - b1) * g;
+  v[k] = b2 * v[k] + (1.0f - b2) * g * g;
+  const float mh = m[k] / c1, vh = v[k] / c2;
+  p[k] = p[k] - lr * mh / (sqrtf(vh) + eps);
+}
+
+// gather training pairs idx[0..nb) into contiguous batch buffers
+__global__ __launch_bounds__(kBlock) void k_gather(int64_t npix, int nb, const int* __restrict__ idx,
+                                                   const double* __restrict__ r, const double* __restrict__ e,
+                                                   double* __restrict__ rb, double* __restrict__ eb)
+{
+  const int64_t n = npix * nb;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
+    const int64_t src = (int64_t)idx[k / npix] * npix + k % npix;
+    rb[k] = r[src];
+    eb[k] = e[src];
+  }
+}
+
+// ---------------------------------------------------------- training data
+__device__ inline uint64_t smix(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ inline double u01(uint64_t h) { return ((h >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+// sine-mode coefficients c[i][k][l] = N(0,1)/|k| up to the sample's cut-off
+// (Box-Muller on counter-based hashes of (seed + i, k, l))
+__global__ __launch_bounds__(kBlock) void k_coeffs(int nsamp, int n, uint64_t seed, double* __restrict__ c)
+{
+  const int64_t tot = (int64_t)nsamp * n * n;
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = q / ((int64_t)n * n);
+    const int k = (int)((q / n) % n) + 1, l = (int)(q % n) + 1;
+    const uint64_t si = smix(seed + (uint64_t)i);
+    const double cut = 4.0 + (n - 4.0) * u01(smix(si ^ 0x5851F42D4C957F2Dull));
+    const double kk = sqrt((double)(k * k + l * l));
+    double v = 0.0;
+    if (kk <= cut) {
+      const uint64_t h1 = smix(si + 2ull * ((uint64_t)k * 4096ull + (uint64_t)l));
+      const uint64_t h2 = smix(h1);
+      v = sqrt(-2.0 * log(u01(h1))) * cos(6.283185307179586 * u01(h2)) / kk;
+    }
+    c[q] = v;
+  }
+}
+// out[i][y][x] = sum_k S[x][k] in[i][k][y'] ... two passes of the separable
+// sine synthesis: pass 0 t[i][k][x] = sum_l c[i][k][l] S[x][l];
+// pass 1 r[i][y][x] = sum_k S[y][k] t[i][k][x]
+__global__ __launch_bounds__(kBlock) void k_synth(int nsamp, int n, int pass, const double* __restrict__ S,
+                                                  const double* __restrict__ in, double* __restrict__ out)
+{
+  const int64_t tot = (int64_t)nsamp * n * n;
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+    const int64_t i = q / ((int64_t)n * n);
+    const int a = (int)((q / n) % n), bcol = (int)(q % n);
+    const double* base = in + i * n * n;
+    double s = 0.0;
+    if (pass == 0)
+      for (int l = 0; l < n; l++) s += base[(int64_t)a * n + l] * S[(int64_t)bcol * n + l];
+    else
+      for (int k = 0; k < n; k++) s += S[(int64_t)a * n + k] * base[(int64_t)k * n + bcol];
+    out[q] = s;
+  }
+}
+
+// batched Jacobi-PCG, one workgroup per pair (constant diagonal dg)
+__device__ inline double block_sum(double v, double* sh)
+{
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = v;
+  __syncthreads();
+  return ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+struct CGState {
+  double rz, bb;
+  int done, iters;
+};
+__global__ __launch_bounds__(kBlock) void k_cg_init(int n, double dg, const double* __restrict__ bvec,
+                                                    double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ p, CGState* __restrict__ st)
+{
+  const int64_t o = (int64_t)blockIdx.x * n * n;
+  __shared__ double sh[kWavesPB];
+  double rz = 0.0, bb = 0.0;
+  for (int k = threadIdx.x; k < n * n; k += kBlock) {
+    const double b = bvec[o + k];
+    x[o + k] = 0.0;
+    r[o + k] = b;
+    p[o + k] = b / dg;
+    rz += b * (b / dg);
+    bb += b * b;
+  }
+  rz = block_sum(rz, sh);
+  bb = block_sum(bb, sh);
+  if (threadIdx.x == 0) st[blockIdx.x] = CGState{rz, bb, bb == 0.0 ? 1 : 0, 0};
+}
+__global__ __launch_bounds__(kBlock) void k_cg_iter(int n, double ratio, double dg, double tol2,
+                                                    double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ p, double* __restrict__ ap,
+                                                    CGState* __restrict__ st)
+{
+  if (st[blockIdx.x].done) return;
+  const int64_t o = (int64_t)blockIdx.x * n * n;
+  __shared__ double sh[kWavesPB];
+  double pap = 0.0;
+  for (int k = threadIdx.x; k < n * n; k += kBlock) {
+    const int i = k % n, j = k / n;
+    const double c = p[o + k];
+    const double w = i > 0 ? p[o + k - 1] : 0.0, e = i < n - 1 ? p[o + k + 1] : 0.0;
+    const double s = j > 0 ? p[o + k - n] : 0.0, no = j < n - 1 ? p[o + k + n] : 0.0;
+    const double v = c - ratio * ((((w + e) + s) + no) - 4.0 * c);
+    ap[o + k] = v;
+    pap += c * v;
+  }
+  pap = block_sum(pap, sh);
+  const double alpha = st[blockIdx.x].rz / pap;
+  double rr = 0.0, rz = 0.0;
+  for (int k = threadIdx.x; k < n * n; k += kBlock) {
+    x[o + k] += alpha * p[o + k];
+    const double rk = r[o + k] - alpha * ap[o + k];
+    r[o + k] = rk;
+    rr += rk * rk;
+    rz += rk * (rk / dg);
+  }
+  rr = block_sum(rr, sh);
+  rz = block_sum(rz, sh);
+  const double beta = rz / st[blockIdx.x].rz;
+  for (int k = threadIdx.x; k < n * n; k += kBlock) p[o + k] = r[o + k] / dg + beta * p[o + k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    CGState& S = st[blockIdx.x];
+    S.rz = rz;
+    S.iters++;
+    if (rr <= tol2 * S.bb) S.done = 1;
+  }
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 8192); }
+
+}  // namespace
+
+// ============================================================== host side
+struct LearnedPrec {
+  int nl = 0;
+  int dil[kMaxL], cin[kMaxL], cout[kMaxL], woff[kMaxL], boff[kMaxL];
+  int np = 0;
+  double out_scale = 1.0;
+  ceda::Exec* ex = nullptr;
+  float* d_p = nullptr;      // FP32 parameters
+  float* d_g = nullptr;      // gradient
+  float *d_m = nullptr, *d_v = nullptr;
+  unsigned short *d_afw = nullptr, *d_abw = nullptr;  // [layer][9*256]
+  int adam_t = 0;
+  // workspace
+  int64_t cap_pix = 0;
+  int cap_nb = 0;
+  unsigned short* act[kMaxL] = {};  // act[l] = input of layer l (l >= 1), NHWC16
+  unsigned short *gA = nullptr, *gB = nullptr;  // gradient ping-pong, NHWC16
+  double *u = nullptr, *dd = nullptr, *ad = nullptr, *gy = nullptr;
+  double *sc = nullptr, *isc = nullptr, *part = nullptr, *lossb = nullptr, *coef = nullptr;
+  float* wpart = nullptr;
+  int part_cap = 0;
+};
+
+namespace {
+
+void lp_pack(LearnedPrec* L)
+{
+  hipStream_t s = L->ex->stream;
+  for (int l = 0; l < L->nl; l++)
+    hipLaunchKernelGGL(k_pack, dim3(9), dim3(256), 0, s, L->d_p + L->woff[l], L->cin[l], L->cout[l],
+                       L->d_afw + (int64_t)l * 9 * 256, l > 0 ? L->d_abw + (int64_t)l * 9 * 256 : nullptr);
+  CEDA_HIP_CHECK(hipGetLastError());
+}
+
+void lp_workspace(LearnedPrec* L, int nb, int64_t npix)
+{
+  const int64_t tot = npix * nb;
+  const int P = (int)std::min<int64_t>(64, (npix + 4095) / 4096);
+  if (tot > L->cap_pix) {
+    CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+    for (int l = 0; l < kMaxL; l++)
+      if (L->act[l]) {
+        CEDA_HIP_CHECK(hipFree(L->act[l]));
+        L->act[l] = nullptr;
+      }
+    void* ps[] = {L->gA, L->gB, L->u, L->dd, L->ad, L->gy};
+    for (void* p : ps)
+      if (p) CEDA_HIP_CHECK(hipFree(p));
+    for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(hipMalloc(&L->act[l], sizeof(unsigned short) * kC * tot));
+    CEDA_HIP_CHECK(hipMalloc(&L->gA, sizeof(unsigned short) * kC * tot));
+    CEDA_HIP_CHECK(hipMalloc(&L->gB, sizeof(unsigned short) * kC * tot));
+    CEDA_HIP_CHECK(hipMalloc(&L->u, sizeof(double) * tot));
+    CEDA_HIP_CHECK(hipMalloc(&L->dd, sizeof(double) * tot));
+    CEDA_HIP_CHECK(hipMalloc(&L->ad, sizeof(double) * tot));
+    CEDA_HIP_CHECK(hipMalloc(&L->gy, sizeof(double) * tot));
+    L->cap_pix = tot;
+  }
+  if (nb > L->cap_nb || nb * P * 2 > L->part_cap) {
+    CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+    void* ps[] = {L->sc, L->isc, L->part, L->lossb, L->coef};
+    for (void* p : ps)
+      if (p) CEDA_HIP_CHECK(hipFree(p));
+    const int cnb = std::max(nb, L->cap_nb);
+    const int pc = std::max(cnb * 64 * 2, L->part_cap);
+    CEDA_HIP_CHECK(hipMalloc(&L->sc, sizeof(double) * cnb));
+    CEDA_HIP_CHECK(hipMalloc(&L->isc, sizeof(double) * cnb));
+    CEDA_HIP_CHECK(hipMalloc(&L->part, sizeof(double) * pc));
+    CEDA_HIP_CHECK(hipMalloc(&L->lossb, sizeof(double) * cnb));
+    CEDA_HIP_CHECK(hipMalloc(&L->coef, sizeof(double) * cnb));
+    L->cap_nb = cnb;
+    L->part_cap = pc;
+  }
+  if (!L->wpart) CEDA_HIP_CHECK(hipMalloc(&L->wpart, sizeof(float) * kPW * kWGradWaves));
+}
+
+int rms_parts(int64_t npix) { return (int)std::min<int64_t>(64, (npix + 4095) / 4096); }
+
+// per-image scale s = RMS(in) and 1/s
+void lp_scale(LearnedPrec* L, int nb, int64_t npix, const double* in)
+{
+  hipStream_t s = L->ex->stream;
+  const int P = rms_parts(npix);
+  hipLaunchKernelGGL(k_sumsq, dim3(P, nb), dim3(kBlock), 0, s, npix, P, in, L->part);
+  hipLaunchKernelGGL(k_rms_final, dim3((nb + 63) / 64), dim3(64), 0, s, nb, P, npix, L->part, L->sc, L->isc);
+}
+
+template <int CIN, int COUT, bool RELU>
+void conv(LearnedPrec* L, const ConvArgs& a)
+{
+  const int64_t blocks = (int64_t)a.nb * a.ny * ((a.nx + 15) / 16);
+  const unsigned grid = (unsigned)std::min<int64_t>((blocks + kWavesPB - 1) / kWavesPB, 16384);
+  hipLaunchKernelGGL((k_conv<CIN, COUT, RELU>), dim3(grid), dim3(kBlock), 0, L->ex->stream, a);
+}
+
+// forward pass; the output (FP64) goes to out with out = [out +] scale*out_scale*N
+// (scale = per-image RMS, or 1 in normalised units when normalised = true)
+void lp_forward(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto,
+                bool normalised)
+{
+  ConvArgs a{};
+  a.nb = nb;
+  a.nx = nx;
+  a.ny = ny;
+  for (int l = 0; l < L->nl; l++) {
+    a.dil = L->dil[l];
+    a.apack = L->d_afw + (int64_t)l * 9 * 256;
+    a.bias = L->d_p + L->boff[l];
+    a.mask = nullptr;
+    const bool first = (l == 0), last = (l == L->nl - 1);
+    a.in = first ? (const void*)in : (const void*)L->act[l];
+    a.inv_scale = first ? L->isc : nullptr;
+    if (last) {
+      a.out = out;
+      a.scale = normalised ? nullptr : L->sc;
+      a.out_mul = L->out_scale;
+      a.addto = addto;
+      if (first) conv<1, 1, false>(L, a);
+      else conv<kC, 1, false>(L, a);
+    } else {
+      a.out = L->act[l + 1];
+      if (first) conv<1, kC, true>(L, a);
+      else conv<kC, kC, true>(L, a);
+    }
+  }
+  CEDA_HIP_CHECK(hipGetLastError());
+}
+
+template <int CIN, int COUT>
+void wgrad(LearnedPrec* L, int l, int nb, int nx, int ny, const void* x, const void* dy)
+{
+  WGradArgs w{nb, nx, ny, L->dil[l], x, L->isc, dy, L->wpart};
+  hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(kWGradWaves / kWavesPB), dim3(kBlock), 0, L->ex->stream, w);
+  const int nparam = L->cout[l] * L->cin[l] * 9 + L->cout[l];
+  hipLaunchKernelGGL(k_wreduce, dim3((nparam + kBlock - 1) / kBlock), dim3(kBlock), 0, L->ex->stream,
+                     L->wpart, kWGradWaves, L->cin[l], L->cout[l], L->d_g + L->woff[l], L->d_g + L->boff[l]);
+}
+
+// loss (device scalar *dloss) and parameter gradient (L->d_g) of a batch
+void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const double* r,
+                 const double* e, double* dloss)
+{
+  hipStream_t s = L->ex->stream;
+  const int64_t npix = (int64_t)nx * ny, tot = npix * nb;
+  lp_workspace(L, nb, npix);
+  lp_scale(L, nb, npix, r);
+  lp_forward(L, nb, nx, ny, r, L->u, 0, true);
+  const unsigned gg = grid_for(tot);
+  hipLaunchKernelGGL(k_diff, dim3(gg), dim3(kBlock), 0, s, npix, nb, L->u, e, L->isc, L->dd);
+  hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nb, nx, ny, ratio, L->dd, L->ad);
+  const int P = rms_parts(npix);
+  hipLaunchKernelGGL(k_dot2, dim3(P, nb), dim3(kBlock), 0, s, npix, P, L->dd, L->ad, e, r, L->part);
+  hipLaunchKernelGGL(k_loss_final, dim3((nb + 63) / 64), dim3(64), 0, s, nb, P, L->part, L->isc,
+                     L->out_scale, L->lossb, L->coef);
+  hipLaunchKernelGGL(k_mean, dim3(1), dim3(1), 0, s, nb, L->lossb, dloss);
+  // dL/d(pre-activation of the last layer) = coef_b * A d
+  hipLaunchKernelGGL(k_scale_img, dim3(gg), dim3(kBlock), 0, s, npix, nb, L->coef, L->ad, L->gy);
+  // backward through the layers
+  const int last = L->nl - 1;
+  const void* dy = L->gy;  // FP64 for the last layer
+  unsigned short* gout = L->gA;
+  for (int l = last; l >= 0; l--) {
+    const bool first = (l == 0), isl = (l == last);
+    const void* x = first ? (const void*)r : (const void*)L->act[l];
+    if (first && isl) wgrad<1, 1>(L, l, nb, nx, ny, x, dy);
+    else if (first) wgrad<1, kC>(L, l, nb, nx, ny, x, dy);
+    else if (isl) wgrad<kC, 1>(L, l, nb, nx, ny, x, dy);
+    else wgrad<kC, kC>(L, l, nb, nx, ny, x, dy);
+    if (first) break;
+    ConvArgs a{};
+    a.nb = nb;
+    a.nx = nx;
+    a.ny = ny;
+    a.dil = L->dil[l];
+    a.in = dy;
+    a.out = gout;
+    a.apack = L->d_abw + (int64_t)l * 9 * 256;
+    a.mask = L->act[l];
+    if (isl) conv<1, kC, false>(L, a);
+    else conv<kC, kC, false>(L, a);
+    dy = gout;
+    gout = (gout == L->gA) ? L->gB : L->gA;
+  }
+  CEDA_HIP_CHECK(hipGetLastError());
+}
+
+void adam_step(LearnedPrec* L, double lr)
+{
+  L->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - powf(b1, (float)L->adam_t), c2 = 1.0f - powf(b2, (float)L->adam_t);
+  hipLaunchKernelGGL(k_adam, dim3((L->np + 255) / 256), dim3(256), 0, L->ex->stream, L->np, L->d_p, L->d_g,
+                     L->d_m, L->d_v, (float)lr, b1, b2, 1e-8f, c1, c2);
+  lp_pack(L);
+}
+
+}  // namespace
+
+extern "C" {
+
+LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx)
+{
+  if (nlayer < 2 || nlayer > kMaxL || !dil || !ctx) return nullptr;
+  for (int l = 0; l < nlayer; l++)
+    if (dil[l] < 1 || dil[l] > 64) return nullptr;
+  LearnedPrec* L = new LearnedPrec();
+  L->nl = nlayer;
+  L->ex = ceda::context_exec(ctx);
+  int off = 0;
+  for (int l = 0; l < nlayer; l++) {
+    L->dil[l] = dil[l];
+    L->cin[l] = (l == 0) ? 1 : kC;
+    L->cout[l] = (l == nlayer - 1) ? 1 : kC;
+    L->woff[l] = off;
+    off += L->cout[l] * L->cin[l] * 9;
+    L->boff[l] = off;
+    off += L->cout[l];
+  }
+  L->np = off;
+  CEDA_HIP_CHECK(hipMalloc(&L->d_p, sizeof(float) * off));
+  CEDA_HIP_CHECK(hipMalloc(&L->d_g, sizeof(float) * off));
+  CEDA_HIP_CHECK(hipMalloc(&L->d_m, sizeof(float) * off));
+  CEDA_HIP_CHECK(hipMalloc(&L->d_v, sizeof(float) * off));
+  CEDA_HIP_CHECK(hipMemset(L->d_p, 0, sizeof(float) * off));
+  CEDA_HIP_CHECK(hipMemset(L->d_m, 0, sizeof(float) * off));
+  CEDA_HIP_CHECK(hipMemset(L->d_v, 0, sizeof(float) * off));
+  CEDA_HIP_CHECK(hipMalloc(&L->d_afw, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(hipMalloc(&L->d_abw, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(hipMemset(L->d_afw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(hipMemset(L->d_abw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
+  lp_pack(L);
+  CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+  return L;
+}
+
+void LearnedPrec_Destroy(LearnedPrec* L)
+{
+  if (!L) return;
+  CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+  void* ps[] = {L->d_p, L->d_g, L->d_m, L->d_v, L->d_afw, L->d_abw, L->gA, L->gB, L->u, L->dd,
+                L->ad, L->gy, L->sc, L->isc, L->part, L->lossb, L->coef, L->wpart};
+  for (void* p : ps)
+    if (p) CEDA_HIP_CHECK(hipFree(p));
+  for (int l = 0; l < kMaxL; l++)
+    if (L->act[l]) CEDA_HIP_CHECK(hipFree(L->act[l]));
+  delete L;
+}
+
+int LearnedPrec_NumParams(const LearnedPrec* L) { return L ? L->np : -1; }
+
+int LearnedPrec_SetParams(LearnedPrec* L, const float* p)
+{
+  if (!L || !p) return -1;
+  CEDA_HIP_CHECK(hipMemcpyAsync(L->d_p, p, sizeof(float) * L->np, hipMemcpyHostToDevice, L->ex->stream));
+  lp_pack(L);
+  CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+  return 0;
+}
+
+int LearnedPrec_GetParams(const LearnedPrec* L, float* p)
+{
+  if (!L || !p) return -1;
+  CEDA_HIP_CHECK(hipMemcpyAsync(p, L->d_p, sizeof(float) * L->np, hipMemcpyDeviceToHost, L->ex->stream));
+  CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+  return 0;
+}
+
+int LearnedPrec_InitIdentity(LearnedPrec* L, double noise, unsigned long seed)
+{
+  if (!L || noise < 0.0) return -1;
+  std::vector<float> p(L->np, 0.0f);
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  for (int l = 0; l < L->nl; l++) {
+    const int nw = L->cout[l] * L->cin[l] * 9;
+    for (int k = 0; k < nw; k++) p[L->woff[l] + k] = (float)(noise * U(rng));
+    auto W = [&](int co, int ci, int t) -> float& { return p[L->woff[l] + (co * L->cin[l] + ci) * 9 + t]; };
+    const bool first = (l == 0), last = (l == L->nl - 1);
+    if (first && last) W(0, 0, 4) += 1.0f;
+    else if (first) {
+      W(0, 0, 4) += 1.0f;   // relu(x)
+      W(1, 0, 4) -= 1.0f;   // relu(-x)
+    } else if (last) {
+      W(0, 0, 4) += 1.0f;
+      W(0, 1, 4) -= 1.0f;
+    } else {
+      W(0, 0, 4) += 1.0f;
+      W(1, 1, 4) += 1.0f;
+    }
+  }
+  L->adam_t = 0;
+  CEDA_HIP_CHECK(hipMemsetAsync(L->d_m, 0, sizeof(float) * L->np, L->ex->stream));
+  CEDA_HIP_CHECK(hipMemsetAsync(L->d_v, 0, sizeof(float) * L->np, L->ex->stream));
+  return LearnedPrec_SetParams(L, p.data());
+}
+
+int LearnedPrec_SetOutputScale(LearnedPrec* L, double s)
+{
+  if (!L || !(s > 0.0)) return -1;
+  L->out_scale = s;
+  return 0;
+}
+
+double LearnedPrec_GetOutputScale(const LearnedPrec* L) { return L ? L->out_scale : 0.0; }
+
+int LearnedPrec_Apply(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto)
+{
+  if (!L || nb < 1 || nx < 1 || ny < 1 || !in || !out) return -1;
+  const int64_t npix = (int64_t)nx * ny;
+  lp_workspace(L, nb, npix);
+  lp_scale(L, nb, npix, in);
+  lp_forward(L, nb, nx, ny, in, out, addto, false);
+  return 0;
+}
+
+int LearnedPrec_LossGrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const double* r,
+                         const double* e, double* loss, float* grad)
+{
+  if (!L || nb < 1 || nx < 1 || ny < 1 || !r || !e || !loss) return -1;
+  double* dl = nullptr;
+  CEDA_HIP_CHECK(hipMalloc(&dl, sizeof(double)));
+  lp_lossgrad(L, nb, nx, ny, ratio, r, e, dl);
+  CEDA_HIP_CHECK(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, L->ex->stream));
+  if (grad)
+    CEDA_HIP_CHECK(hipMemcpyAsync(grad, L->d_g, sizeof(float) * L->np, hipMemcpyDeviceToHost, L->ex->stream));
+  CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+  CEDA_HIP_CHECK(hipFree(dl));
+  return 0;
+}
+
+int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, unsigned long seed,
+                                double* r, double* e, int* max_iters, SUNContext ctx)
+{
+  if (nsamp < 1 || n < 2 || n > 1024 || !(ratio >= 0.0) || !(reltol > 0.0) || !r || !e || !ctx) return -1;
+  hipStream_t s = ceda::context_exec(ctx)->stream;
+  const int64_t tot = (int64_t)nsamp * n * n;
+  std::vector<double> S((size_t)n * n);
+  for (int x = 0; x < n; x++)
+    for (int k = 0; k < n; k++) S[(size_t)x * n + k] = sin(M_PI * (double)(k + 1) * (double)(x + 1) / (double)(n + 1));
+  double *dS = nullptr, *tmp = nullptr, *p = nullptr, *ap = nullptr;
+  CGState* st = nullptr;
+  CEDA_HIP_CHECK(hipMalloc(&dS, sizeof(double) * n * n));
+  CEDA_HIP_CHECK(hipMalloc(&tmp, sizeof(double) * tot));
+  CEDA_HIP_CHECK(hipMalloc(&p, sizeof(double) * tot));
+  CEDA_HIP_CHECK(hipMalloc(&ap, sizeof(double) * tot));
+  CEDA_HIP_CHECK(hipMalloc(&st, sizeof(CGState) * nsamp));
+  CEDA_HIP_CHECK(hipMemcpyAsync(dS, S.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  const unsigned gg = grid_for(tot);
+  // fields: even samples smooth sine sums, odd samples (I - ratio L) applied to
+  // one (rough: the high-frequency content a Krylov residual carries)
+  hipLaunchKernelGGL(k_coeffs, dim3(gg), dim3(kBlock), 0, s, nsamp, n, (uint64_t)seed, p);
+  hipLaunchKernelGGL(k_synth, dim3(gg), dim3(kBlock), 0, s, nsamp, n, 0, dS, p, tmp);
+  hipLaunchKernelGGL(k_synth, dim3(gg), dim3(kBlock), 0, s, nsamp, n, 1, dS, tmp, r);
+  hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nsamp, n, n, ratio, r, tmp);
+  for (int i = 1; i < nsamp; i += 2)
+    CEDA_HIP_CHECK(hipMemcpyAsync(r + (int64_t)i * n * n, tmp + (int64_t)i * n * n, sizeof(double) * n * n,
+                                  hipMemcpyDeviceToDevice, s));
+  // Jacobi-PCG on every pair
+  const double dg = 1.0 + 4.0 * ratio;
+  hipLaunchKernelGGL(k_cg_init, dim3(nsamp), dim3(kBlock), 0, s, n, dg, r, e, tmp, p, st);
+  std::vector<CGState> hst(nsamp);
+  const int maxit = 20000;
+  int it = 0;
+  bool all = false;
+  while (!all && it < maxit) {
+    for (int k = 0; k < 50; k++, it++)
+      hipLaunchKernelGGL(k_cg_iter, dim3(nsamp), dim3(kBlock), 0, s, n, ratio, dg, reltol * reltol, e, tmp, p,
+                         ap, st);
+    CEDA_HIP_CHECK(hipMemcpyAsync(hst.data(), st, sizeof(CGState) * nsamp, hipMemcpyDeviceToHost, s));
+    CEDA_HIP_CHECK(hipStreamSynchronize(s));
+    all = true;
+    for (const CGState& c : hst) all = all && c.done;
+  }
+  int mx = 0;
+  for (const CGState& c : hst) mx = std::max(mx, c.iters);
+  if (max_iters) *max_iters = mx;
+  CEDA_HIP_CHECK(hipGetLastError());
+  CEDA_HIP_CHECK(hipStreamSynchronize(s));
+  void* ps[] = {dS, tmp, p, ap, st};
+  for (void* q : ps) CEDA_HIP_CHECK(hipFree(q));
+  return all ? 0 : 1;
+}
+
+int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const double* r, const double* e,
+                      int steps, int batch, double lr0, unsigned long seed, double* loss_hist)
+{
+  if (!L || nsamp < 1 || n < 2 || steps < 1 || batch < 1 || !r || !e || !(lr0 > 0.0)) return -1;
+  hipStream_t s = L->ex->stream;
+  const int64_t npix = (int64_t)n * n;
+  double *rb = nullptr, *eb = nullptr, *dloss = nullptr;
+  int* didx = nullptr;
+  CEDA_HIP_CHECK(hipMalloc(&rb, sizeof(double) * npix * batch));
+  CEDA_HIP_CHECK(hipMalloc(&eb, sizeof(double) * npix * batch));
+  CEDA_HIP_CHECK(hipMalloc(&dloss, sizeof(double) * steps));
+  CEDA_HIP_CHECK(hipMalloc(&didx, sizeof(int) * batch * steps));
+  std::vector<int> idx((size_t)batch * steps);
+  std::mt19937_64 rng(seed);
+  std::uniform_int_distribution<int> U(0, nsamp - 1);
+  for (int& v : idx) v = U(rng);
+  CEDA_HIP_CHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, s));
+  const unsigned gg = grid_for(npix * batch);
+  for (int it = 0; it < steps; it++) {
+    hipLaunchKernelGGL(k_gather, dim3(gg), dim3(kBlock), 0, s, npix, batch, didx + (int64_t)it * batch, r, e, rb, eb);
+    lp_lossgrad(L, batch, n, n, ratio, rb, eb, dloss + it);
+    const double lr = lr0 * 0.5 * (1.0 + cos(M_PI * (double)it / (double)steps));
+    adam_step(L, lr);
+  }
+  if (loss_hist)
+    CEDA_HIP_CHECK(hipMemcpyAsync(loss_hist, dloss, sizeof(double) * steps, hipMemcpyDeviceToHost, s));
+  CEDA_HIP_CHECK(hipStreamSynchronize(s));
+  void* ps[] = {rb, eb, dloss, didx};
+  for (void* q : ps) CEDA_HIP_CHECK(hipFree(q));
+  return 0;
+}
+
+int DiffusionProblem_SetLearnedPrec(DiffusionProblem* P, LearnedPrec* lp)
+{
+  if (!P) return -1;
+  P->learned = lp;
+  return 0;
+}
+
+int LearnedPSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok, sunbooleantype* jcurPtr,
+                  sunrealtype gamma, void* user_data)
+{
+  DiffusionProblem* P = static_cast<DiffusionProblem*>(user_data);
+  if (!P || !P->learned) return -1;
+  const int r = PSetup(t, u, f, jok, jcurPtr, gamma, P);
+  if (r) return r;
+  const double d = 1.0 + gamma * (2.0 * P->p.kx / (P->dx * P->dx) + 2.0 * P->p.ky / (P->dy * P->dy));
+  P->learned->out_scale = 1.0 / d;
+  return 0;
+}
+
+int LearnedPSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z, sunrealtype gamma,
+                  sunrealtype delta, int lr, void* user_data)
+{
+  DiffusionProblem* P = static_cast<DiffusionProblem*>(user_data);
+  if (!P || !P->learned || N_VGetLength(r) != P->n || N_VGetLength(z) != P->n) return -1;
+  return LearnedPrec_Apply(P->learned, 1, (int)P->p.nx, (int)P->p.ny, N_VGetDeviceArrayPointer(r),
+                           N_VGetDeviceArrayPointer(z), 0);
+}
+
+}  // extern "C"

@@ -144,6 +144,14 @@ int ARKodeSetNonlinear(void* arkode_mem);
 /* [ext] linear solves stop at ||r||_2 <= reltol ||b||_2 instead of the
    ARKODE weighted-norm test (used by the fixed-step implicit benchmark) */
 int ARKodeSetLSRelativeTolerance(void* arkode_mem, sunrealtype reltol);
+/* [ext] fallback of a (F)GMRES linear solve: when it stops at its iteration
+   cap short of the tolerance, the same system is solved again by PCG with at
+   most maxl iterations and the preconditioner solve psolve(user_data) (for
+   example the Jacobi PSolve of preconditioner_jacobi.cpp:44-62, whose setup
+   the primary preconditioner setup must have done); maxl = 0 removes it.  No
+   reference counterpart (the reference uses a single Krylov method). */
+int ARKodeSetLSFallbackPCG(void* arkode_mem, int maxl, ARKLsPrecSolveFn psolve, void* user_data);
+int ARKodeGetNumLSFallbacks(void* arkode_mem, long* nfallbacks);
 int ARKodeGetNumLinIters(void* arkode_mem, long* nliters);
 
 SUNLinearSolver SUNLinSol_PCG(N_Vector y, int pretype, int maxl, SUNContext sunctx);

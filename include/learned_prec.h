@@ -1,0 +1,99 @@
+/* ---------------------------------------------------------------------------
+ * [ext] Learned preconditioner for the implicit path: a small convolutional
+ * network (3x3 convolutions, 16 hidden channels, ReLU between layers, BF16
+ * weights and activations, FP32 accumulation on MFMA) that maps a residual r
+ * to a correction z ~ (I - gamma J)^{-1} r inside FGMRES.
+ *
+ * There is no reference counterpart.  It plugs into the preconditioner slot
+ * of the reference's implicit path (ARKodeSetPreconditioner, main.cpp:
+ * 307-313; the Jacobi pair of preconditioner_jacobi.cpp:9-62 has the same
+ * signatures) and is used with a flexible Krylov method (SUNLinSol_SPFGMR),
+ * because the network is not a fixed linear operator.  The FP64 FGMRES
+ * residual decides convergence; when FGMRES does not converge within its
+ * iteration cap the system is re-solved by Jacobi-PCG (ARKodeSetLSFallbackPCG).
+ *
+ * Network.  nlayer layers (2..8): layer 0 maps 1 -> 16 channels, the hidden
+ * layers 16 -> 16, the last 16 -> 1; layer l has dilation dil[l].  Zero
+ * padding at the grid edge (the Dirichlet ghosts).  Applied to an image r:
+ *   s = RMS(r);  x_0 = bf16(r / s);  x_{l+1} = bf16(relu(W_l * x_l + b_l));
+ *   N(r) = s * out_scale * (W_L * x_L + b_L)      (FP64 output)
+ * Parameters (FP32, NumParams of them): for each layer W[cout][cin][3][3]
+ * (row-major taps, dy outer) then b[cout].
+ * ------------------------------------------------------------------------- */
+#ifndef LEARNED_PREC_H
+#define LEARNED_PREC_H
+
+#include "arkode_hip.h"
+#include "diffusion_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct LearnedPrec LearnedPrec;
+
+LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx);
+void LearnedPrec_Destroy(LearnedPrec* lp);
+int LearnedPrec_NumParams(const LearnedPrec* lp);
+/* host FP32 parameters in the layout above */
+int LearnedPrec_SetParams(LearnedPrec* lp, const float* params);
+int LearnedPrec_GetParams(const LearnedPrec* lp, float* params);
+/* deterministic start: the identity map through the ReLUs (channel 0 carries
+   relu(x), channel 1 relu(-x), the last layer takes their difference) plus
+   uniform noise of the given amplitude (seeded) on every weight */
+int LearnedPrec_InitIdentity(LearnedPrec* lp, double noise, unsigned long seed);
+/* multiplier of the network output (1/diag of the operator it is trained for) */
+int LearnedPrec_SetOutputScale(LearnedPrec* lp, double out_scale);
+double LearnedPrec_GetOutputScale(const LearnedPrec* lp);
+
+/* out = N(in) for nb images of nx*ny pixels (image-major, x fastest), FP64
+   device arrays; addto = 1 adds N(in) to out */
+int LearnedPrec_Apply(LearnedPrec* lp, int nb, int nx, int ny, const double* in, double* out,
+                      int addto);
+
+/* training objective on a batch of pairs (r_b, e_b) with (I - ratio*L) e = r
+   on nb Dirichlet grids of nx*ny nodes (L the unit-spacing 5-point Laplacian,
+   zero boundary), in units normalised by s_b = RMS(r_b):
+     loss = mean_b  d_b^T A d_b / (e_b^T r_b),  d_b = N(r_b) - e_b
+   (the squared energy-norm error of the correction, relative to that of e);
+   writes the loss and, if grad is not NULL, its gradient with respect to the
+   parameters (host FP32, NumParams values).  r, e: FP64 device arrays. */
+int LearnedPrec_LossGrad(LearnedPrec* lp, int nb, int nx, int ny, double ratio, const double* r,
+                         const double* e, double* loss, float* grad);
+
+/* training pairs on the GPU: nsamp smooth random fields r_i (sum of sine
+   modes with amplitude ~ N(0,1)/|k| up to a random cut-off, seed + i) on n*n
+   Dirichlet grids, e_i from Jacobi-PCG on (I - ratio*L) e = r to relative
+   residual reltol (all pairs solved together, per-pair step lengths); r, e:
+   device arrays of nsamp*n*n doubles.  Writes the largest iteration count. */
+int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol,
+                                unsigned long seed, double* r, double* e, int* max_iters,
+                                SUNContext ctx);
+
+/* Adam (beta 0.9/0.999) on minibatches of `batch` pairs drawn (seeded) from
+   the training set, learning rate lr0 with cosine decay over `steps`;
+   loss_hist (host, may be NULL) receives every step's loss.  All on the GPU
+   with the forward and backward kernels of this file. */
+int LearnedPrec_Train(LearnedPrec* lp, int nsamp, int n, double ratio, const double* r,
+                      const double* e, int steps, int batch, double lr0, unsigned long seed,
+                      double* loss_hist);
+
+/* attach a trained network to a problem (NULL detaches); the problem does
+   not own it */
+int DiffusionProblem_SetLearnedPrec(DiffusionProblem* prob, LearnedPrec* lp);
+/* preconditioner callbacks for ARKodeSetPreconditioner with the same
+   user_data as PSetup/PSolve (the DiffusionProblem, which also carries the
+   RHS, as the reference's UserData does, main.cpp:266, 307-313).  Setup
+   refreshes the Jacobi diagonal (the fallback's preconditioner, so the pair
+   PSetup/PSolve stays valid) and sets the network output scale to
+   1/(1 + gamma*(2kx/dx^2 + 2ky/dy^2)), the inverse diagonal of I - gamma*J
+   for constant coefficients; solve: z = N(r). */
+int LearnedPSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok,
+                  sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
+int LearnedPSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z,
+                  sunrealtype gamma, sunrealtype delta, int lr, void* user_data);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -124,7 +124,14 @@ double orc_vwrmsnorm(const double* x, const double* w, int64_t n);
 double orc_vdot(const double* x, const double* y, int64_t n);
 double orc_vmaxnorm(const double* x, int64_t n);
 
+/* learned preconditioner network (orc_learned.c) */
+int orc_lp_num_params(int nl, const int* dil);
+double orc_lp_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,
+                       int ny, double ratio, const double* r, const double* e, int emu, double* grad,
+                       double* u_out);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

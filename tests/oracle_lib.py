@@ -193,3 +193,31 @@ def dirk_table(order):
     c = np.zeros(4)
     s = lib().orc_dirk_table(order, ptr(A), ptr(b), ptr(bh), ptr(c))
     return A.reshape(4, 4)[:s, :s], b[:s], bh[:s], c[:s]
+
+
+def lp_num_params(dil):
+    d = np.ascontiguousarray(dil, dtype=np.int32)
+    f = lib().orc_lp_num_params
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    return f(len(d), d.ctypes.data_as(C.POINTER(C.c_int)))
+
+
+def lp_lossgrad(dil, params, out_scale, nb, nx, ny, ratio, r, e, emu=1, want_grad=True):
+    """Loss, parameter gradient and normalised network output of the learned
+    preconditioner restated on the CPU (emu=1: BF16 roundings of the GPU path
+    emulated; emu=0: exact FP64)."""
+    d = np.ascontiguousarray(dil, dtype=np.int32)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    g = np.zeros(len(p)) if want_grad else None
+    u = np.zeros(nb * nx * ny)
+    f = lib().orc_lp_lossgrad
+    f.restype = C.c_double
+    f.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_double, C.c_int, C.c_int,
+                  C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    loss = f(len(d), d.ctypes.data_as(C.POINTER(C.c_int)), p.ctypes.data_as(C.POINTER(C.c_float)),
+             out_scale, nb, nx, ny, ratio, ptr(r), ptr(e), emu, ptr(g) if g is not None else None, ptr(u))
+    return loss, g, u

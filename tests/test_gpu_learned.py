@@ -1,0 +1,183 @@
+"""The learned preconditioner on the GPU: the MFMA forward and backward
+kernels against the CPU oracle (BF16 roundings emulated there, FP32 vs FP64
+accumulation: agreement to rounding), the Jacobi-PCG training pairs against
+the system they solve, training on the GPU, and the network inside FGMRES on
+the implicit path (same solution as Jacobi-PCG, fewer Krylov iterations than
+the untrained network, the Jacobi-PCG fallback at the iteration cap)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from hiplib import package
+
+pytestmark = pytest.mark.gpu
+
+DIL = (1, 1, 1, 1)
+
+
+@pytest.fixture(scope="module")
+def pk():
+    return package()
+
+
+@pytest.fixture(scope="module")
+def ctx(pk):
+    c = pk.Context()
+    yield c
+    c.close()
+
+
+def apply_A(x, nb, nx, ny, ratio):
+    x = x.reshape(nb, ny, nx)
+    p = np.pad(x, ((0, 0), (1, 1), (1, 1)))
+    lap = (((p[:, 1:-1, :-2] + p[:, 1:-1, 2:]) + p[:, :-2, 1:-1]) + p[:, 2:, 1:-1]) - 4.0 * x
+    return (x - ratio * lap).reshape(-1)
+
+
+def random_params(n, seed, scale=0.25):
+    rng = np.random.default_rng(seed)
+    return (scale * rng.standard_normal(n)).astype(np.float32)
+
+
+@pytest.mark.parametrize("nb,nx,ny", [(1, 16, 16), (3, 37, 21), (2, 64, 9)])
+def test_forward_matches_oracle(pk, ctx, nb, nx, ny):
+    net = pk.LearnedPrec(ctx, DIL)
+    p = random_params(net.nparams, nb * nx)
+    net.set_params(p)
+    net.set_output_scale(0.3)
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal(nb * nx * ny)
+    R, Z = pk.Vector(ctx, n=r.size), pk.Vector(ctx, n=r.size)
+    R.set(r)
+    assert net.apply(nb, nx, ny, R, Z) == 0
+    z = Z.get()
+    s = np.sqrt((r.reshape(nb, -1) ** 2).mean(axis=1))
+    _, _, u = O.lp_lossgrad(DIL, p, 0.3, nb, nx, ny, 1.0, r, r, emu=1, want_grad=False)
+    ref = u.reshape(nb, -1) * s[:, None]
+    err = z.reshape(nb, -1) - ref
+    assert np.abs(err).max() <= 1e-2 * np.abs(ref).max()
+    assert np.sqrt((err ** 2).mean()) <= 2e-3 * np.sqrt((ref ** 2).mean())
+    # accumulate into an existing vector
+    Z2 = pk.Vector(ctx, n=r.size)
+    Z2.set(np.ones(r.size))
+    assert net.apply(nb, nx, ny, R, Z2, 1) == 0
+    np.testing.assert_allclose(Z2.get(), z + 1.0, rtol=0, atol=1e-12 * max(1.0, np.abs(z).max()))
+    net.release()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_loss_and_gradient_match_oracle(pk, ctx, seed):
+    nb, nx, ny, ratio = 2, 33, 20, 40.0
+    net = pk.LearnedPrec(ctx, DIL)
+    if seed == 0:
+        net.init_identity(0.05, 3)
+        p = net.params()
+    else:
+        p = random_params(net.nparams, 11, 0.2)
+        net.set_params(p)
+    net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
+    rng = np.random.default_rng(seed)
+    e = rng.standard_normal(nb * nx * ny)
+    r = apply_A(e, nb, nx, ny, ratio)
+    R, E = pk.Vector(ctx, n=r.size), pk.Vector(ctx, n=r.size)
+    R.set(r)
+    E.set(e)
+    loss, g = net.loss_grad(nb, nx, ny, ratio, R, E)
+    lref, gref, _ = O.lp_lossgrad(DIL, p, 1.0 / (1.0 + 4.0 * ratio), nb, nx, ny, ratio, r, e, emu=1)
+    assert abs(loss - lref) <= 1e-3 * abs(lref)
+    assert np.linalg.norm(g - gref) <= 1e-2 * np.linalg.norm(gref)
+    net.release()
+
+
+def test_training_pairs_solve_the_system(pk, ctx):
+    nsamp, n, ratio = 24, 32, 60.0
+    R, E, iters, rc = pk.training_set(ctx, nsamp, n, ratio, reltol=1e-10, seed=1234)
+    assert rc == 0 and iters > 5
+    r, e = R.get().reshape(nsamp, -1), E.get().reshape(nsamp, -1)
+    for i in range(nsamp):
+        res = apply_A(e[i], 1, n, n, ratio) - r[i]
+        assert np.linalg.norm(res) <= 1.01e-10 * np.linalg.norm(r[i]), i
+    # smooth fields and rough (operator-applied) ones alternate
+    rough = np.array([np.abs(np.diff(r[i].reshape(n, n), axis=1)).mean() / np.abs(r[i]).mean()
+                      for i in range(nsamp)])
+    assert rough[1::2].mean() > 1.3 * rough[0::2].mean()
+
+
+def trained_net(pk, ctx, ratio, n=32, nsamp=512, steps=300, batch=16, seed=0):
+    R, E, _, rc = pk.training_set(ctx, nsamp, n, ratio, reltol=1e-12, seed=1234)
+    assert rc == 0
+    net = pk.LearnedPrec(ctx, DIL)
+    net.init_identity(1e-2, seed)
+    net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
+    hist = net.train(nsamp, n, ratio, R, E, steps, batch, 2e-3, seed)
+    return net, hist
+
+
+def test_training_lowers_the_loss(pk, ctx):
+    net, hist = trained_net(pk, ctx, 80.0)
+    assert np.all(np.isfinite(hist))
+    assert hist[-30:].mean() < 0.8 * hist[:10].mean(), (hist[:10].mean(), hist[-30:].mean())
+    net.release()
+
+
+def dirk_run(pk, ctx, prec, tout, dt, side, net=None, maxl=400, mrestart=30):
+    G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=side, ny=side, kx=1.0, ky=1.0)
+    if net is not None:
+        net.attach(G)
+    ls = "cg" if prec == "jacobi" else "fgmres"
+    I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=maxl, precond="learned" if net is not None else True)
+    if ls == "fgmres":
+        pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
+    I.ARKodeSetOrder(3)
+    I.ARKodeSetFixedStep(dt)
+    I.ARKodeSetLSRelativeTolerance(1e-8)
+    I.ARKodeSetMaxNumSteps(100000)
+    flag, t = I.evolve(tout)
+    out = dict(flag=flag, t=t, y=I.y.get(), stats=I.stats(), fallbacks=I.fallbacks())
+    I.free()
+    return out
+
+
+def test_learned_fgmres_on_the_implicit_path(pk, ctx):
+    side, dt = 64, 0.02
+    dx = 1.0 / (side + 1)
+    gamma_a = 1767732205903.0 / 4055673282236.0  # ARK3(2)4L[2]SA diagonal
+    ratio = gamma_a * dt / dx ** 2
+    net, _ = trained_net(pk, ctx, ratio, steps=400)
+    ident = pk.LearnedPrec(ctx, DIL)
+    ident.init_identity(0.0, 0)
+    ref = dirk_run(pk, ctx, "jacobi", 0.1, dt, side)
+    lrn = dirk_run(pk, ctx, "learned", 0.1, dt, side, net=net)
+    idn = dirk_run(pk, ctx, "learned", 0.1, dt, side, net=ident)
+    for o in (ref, lrn, idn):
+        assert o["flag"] == 0 and abs(o["t"] - 0.1) < 1e-12
+    scale = np.abs(ref["y"]).max()
+    assert np.abs(lrn["y"] - ref["y"]).max() <= 1e-6 * scale
+    assert np.abs(idn["y"] - ref["y"]).max() <= 1e-6 * scale
+    assert lrn["fallbacks"] == 0
+    # what the trained network gains over the untrained (Jacobi-equivalent) one
+    assert lrn["stats"]["nli"] < 0.7 * idn["stats"]["nli"], (lrn["stats"]["nli"], idn["stats"]["nli"])
+    # and the CPU oracle's Jacobi-PCG run of the same problem
+    P = O.OracleProblem(O.MMS, nx=side, ny=side, kx=1.0, ky=1.0)
+    Rr = O.OracleIntegrator(P, O.DIRK, P.initial())
+    Rr.set_order(3)
+    Rr.set_fixed_step(dt)
+    Rr.set_lin_solver(400, 0.0, 0, 1)
+    Rr.set_ls_reltol(1e-8)
+    fr, tr, yr = Rr.evolve(0.1)
+    assert fr == 0
+    assert np.abs(lrn["y"] - yr).max() <= 1e-6 * scale
+    net.release()
+    ident.release()
+
+
+def test_fallback_to_jacobi_pcg_at_the_cap(pk, ctx):
+    side, dt = 48, 0.02
+    net = pk.LearnedPrec(ctx, DIL)
+    net.init_identity(0.0, 0)
+    ref = dirk_run(pk, ctx, "jacobi", 0.04, dt, side)
+    capped = dirk_run(pk, ctx, "learned", 0.04, dt, side, net=net, maxl=3, mrestart=3)
+    assert capped["flag"] == 0 and capped["fallbacks"] > 0
+    scale = np.abs(ref["y"]).max()
+    assert np.abs(capped["y"] - ref["y"]).max() <= 1e-6 * scale
+    net.release()

@@ -1,0 +1,83 @@
+"""The CPU oracle of the learned preconditioner, pinned before it is used to
+check the GPU kernels: its forward pass against an independent numpy
+convolution, and its reverse-mode gradient against central differences of
+its own exact-FP64 loss (a wrong tap flip, a dropped ReLU mask or a wrong
+loss scaling fails one of them)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+
+def np_forward(dil, params, out_scale, nb, nx, ny, r):
+    """Independent restatement: per layer, a sum over the 9 shifted copies of
+    the zero-padded input (numpy slicing), ReLU between layers."""
+    x = r.reshape(nb, ny, nx)
+    s = np.sqrt((x ** 2).mean(axis=(1, 2), keepdims=True))
+    x = (x / s)[..., None]
+    off = 0
+    for l, d in enumerate(dil):
+        cin = 1 if l == 0 else 16
+        cout = 1 if l == len(dil) - 1 else 16
+        W = params[off:off + cout * cin * 9].reshape(cout, cin, 3, 3).astype(np.float64)
+        off += cout * cin * 9
+        b = params[off:off + cout].astype(np.float64)
+        off += cout
+        xp = np.pad(x, ((0, 0), (d, d), (d, d), (0, 0)))
+        out = np.zeros((nb, ny, nx, cout)) + b
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                sl = xp[:, d + dy * d:d + dy * d + ny, d + dx * d:d + dx * d + nx, :]
+                out += np.einsum("byxc,oc->byxo", sl, W[:, :, dy + 1, dx + 1])
+        x = out if l == len(dil) - 1 else np.maximum(out, 0.0)
+    return out_scale * x[..., 0].reshape(-1)
+
+
+def problem(seed, dil, nb=2, nx=7, ny=6):
+    rng = np.random.default_rng(seed)
+    n = O.lp_num_params(dil)
+    params = (0.3 * rng.standard_normal(n)).astype(np.float32)
+    r = rng.standard_normal(nb * nx * ny)
+    e = rng.standard_normal(nb * nx * ny) + 0.5 * r
+    return params, r, e
+
+
+@pytest.mark.parametrize("dil", [(1, 1), (1, 2, 1), (1, 1, 2, 1)])
+def test_oracle_forward_matches_numpy(dil):
+    params, r, e = problem(1, dil)
+    _, _, u = O.lp_lossgrad(dil, params, 0.7, 2, 7, 6, 3.0, r, e, emu=0, want_grad=False)
+    ref = np_forward(dil, params, 0.7, 2, 7, 6, r)
+    assert np.abs(u - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dil", [(1, 1), (1, 2, 1), (1, 1, 2, 1)])
+def test_oracle_gradient_matches_finite_differences(dil):
+    params, r, e = problem(2, dil)
+    ratio = 5.0
+    loss, g, _ = O.lp_lossgrad(dil, params, 0.4, 2, 7, 6, ratio, r, e, emu=0)
+    p64 = params.astype(np.float64)
+    fd = np.zeros(len(p64))
+    for k in range(len(p64)):
+        h = 1e-4 * max(1.0, abs(p64[k]))
+        # the oracle takes FP32 parameters: step in a way FP32 represents exactly
+        pp, pm = params.copy(), params.copy()
+        pp[k] = np.float32(p64[k] + h)
+        pm[k] = np.float32(p64[k] - h)
+        hh = float(pp[k]) - float(pm[k])
+        lp_, _, _ = O.lp_lossgrad(dil, pp, 0.4, 2, 7, 6, ratio, r, e, emu=0, want_grad=False)
+        lm_, _, _ = O.lp_lossgrad(dil, pm, 0.4, 2, 7, 6, ratio, r, e, emu=0, want_grad=False)
+        fd[k] = (lp_ - lm_) / hh
+    scale = np.abs(g).max()
+    # the loss is piecewise quadratic (ReLU); a step crossing a kink is rare
+    bad = np.abs(fd - g) > 1e-7 * scale
+    assert bad.mean() < 0.005, (np.abs(fd - g).max() / scale, bad.sum())
+
+
+def test_oracle_bf16_emulation_is_a_small_perturbation():
+    dil = (1, 1, 1)
+    params, r, e = problem(3, dil)
+    l0, g0, u0 = O.lp_lossgrad(dil, params, 0.5, 2, 7, 6, 2.0, r, e, emu=0)
+    l1, g1, u1 = O.lp_lossgrad(dil, params, 0.5, 2, 7, 6, 2.0, r, e, emu=1)
+    rel = np.abs(u1 - u0).max() / np.abs(u0).max()
+    assert 0.0 < rel < 0.05
+    assert np.linalg.norm(g1 - g0) / np.linalg.norm(g0) < 0.05
