@@ -246,6 +246,100 @@ def batch_workload(pk, nprob, tout):
     print(json.dumps(out))
 
 
+def implicit_workload(pk, args):
+    """The implicit path: mms-dirichlet 2048^2, kx = ky = 1, ARKStep DIRK3
+    (ARK3(2)4L[2]SA), fixed dt = 1e-3, linear solves to relative residual
+    1e-8: Jacobi-PCG against FGMRES with the learned preconditioner (Jacobi-PCG
+    fallback at the iteration cap).  The network (4 layers of 3x3
+    convolutions, 16 channels, BF16) is trained on the GPU first, on pairs
+    from Jacobi-PCG solves on 64^2 grids with the same gamma*dt/dx^2, outside
+    the timed solves.  A prefix of `--isteps` steps is timed for each method;
+    errors are L2 against the exact solution and between the two runs."""
+    import numpy as np
+
+    ctx = pk.Context()
+    side, dt, n_steps = args.iside, 1e-3, args.isteps
+    dx = 1.0 / (side + 1)
+    gamma_a = 1767732205903.0 / 4055673282236.0
+    ratio = gamma_a * dt / dx ** 2
+    log(f"implicit: {side}^2, dt {dt}, gamma*dt/dx^2 = {ratio:.1f}, {n_steps} steps per method")
+    # training
+    t0 = time.perf_counter()
+    tn = args.train_n
+    R, E, iters, rc = pk.training_set(ctx, args.nsamp, tn, ratio, reltol=1e-12, seed=1234)
+    hip_sync()
+    t_data = time.perf_counter() - t0
+    assert rc == 0
+    net = pk.LearnedPrec(ctx, (1, 1, 1, 1))
+    net.init_identity(8e-4, 0)
+    net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
+    t0 = time.perf_counter()
+    hist = net.train(args.nsamp, tn, ratio, R, E, args.train_steps, args.train_batch, args.train_lr, 0)
+    hip_sync()
+    t_train = time.perf_counter() - t0
+    log(f"training set {args.nsamp} pairs of {tn}^2 (Jacobi-PCG <= {iters} its) {t_data:.1f} s; "
+        f"{args.train_steps} Adam steps {t_train:.1f} s; loss {hist[:20].mean():.3f} -> {hist[-50:].mean():.3f}")
+    R.destroy()
+    E.destroy()
+    ident = pk.LearnedPrec(ctx, (1, 1, 1, 1))
+    ident.init_identity(0.0, 0)
+
+    def run(kind, mrestart=args.mrestart):
+        G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=side, ny=side, kx=1.0, ky=1.0)
+        if kind != "jacobi":
+            (net if kind == "learned" else ident).attach(G)
+        ls = "cg" if kind == "jacobi" else "fgmres"
+        I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=20000 if ls == "cg" else args.imaxl,
+                          precond=True if kind == "jacobi" else "learned")
+        if ls == "fgmres":
+            pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
+        I.ARKodeSetOrder(3)
+        I.ARKodeSetFixedStep(dt)
+        I.ARKodeSetLSRelativeTolerance(1e-8)
+        I.ARKodeSetMaxNumSteps(1000000)
+        flag, t = I.evolve(dt)  # first step untimed (setup, allocation)
+        hip_sync()
+        s0 = I.stats()
+        t0 = time.perf_counter()
+        for k in range(n_steps):
+            flag, t = I.evolve(dt * (k + 2))
+            assert flag == 0, flag
+            st = I.stats()
+            log(f"  {kind} step {k + 1}: {time.perf_counter() - t0:.2f} s, Krylov its {st['nli'] - s0['nli']}, "
+                f"fallbacks {I.fallbacks()}")
+        hip_sync()
+        el = time.perf_counter() - t0
+        s1 = I.stats()
+        y = I.y.get()
+        ex = G.exact(t).get()
+        out = dict(s_per_step=el / n_steps, lin_iters=(s1["nli"] - s0["nli"]) / n_steps,
+                   newton_iters=(s1["nni"] - s0["nni"]) / n_steps, fallbacks=I.fallbacks(),
+                   l2_err_vs_exact=float(np.sqrt(np.mean((y - ex) ** 2))), t=t)
+        I.free()
+        G.destroy()
+        log(f"{kind}: {out}")
+        return out, y
+
+    jac, yj = run("jacobi")
+    lrn, yl = run("learned")
+    idn, yi = run("identity")
+    rel = float(np.sqrt(np.mean((yl - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
+    out = {"metric": "time-to-solution at fixed L2 err, learned-preconditioned FGMRES vs Jacobi-PCG",
+           "config": f"mms-dirichlet {side}x{side} kx=ky=1 FP64, DIRK3 fixed dt=1e-3, LS reltol 1e-8",
+           "steps_timed": n_steps, "gamma_dt_over_dx2": ratio,
+           "jacobi_pcg": jac, "learned_fgmres": lrn, "untrained_fgmres": idn,
+           "speedup_learned_vs_jacobi_pcg": jac["s_per_step"] / lrn["s_per_step"],
+           "krylov_iters_ratio_learned_vs_untrained": lrn["lin_iters"] / idn["lin_iters"],
+           "rel_l2_diff_learned_vs_jacobi_pcg": rel,
+           "training": {"pairs": args.nsamp, "grid": tn, "data_s": t_data, "train_s": t_train,
+                        "steps": args.train_steps, "batch": args.train_batch, "lr": args.train_lr,
+                        "loss_first20": float(hist[:20].mean()), "loss_last50": float(hist[-50:].mean())},
+           "fgmres_restart": args.mrestart}
+    net.release()
+    ident.release()
+    print(json.dumps(out))
+
+
 def cpu_baseline():
     """One adaptive RKL2 step of the same workload on the CPU oracle (serial)."""
     import oracle_lib as O
@@ -275,8 +369,18 @@ def main():
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
                     help="only launch the hot-path kernels REPS times each (profiling workload)")
     ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
-    ap.add_argument("--workload", default="sts", choices=["sts", "batch"],
-                    help="sts: the headline 4096^2 RKL2 line; batch: the 256-problem batched comparison")
+    ap.add_argument("--workload", default="sts", choices=["sts", "batch", "implicit"],
+                    help="sts: the headline 4096^2 RKL2 line; batch: the 256-problem batched "
+                         "comparison; implicit: learned-preconditioned FGMRES vs Jacobi-PCG")
+    ap.add_argument("--iside", type=int, default=2048)
+    ap.add_argument("--isteps", type=int, default=3)
+    ap.add_argument("--nsamp", type=int, default=20000)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-n", type=int, default=64, help="side of the training grids")
+    ap.add_argument("--train-batch", type=int, default=16)
+    ap.add_argument("--train-lr", type=float, default=3e-4)
+    ap.add_argument("--mrestart", type=int, default=30)
+    ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--tout", type=float, default=0.5)
     args = ap.parse_args()
@@ -292,6 +396,9 @@ def main():
         return
     if args.workload == "batch":
         batch_workload(pk, args.nprob, args.tout)
+        return
+    if args.workload == "implicit":
+        implicit_workload(pk, args)
         return
     ctx = pk.Context()
     G, I = make_integrator(pk, ctx, args.fastpath)

@@ -103,18 +103,21 @@ def test_training_pairs_solve_the_system(pk, ctx):
     assert rough[1::2].mean() > 1.3 * rough[0::2].mean()
 
 
-def trained_net(pk, ctx, ratio, n=32, nsamp=512, steps=300, batch=16, seed=0):
+def trained_net(pk, ctx, ratio, n=32, nsamp=2048, steps=1500, batch=16, seed=0, lr=3e-4):
+    """The training recipe of the benchmark: identity start with small noise,
+    Adam at a small learning rate (larger rates fit the training pairs better
+    but give networks that stall FGMRES)."""
     R, E, _, rc = pk.training_set(ctx, nsamp, n, ratio, reltol=1e-12, seed=1234)
     assert rc == 0
     net = pk.LearnedPrec(ctx, DIL)
-    net.init_identity(1e-2, seed)
+    net.init_identity(8e-4, seed)
     net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
-    hist = net.train(nsamp, n, ratio, R, E, steps, batch, 2e-3, seed)
+    hist = net.train(nsamp, n, ratio, R, E, steps, batch, lr, seed)
     return net, hist
 
 
 def test_training_lowers_the_loss(pk, ctx):
-    net, hist = trained_net(pk, ctx, 80.0)
+    net, hist = trained_net(pk, ctx, 80.0, steps=600, lr=2e-3)
     assert np.all(np.isfinite(hist))
     assert hist[-30:].mean() < 0.8 * hist[:10].mean(), (hist[:10].mean(), hist[-30:].mean())
     net.release()
@@ -143,7 +146,7 @@ def test_learned_fgmres_on_the_implicit_path(pk, ctx):
     dx = 1.0 / (side + 1)
     gamma_a = 1767732205903.0 / 4055673282236.0  # ARK3(2)4L[2]SA diagonal
     ratio = gamma_a * dt / dx ** 2
-    net, _ = trained_net(pk, ctx, ratio, steps=400)
+    net, _ = trained_net(pk, ctx, ratio)
     ident = pk.LearnedPrec(ctx, DIL)
     ident.init_identity(0.0, 0)
     ref = dirk_run(pk, ctx, "jacobi", 0.1, dt, side)
