@@ -322,7 +322,10 @@ def implicit_workload(pk, args):
 
     jac, yj = run("jacobi")
     lrn, yl = run("learned")
-    idn, yi = run("identity")
+    if args.no_untrained:
+        idn = {"lin_iters": float("nan"), "s_per_step": float("nan")}
+    else:
+        idn, yi = run("identity")
     rel = float(np.sqrt(np.mean((yl - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
     out = {"metric": "time-to-solution at fixed L2 err, learned-preconditioned FGMRES vs Jacobi-PCG",
            "config": f"mms-dirichlet {side}x{side} kx=ky=1 FP64, DIRK3 fixed dt=1e-3, LS reltol 1e-8",
@@ -379,6 +382,7 @@ def main():
     ap.add_argument("--train-n", type=int, default=64, help="side of the training grids")
     ap.add_argument("--train-batch", type=int, default=16)
     ap.add_argument("--train-lr", type=float, default=3e-4)
+    ap.add_argument("--no-untrained", action="store_true", help="skip the untrained-network FGMRES run")
     ap.add_argument("--mrestart", type=int, default=30)
     ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")
     ap.add_argument("--nprob", type=int, default=256)

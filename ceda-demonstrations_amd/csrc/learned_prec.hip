@@ -232,26 +232,34 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const WGradArgs a)
   if (g == 0) P[9 * 256 + n] = bsum;
 }
 
-// sum of the per-wavefront partials in wavefront order, into the parameter
-// layout W[cout][cin][3][3], b[cout]
+// sum of the per-wavefront partials into the parameter layout W[cout][cin][3][3],
+// b[cout]: 64 consecutive partial offsets per block (coalesced reads), each
+// summed over the wavefronts in 4 interleaved groups, the groups combined in
+// a fixed order (deterministic), then scattered to the parameter it belongs to
 __global__ __launch_bounds__(kBlock) void k_wreduce(const float* __restrict__ partial, int nwaves,
                                                     int cin, int cout, float* __restrict__ gw,
                                                     float* __restrict__ gb)
 {
-  const int nw = cout * cin * 9;
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= nw + cout) return;
-  int off;
-  if (k < nw) {
-    const int co = k / (cin * 9), ci = (k / 9) % cin, t = k % 9;
-    // partial tile element (m, n) of tap tile T
-    if (cin == kC) off = t * 256 + co * 16 + ci;  // m = co, n = ci
-    else off = co * 16 + t;                         // single tile, n = tap
-  } else off = 9 * 256 + (k - nw);
+  const int q = threadIdx.x / 64, off = blockIdx.x * 64 + (threadIdx.x & 63);
   float s = 0.0f;
-  for (int w = 0; w < nwaves; w++) s += partial[(int64_t)w * kPW + off];
-  if (k < nw) gw[k] = s;
-  else gb[k - nw] = s;
+  if (off < kPW)
+    for (int w = q; w < nwaves; w += 4) s += partial[(int64_t)w * kPW + off];
+  __shared__ float sh[4][64];
+  sh[q][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (q != 0 || off >= kPW) return;
+  const float v = ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
+  if (off >= 9 * 256) {
+    const int b = off - 9 * 256;
+    if (b < cout) gb[b] = v;
+    return;
+  }
+  const int t = off / 256, m = (off % 256) / 16, n = off % 16;
+  if (cin == kC) {
+    if (m < cout) gw[(m * cin + n) * 9 + t] = v;  // tile t: m = co, n = ci
+  } else if (t == 0 && n < 9 && m < cout) {
+    gw[m * 9 + n] = v;  // single tile: m = co, n = tap
+  }
 }
 
 // ------------------------------------------------- packing of the A operands
@@ -672,10 +680,12 @@ template <int CIN, int COUT>
 void wgrad(LearnedPrec* L, int l, int nb, int nx, int ny, const void* x, const void* dy)
 {
   WGradArgs w{nb, nx, ny, L->dil[l], x, L->isc, dy, L->wpart};
-  hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(kWGradWaves / kWavesPB), dim3(kBlock), 0, L->ex->stream, w);
-  const int nparam = L->cout[l] * L->cin[l] * 9 + L->cout[l];
-  hipLaunchKernelGGL(k_wreduce, dim3((nparam + kBlock - 1) / kBlock), dim3(kBlock), 0, L->ex->stream,
-                     L->wpart, kWGradWaves, L->cin[l], L->cout[l], L->d_g + L->woff[l], L->d_g + L->boff[l]);
+  // about 8 pixel blocks per wavefront, at most kWGradWaves partials
+  const int64_t nblk = (int64_t)nb * ny * ((nx + 15) / 16);
+  const int nw = (int)std::min<int64_t>(kWGradWaves, std::max<int64_t>(kWavesPB, ((nblk + 7) / 8 + 3) / 4 * 4));
+  hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(nw / kWavesPB), dim3(kBlock), 0, L->ex->stream, w);
+  hipLaunchKernelGGL(k_wreduce, dim3((kPW + 63) / 64), dim3(kBlock), 0, L->ex->stream, L->wpart, nw,
+                     L->cin[l], L->cout[l], L->d_g + L->woff[l], L->d_g + L->boff[l]);
 }
 
 // loss (device scalar *dloss) and parameter gradient (L->d_g) of a batch

@@ -516,6 +516,77 @@ double raw_sts_error_wsqr(Exec* ex, int64_t n, const double* yn, const double* y
 
 }  // namespace ceda
 
+// DotProdMulti with more than 8 vectors: groups of 8 vectors in one launch
+// (blockIdx.y = group), 8 accumulators per thread instead of up to 64, so the
+// kernel keeps enough wavefronts in flight to stream at HBM rate.  Every dot
+// product is accumulated and combined exactly as by k_reduce with
+// RDotMulti (same grid-stride partition, same wavefront and workgroup trees,
+// same partial layout), so the results are bitwise those of the one-group
+// kernel.
+struct DotGroups {
+  const double* x;
+  const double* y[NVHIP_MAX_FUSED];
+  int nv;
+};
+__global__ __launch_bounds__(kBlock) void k_dotmulti_groups(int64_t n, DotGroups r,
+                                                            double* partials)
+{
+  const int base = blockIdx.y * 8, ntot = 8 * gridDim.y;
+  double acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) acc[j] = 0.0;
+  const int64_t np = n >> 1;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < np; k += stride) {
+    const double2 xv = ld2(r.x, 2 * k);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      if (base + j >= r.nv) break;
+      const double2 yv = ld2(r.y[base + j], 2 * k);
+      acc[j] += xv.x * yv.x;
+      acc[j] += xv.y * yv.y;
+    }
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int j = 0; j < 8 && base + j < r.nv; j++) acc[j] += r.x[n - 1] * r.y[base + j][n - 1];
+  }
+  __shared__ double sh[kBlock / kWave][8];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const double v = wave_reduce<RSUM>(acc[j]);
+    if (lane == 0) sh[wave][j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double v = sh[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kBlock / kWave; w++) v = rcombine<RSUM>(v, sh[w][threadIdx.x]);
+    partials[(int64_t)blockIdx.x * ntot + base + threadIdx.x] = v;
+  }
+}
+
+static void dotmulti_groups(int nvec, N_Vector x, N_Vector* Y, sunrealtype* d)
+{
+  DotGroups r;
+  r.x = dptr(x);
+  r.nv = nvec;
+  for (int k = 0; k < NVHIP_MAX_FUSED; k++) r.y[k] = (k < nvec) ? dptr(Y[k]) : nullptr;
+  Exec* ex = content(x)->exec;
+  const int64_t n = vlen(x);
+  const unsigned g = reduce_grid(n);
+  const int ngroups = (nvec + 7) / 8;
+  hipLaunchKernelGGL(k_dotmulti_groups, dim3(g, ngroups), dim3(kBlock), 0, ex->stream, n, r, ex->d_partials);
+  CEDA_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final<RSUM>, dim3(1), dim3(kBlock), 0, ex->stream, (int)g, 8 * ngroups,
+                     ex->d_partials, ex->d_result);
+  CEDA_HIP_CHECK(hipGetLastError());
+  CEDA_HIP_CHECK(hipMemcpyAsync(ex->h_result, ex->d_result, sizeof(double) * nvec, hipMemcpyDeviceToHost,
+                                ex->stream));
+  CEDA_HIP_CHECK(hipStreamSynchronize(ex->stream));
+  for (int k = 0; k < nvec; k++) d[k] = ex->h_result[k];
+}
+
 template <int NACC>
 static void dotmulti(int nvec, N_Vector x, N_Vector* Y, sunrealtype* d)
 {
@@ -794,9 +865,7 @@ SUNErrCode N_VDotProdMulti_HipNative(int nvec, N_Vector x, N_Vector* Y, sunrealt
   else if (nvec == 2) dotmulti<2>(nvec, x, Y, dotprods);
   else if (nvec <= 4) dotmulti<4>(nvec, x, Y, dotprods);
   else if (nvec <= 8) dotmulti<8>(nvec, x, Y, dotprods);
-  else if (nvec <= 16) dotmulti<16>(nvec, x, Y, dotprods);
-  else if (nvec <= 32) dotmulti<32>(nvec, x, Y, dotprods);
-  else dotmulti<64>(nvec, x, Y, dotprods);
+  else dotmulti_groups(nvec, x, Y, dotprods);
   return 0;
 }
 
