@@ -285,14 +285,18 @@ def implicit_workload(pk, args):
     ident.init_identity(0.0, 0)
 
     def run(kind, mrestart=args.mrestart):
+        """kind: 'jacobi' (Jacobi-PCG), 'learned' (FGMRES + network), 'learned-fcg'
+        (flexible PCG + network), 'identity' (FGMRES + untrained network)."""
         G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=side, ny=side, kx=1.0, ky=1.0)
         if kind != "jacobi":
-            (net if kind == "learned" else ident).attach(G)
-        ls = "cg" if kind == "jacobi" else "fgmres"
+            (ident if kind == "identity" else net).attach(G)
+        ls = {"jacobi": "cg", "learned-fcg": "fcg"}.get(kind, "fgmres")
         I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=20000 if ls == "cg" else args.imaxl,
                           precond=True if kind == "jacobi" else "learned")
         if ls == "fgmres":
             pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
+        if args.jtimes:
+            I.analytic_jtimes()
         I.ARKodeSetOrder(3)
         I.ARKodeSetFixedStep(dt)
         I.ARKodeSetLSRelativeTolerance(1e-8)
@@ -322,16 +326,21 @@ def implicit_workload(pk, args):
 
     jac, yj = run("jacobi")
     lrn, yl = run("learned")
+    fcg, yf = run("learned-fcg")
     if args.no_untrained:
         idn = {"lin_iters": float("nan"), "s_per_step": float("nan")}
     else:
         idn, yi = run("identity")
     rel = float(np.sqrt(np.mean((yl - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
+    relf = float(np.sqrt(np.mean((yf - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
     out = {"metric": "time-to-solution at fixed L2 err, learned-preconditioned FGMRES vs Jacobi-PCG",
            "config": f"mms-dirichlet {side}x{side} kx=ky=1 FP64, DIRK3 fixed dt=1e-3, LS reltol 1e-8",
            "steps_timed": n_steps, "gamma_dt_over_dx2": ratio,
-           "jacobi_pcg": jac, "learned_fgmres": lrn, "untrained_fgmres": idn,
+           "jtimes": "analytic (stencil)" if args.jtimes else "difference quotient (as the reference)",
+           "jacobi_pcg": jac, "learned_fgmres": lrn, "learned_flexible_pcg": fcg, "untrained_fgmres": idn,
            "speedup_learned_vs_jacobi_pcg": jac["s_per_step"] / lrn["s_per_step"],
+           "speedup_learned_fcg_vs_jacobi_pcg": jac["s_per_step"] / fcg["s_per_step"],
+           "rel_l2_diff_learned_fcg_vs_jacobi_pcg": relf,
            "krylov_iters_ratio_learned_vs_untrained": lrn["lin_iters"] / idn["lin_iters"],
            "rel_l2_diff_learned_vs_jacobi_pcg": rel,
            "training": {"pairs": args.nsamp, "grid": tn, "data_s": t_data, "train_s": t_train,
@@ -383,6 +392,7 @@ def main():
     ap.add_argument("--train-batch", type=int, default=16)
     ap.add_argument("--train-lr", type=float, default=3e-4)
     ap.add_argument("--no-untrained", action="store_true", help="skip the untrained-network FGMRES run")
+    ap.add_argument("--jtimes", action="store_true", help="analytic J v (stencil) instead of the difference quotient")
     ap.add_argument("--mrestart", type=int, default=30)
     ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")
     ap.add_argument("--nprob", type=int, default=256)

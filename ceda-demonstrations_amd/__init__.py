@@ -125,6 +125,7 @@ _SIGS = {
     "ARKodeSetLinear": (i32, [vp, i32]),
     "ARKodeSetNonlinear": (i32, [vp]),
     "ARKodeSetLSFallbackPCG": (i32, [vp, i32, vp, vp]),
+    "ARKodeSetJacTimes": (i32, [vp, vp, vp]),
     "ARKodeGetNumLSFallbacks": (i32, [vp, C.POINTER(C.c_long)]),
     "LearnedPrec_Create": (vp, [i32, C.POINTER(i32), vp]),
     "LearnedPrec_Destroy": (None, [vp]),
@@ -141,6 +142,7 @@ _SIGS = {
     "DiffusionProblem_SetLearnedPrec": (i32, [vp, vp]),
     "ARKodeSetLSRelativeTolerance": (i32, [vp, d]),
     "SUNLinSol_PCG": (vp, [vp, i32, i32, vp]),
+    "SUNLinSol_FPCG": (vp, [vp, i32, i32, vp]),
     "SUNLinSol_SPGMR": (vp, [vp, i32, i32, vp]),
     "SUNLinSol_SPFGMR": (vp, [vp, i32, i32, vp]),
     "SUNLinSol_SPFGMRSetMaxRestartDim": (i32, [vp, i32]),
@@ -468,7 +470,8 @@ class Integrator:
         else:
             self.h = L.ARKStepCreate(None, rhs, t0, y0.h, prob.ctx.h)
             pre = SUN_PREC_RIGHT if precond else SUN_PREC_NONE
-            make = {"cg": L.SUNLinSol_PCG, "gmres": L.SUNLinSol_SPGMR, "fgmres": L.SUNLinSol_SPFGMR}[ls]
+            make = {"cg": L.SUNLinSol_PCG, "gmres": L.SUNLinSol_SPGMR, "fgmres": L.SUNLinSol_SPFGMR,
+                    "fcg": L.SUNLinSol_FPCG}[ls]
             self.LS = make(y0.h, pre, maxl, prob.ctx.h)
             L.ARKodeSetLinearSolver(self.h, self.LS, None)
             if precond == "learned":
@@ -505,6 +508,10 @@ class Integrator:
         out = (C.c_long * 18)()
         lib().ARKodeGetStatsArray(self.h, out, 18)
         return dict(zip(STATS, list(out)))
+
+    def analytic_jtimes(self):
+        """J v from the stencil (JTimes) instead of the difference quotient."""
+        return lib().ARKodeSetJacTimes(self.h, None, fn_ptr("JTimes"))
 
     def fallbacks(self):
         n = C.c_long()

@@ -23,7 +23,7 @@ struct SUNDomEigEstimator_ {
   N_Vector V = nullptr, q = nullptr, work = nullptr;
   int started = 0;
 };
-enum { LS_PCG = 0, LS_SPGMR = 1, LS_SPFGMR = 2 };
+enum { LS_PCG = 0, LS_SPGMR = 1, LS_SPFGMR = 2, LS_FPCG = 3 };
 struct SUNLinearSolver_ {
   int type = LS_PCG;
   int pretype = SUN_PREC_NONE;
@@ -99,6 +99,8 @@ struct ArkMem {
   SUNLinearSolver LS = nullptr;
   ARKLsPrecSetupFn psetup = nullptr;
   ARKLsPrecSolveFn psolve = nullptr;
+  ARKLsJacTimesSetupFn jtsetup = nullptr;
+  ARKLsJacTimesVecFn jtimes = nullptr;
   // statistics (same layout as the CPU restatement)
   long nst = 0, attempts = 0, nfe = 0, netf = 0, nstlim = 0, ndomeig = 0, ndee_iters = 0,
        stage_max = 0, last_stages = 0, nfe_dee = 0, nli = 0, npsetup = 0, npsolve = 0,

@@ -649,10 +649,18 @@ int lsrk_takestep(ArkMem* A, double* dsm)
 }
 
 // ------------------------------------------------------------ DIRK
+// (I - gamma J) v with J v from the user's Jtimes, or the difference quotient
+// of arkLs (the reference's setting: no Jtimes, main.cpp:302-329)
 void atimes(ArkMem* A, double t, double gamma, N_Vector v, N_Vector Av, int* err)
 {
-  const int r = dq_jv(A, t, A->zcur, A->fz, v, A->jv, A->jwork, &A->nfe_ls);
-  if (r) *err = r;
+  if (A->jtimes) {
+    const int r = A->jtimes(v, A->jv, t, A->zcur, A->fz, A->ud, A->jwork);
+    if (r < 0) *err = ARK_RHSFUNC_FAIL;
+    else if (r > 0) *err = kConvRecover;
+  } else {
+    const int r = dq_jv(A, t, A->zcur, A->fz, v, A->jv, A->jwork, &A->nfe_ls);
+    if (r) *err = r;
+  }
   A->njtimes++;
   N_VLinearSum(1.0, v, -gamma, A->jv, Av);
 }
@@ -674,9 +682,14 @@ double lnorm(ArkMem* A, N_Vector r)
 }
 
 // (I - gamma J) x = b by PCG on solver S with preconditioner solve ps/ud
-// (ps = nullptr: none); 0 converged, 1 reduced, 2 failed, <0 fatal
+// (ps = nullptr: none); 0 converged, 1 reduced, 2 failed, <0 fatal.  The
+// flexible variant takes the Polak-Ribiere step
+//   beta = z_{k+1}.(r_{k+1} - r_k) / z_k.r_k = -alpha z_{k+1}.(A p_k) / z_k.r_k,
+// which keeps CG converging when the preconditioner changes from one
+// application to the next (a network, an inner iteration): Golub & Ye, SIAM
+// J. Sci. Comput. 21 (1999); Notay, SIAM J. Sci. Comput. 22 (2000).
 int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double t, double gamma,
-             N_Vector b, N_Vector x)
+             N_Vector b, N_Vector x, bool flexible = false)
 {
   auto prec = [&](double delta, N_Vector r, N_Vector z) -> int {
     if (ps) {
@@ -709,8 +722,17 @@ int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double
     rnorm = lnorm(A, S->r);
     if (rnorm <= delta) return 0;
     if (prec(delta, S->r, S->z)) return -1;
-    const double rznew = N_VDotProd(S->r, S->z);
-    const double beta = rznew / rz;
+    double beta, rznew;
+    if (flexible) {
+      double d2[2];
+      N_Vector Y2[2] = {S->r, S->Ap};
+      N_VDotProdMulti(2, S->z, Y2, d2);
+      rznew = d2[0];
+      beta = -alpha * d2[1] / rz;
+    } else {
+      rznew = N_VDotProd(S->r, S->z);
+      beta = rznew / rz;
+    }
     rz = rznew;
     N_VLinearSum(1.0, S->z, beta, S->p, S->p);
   }
@@ -720,7 +742,7 @@ int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double
 int pcg(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
 {
   const bool pre = A->psolve && A->LS->pretype != SUN_PREC_NONE;
-  return pcg_with(A, A->LS, pre ? A->psolve : nullptr, A->ud, t, gamma, b, x);
+  return pcg_with(A, A->LS, pre ? A->psolve : nullptr, A->ud, t, gamma, b, x, A->LS->type == LS_FPCG);
 }
 
 // restarted flexible GMRES, right preconditioning, classical Gram-Schmidt
@@ -820,7 +842,7 @@ int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
 int lsolve(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
 {
   if (A->LS->type == LS_PCG) return pcg(A, t, gamma, b, x);
-  const int r = fgmres(A, t, gamma, b, x);
+  const int r = (A->LS->type == LS_FPCG) ? pcg(A, t, gamma, b, x) : fgmres(A, t, gamma, b, x);
   if (r > 0 && A->fbLS) {
     A->nls_fallback++;
     return pcg_with(A, A->fbLS, A->fb_psolve, A->fb_ud, t, gamma, b, x);
@@ -847,6 +869,7 @@ int newton(ArkMem* A, double t, double gamma, int force_setup)
   for (int m = 0; m < A->maxcor; m++) {
     int r = call_rhs(A, t, A->zcur, A->fz);
     if (r) return r;
+    if (m == 0 && A->jtimes && A->jtsetup && A->jtsetup(t, A->zcur, A->fz, A->ud)) return ARK_LSETUP_FAIL;
     double c3[3] = {1.0, -1.0, gamma};
     N_Vector X3[3] = {A->sdata, A->zcur, A->fz};
     N_VLinearCombination(3, c3, X3, A->rhsv);
@@ -1403,6 +1426,14 @@ int ARKodeSetLinearSolver(void* m, SUNLinearSolver LS, SUNMatrix)
   return 0;
 }
 
+int ARKodeSetJacTimes(void* m, ARKLsJacTimesSetupFn jtsetup, ARKLsJacTimesVecFn jtimes)
+{
+  if (!m) return ARK_MEM_NULL;
+  AM(m)->jtsetup = jtsetup;
+  AM(m)->jtimes = jtimes;
+  return 0;
+}
+
 int ARKodeSetPreconditioner(void* m, ARKLsPrecSetupFn ps, ARKLsPrecSolveFn pv)
 {
   if (!m) return ARK_MEM_NULL;
@@ -1478,7 +1509,7 @@ static SUNLinearSolver make_ls(int type, N_Vector y, int pretype, int maxl)
   S->pretype = pretype;
   S->maxl = (maxl > 0) ? maxl : 5;
   S->r = N_VClone(y);
-  if (type == LS_PCG) {
+  if (type == LS_PCG || type == LS_FPCG) {
     S->p = N_VClone(y);
     S->z = N_VClone(y);
     S->Ap = N_VClone(y);
@@ -1499,6 +1530,11 @@ static void gmres_alloc(SUNLinearSolver S, N_Vector y)
 SUNLinearSolver SUNLinSol_PCG(N_Vector y, int pretype, int maxl, SUNContext)
 {
   return make_ls(LS_PCG, y, pretype, maxl);
+}
+
+SUNLinearSolver SUNLinSol_FPCG(N_Vector y, int pretype, int maxl, SUNContext)
+{
+  return make_ls(LS_FPCG, y, pretype, maxl);
 }
 
 SUNLinearSolver SUNLinSol_SPFGMR(N_Vector y, int pretype, int maxl, SUNContext)

@@ -874,6 +874,11 @@ int laplacian(sunrealtype, N_Vector u, N_Vector f, void* user_data)
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int JTimes(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector, N_Vector, void* user_data, N_Vector)
+{
+  return laplacian(t, v, Jv, user_data);
+}
+
 // diffusion_2D.cpp:23-35 (+ README.md:28 forcing for the manufactured problem)
 int diffusion(sunrealtype t, N_Vector u, N_Vector f, void* user_data)
 {

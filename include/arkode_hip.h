@@ -137,6 +137,9 @@ int LSRKStepGetSpectralRadius(void* arkode_mem, sunrealtype* rho);
 /* ---- implicit solve ---- */
 int ARKodeSetLinearSolver(void* arkode_mem, SUNLinearSolver LS, SUNMatrix A);
 int ARKodeSetPreconditioner(void* arkode_mem, ARKLsPrecSetupFn psetup, ARKLsPrecSolveFn psolve);
+/* SUNDIALS ARKodeSetJacTimes: user J*v instead of the difference quotient
+   (the reference sets none, main.cpp:302-329); jtsetup may be NULL */
+int ARKodeSetJacTimes(void* arkode_mem, ARKLsJacTimesSetupFn jtsetup, ARKLsJacTimesVecFn jtimes);
 int ARKodeSetLSetupFrequency(void* arkode_mem, int msbp);
 int ARKodeSetEpsLin(void* arkode_mem, sunrealtype eplifac);
 int ARKodeSetLinear(void* arkode_mem, int timedepend);
@@ -155,6 +158,12 @@ int ARKodeGetNumLSFallbacks(void* arkode_mem, long* nfallbacks);
 int ARKodeGetNumLinIters(void* arkode_mem, long* nliters);
 
 SUNLinearSolver SUNLinSol_PCG(N_Vector y, int pretype, int maxl, SUNContext sunctx);
+/* [ext] flexible PCG (Polak-Ribiere beta) for preconditioners that vary from
+   one application to the next, such as the learned network; no reference
+   counterpart.  With a fixed linear preconditioner it takes the PCG steps up
+   to rounding.  The (F)GMRES fallback of ARKodeSetLSFallbackPCG applies to it
+   too. */
+SUNLinearSolver SUNLinSol_FPCG(N_Vector y, int pretype, int maxl, SUNContext sunctx);
 SUNLinearSolver SUNLinSol_SPGMR(N_Vector y, int pretype, int maxl, SUNContext sunctx);
 SUNLinearSolver SUNLinSol_SPFGMR(N_Vector y, int pretype, int maxl, SUNContext sunctx);
 /* [ext] Krylov restart length of SPGMR/SPFGMR (default 30) */

@@ -187,6 +187,9 @@ typedef int (*ARKLsPrecSetupFn)(sunrealtype t, N_Vector y, N_Vector fy,
 typedef int (*ARKLsPrecSolveFn)(sunrealtype t, N_Vector y, N_Vector fy,
                                 N_Vector r, N_Vector z, sunrealtype gamma,
                                 sunrealtype delta, int lr, void* user_data);
+typedef int (*ARKLsJacTimesSetupFn)(sunrealtype t, N_Vector y, N_Vector fy, void* user_data);
+typedef int (*ARKLsJacTimesVecFn)(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y,
+                                  N_Vector fy, void* user_data, N_Vector tmp);
 
 #ifdef __cplusplus
 }

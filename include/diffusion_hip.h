@@ -92,6 +92,12 @@ int DiffusionProblem_StagePass(DiffusionProblem* prob, int K, const double* coef
 int Initial(sunrealtype t, N_Vector u, void* user_data);
 int Exact(sunrealtype t, N_Vector u, void* user_data);
 int laplacian(sunrealtype t, N_Vector u, N_Vector f, void* user_data);
+/* [ext] ARKLsJacTimesVecFn: J v of the RHS, exactly the 5-point operator of
+   laplacian() (diffusion.cpp:9-209; the manufactured forcing does not depend
+   on u), for ARKodeSetJacTimes; the reference uses arkLs's difference
+   quotient instead (main.cpp:302-329) */
+int JTimes(N_Vector v, N_Vector Jv, sunrealtype t, N_Vector y, N_Vector fy, void* user_data,
+           N_Vector tmp);
 int diffusion(sunrealtype t, N_Vector u, N_Vector f, void* user_data);
 int PSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok,
            sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);

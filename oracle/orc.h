@@ -99,6 +99,7 @@ void orc_set_order(orc_integrator* I, int order);
 void orc_set_linear(orc_integrator* I, int linear);
 void orc_set_lin_solver(orc_integrator* I, int maxl, double epslin, int msbp, int precond);
 void orc_set_ls_reltol(orc_integrator* I, double reltol);
+void orc_set_jtimes_analytic(orc_integrator* I, int on);
 int orc_evolve(orc_integrator* I, double tout, double* yout, double* tret);
 int orc_get_stats(const orc_integrator* I, long* out, int nout);
 double orc_get_current_h(const orc_integrator* I);

@@ -54,6 +54,7 @@ struct orc_integrator {
   double *dee_V, *dee_q, *dee_work;
   /* DIRK */
   int order, stages, linear, maxl, msbp, precond, psetup_done, maxcor;
+  int jtimes_analytic;
   double A[16], b[4], bh[4], c[4];
   double* F[4];
   double *sdata, *zcur, *fz, *rhs, *dz, *r, *pv, *zv, *Ap, *jwork, *jv;
@@ -294,6 +295,7 @@ void orc_set_lin_solver(orc_integrator* I, int maxl, double epslin, int msbp, in
   I->precond = precond;
 }
 void orc_set_ls_reltol(orc_integrator* I, double reltol) { I->ls_reltol = reltol; }
+void orc_set_jtimes_analytic(orc_integrator* I, int on) { I->jtimes_analytic = on ? 1 : 0; }
 
 /* ---------------- step-size controllers ---------------- */
 static double ctrl_estimate(const orc_integrator* I, double h, double dsm)
@@ -602,9 +604,13 @@ static int lsrk_takestep(orc_integrator* I, double* dsm)
 }
 
 /* ---------------- DIRK: Newton + PCG + Jacobi ---------------- */
+/* (I - gamma J) v; J v by the difference quotient of arkLs (the reference sets
+   no Jtimes, main.cpp:302-329), or exactly as the 5-point stencil (the forcing
+   does not depend on u) when the analytic option is on */
 static void atimes(orc_integrator* I, double t, double gamma, const double* v, double* Av)
 {
-  dq_jv(I, t, I->zcur, I->fz, v, I->jv, I->jwork, &I->nfe_ls);
+  if (I->jtimes_analytic) orc_stencil(I->P, v, I->jv);
+  else dq_jv(I, t, I->zcur, I->fz, v, I->jv, I->jwork, &I->nfe_ls);
   I->njtimes++;
   vlinsum(1.0, v, -gamma, I->jv, Av, I->n);
 }

@@ -66,6 +66,7 @@ def lib():
             "orc_set_linear": (None, [vp, i]),
             "orc_set_lin_solver": (None, [vp, i, d, i, i]),
             "orc_set_ls_reltol": (None, [vp, d]),
+            "orc_set_jtimes_analytic": (None, [vp, C.c_int]),
             "orc_evolve": (i, [vp, d, dp, dp]),
             "orc_get_stats": (i, [vp, C.POINTER(C.c_long), i]),
             "orc_get_current_h": (d, [vp]),

@@ -29,7 +29,7 @@ def ctx(pk):
 
 def run_pair(pk, ctx, kind, method, tout, prob_kw, rtol=1e-5, atol=1e-10, controller=0,
              fixed=0.0, internal=False, deefreq=25, fast=0, maxsteps=100000, dirk=None,
-             stagemax=200):
+             stagemax=200, jtimes=False):
     G = pk.Problem(ctx, kind, **prob_kw)
     P = O.OracleProblem(kind, **prob_kw)
     y0 = G.initial()
@@ -52,6 +52,9 @@ def run_pair(pk, ctx, kind, method, tout, prob_kw, rtol=1e-5, atol=1e-10, contro
         if d.get("reltol"):
             I.ARKodeSetLSRelativeTolerance(d["reltol"])
             R.set_ls_reltol(d["reltol"])
+        if jtimes:
+            I.analytic_jtimes()
+            R.set_jtimes_analytic(1)
     else:
         if internal:
             I.internal_eig(100, 0.01)
@@ -176,6 +179,19 @@ def test_dirk_fixed_step_reltol(pk, ctx):
     o = run_pair(pk, ctx, 1, "dirk", 0.01, dict(nx=96, ny=96), fixed=1e-3,
                  dirk=dict(order=3, maxl=2000, reltol=1e-8))
     check(o, fields=("nst", "nni", "nli"))
+
+
+@pytest.mark.parametrize("fixed", [0.0, 1e-3])
+def test_dirk_analytic_jtimes(pk, ctx, fixed):
+    """J v from the stencil on both sides: the same steps and Krylov
+    iterations as the oracle, and no RHS evaluations spent on J v."""
+    o = run_pair(pk, ctx, 1, "dirk", 0.01, dict(nx=80, ny=72), fixed=fixed, controller=0,
+                 dirk=dict(order=3, maxl=2000, reltol=1e-8), jtimes=True)
+    # a residual test that lands on its threshold can take one more Krylov
+    # iteration on one side (GPU tree vs serial reduction order)
+    check(o, fields=("nst", "nni", "nfe_ls"))
+    assert abs(o["sg"]["nli"] - o["sr"]["nli"]) <= max(1, 0.01 * o["sr"]["nli"])
+    assert o["sg"]["nfe_ls"] == 0 and o["sg"]["njtimes"] > 0
 
 
 def test_rhs_failure_is_fatal(pk, ctx):

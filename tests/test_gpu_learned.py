@@ -123,14 +123,16 @@ def test_training_lowers_the_loss(pk, ctx):
     net.release()
 
 
-def dirk_run(pk, ctx, prec, tout, dt, side, net=None, maxl=400, mrestart=30):
+def dirk_run(pk, ctx, prec, tout, dt, side, net=None, maxl=400, mrestart=30, ls=None, jtimes=False):
     G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=side, ny=side, kx=1.0, ky=1.0)
     if net is not None:
         net.attach(G)
-    ls = "cg" if prec == "jacobi" else "fgmres"
+    ls = ls or ("cg" if prec == "jacobi" else "fgmres")
     I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=maxl, precond="learned" if net is not None else True)
     if ls == "fgmres":
         pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
+    if jtimes:
+        I.analytic_jtimes()
     I.ARKodeSetOrder(3)
     I.ARKodeSetFixedStep(dt)
     I.ARKodeSetLSRelativeTolerance(1e-8)
@@ -183,4 +185,22 @@ def test_fallback_to_jacobi_pcg_at_the_cap(pk, ctx):
     assert capped["flag"] == 0 and capped["fallbacks"] > 0
     scale = np.abs(ref["y"]).max()
     assert np.abs(capped["y"] - ref["y"]).max() <= 1e-6 * scale
+    net.release()
+
+
+def test_flexible_pcg(pk, ctx):
+    """Flexible PCG: with the fixed Jacobi preconditioner it takes the PCG
+    steps; with the network it reaches the same solution."""
+    side, dt = 64, 0.02
+    dx = 1.0 / (side + 1)
+    ratio = 1767732205903.0 / 4055673282236.0 * dt / dx ** 2
+    ref = dirk_run(pk, ctx, "jacobi", 0.06, dt, side, jtimes=True)
+    fj = dirk_run(pk, ctx, "jacobi", 0.06, dt, side, ls="fcg", jtimes=True)
+    assert fj["flag"] == 0 and abs(fj["stats"]["nli"] - ref["stats"]["nli"]) <= 3
+    scale = np.abs(ref["y"]).max()
+    assert np.abs(fj["y"] - ref["y"]).max() <= 1e-8 * scale
+    net, _ = trained_net(pk, ctx, ratio)
+    fl = dirk_run(pk, ctx, "learned", 0.06, dt, side, net=net, ls="fcg", jtimes=True)
+    assert fl["flag"] == 0 and fl["fallbacks"] == 0
+    assert np.abs(fl["y"] - ref["y"]).max() <= 1e-6 * scale
     net.release()

@@ -323,6 +323,21 @@ def test_pcg_solve_matches_dst_exact(gamma):
     np.testing.assert_allclose(x, g[f"g{gamma}_x"].ravel(), rtol=0, atol=2e-8 * np.abs(x).max())
 
 
+@pytest.mark.parametrize("gamma", [1e-3, 1e-1])
+def test_pcg_with_analytic_jtimes_matches_dst_exact_tightly(gamma):
+    """J v from the stencil (no difference quotient): the solve now agrees with
+    the DST-exact solution to the solver tolerance."""
+    g = gold("dirichlet_solve.npz")
+    nx, ny, kx, ky = g["params"]
+    P = O.OracleProblem(O.MMS, nx=int(nx), ny=int(ny), kx=kx, ky=ky)
+    I = O.OracleIntegrator(P, O.DIRK, P.initial())
+    I.set_jtimes_analytic(1)
+    b = g[f"g{gamma}_b"].ravel()
+    ret, x, it = I.linear_solve(0.2, gamma, b, 1e-13, 2000)
+    assert ret == 0 and it > 0
+    np.testing.assert_allclose(x, g[f"g{gamma}_x"].ravel(), rtol=0, atol=1e-11 * np.abs(x).max())
+
+
 def test_dirk3_fixed_step_third_order():
     g = gold("mms_dst.npz")
     nx, ny, kx, ky = g["24x20_params"]
