@@ -285,14 +285,16 @@ def implicit_workload(pk, args):
     ident.init_identity(0.0, 0)
 
     def run(kind, mrestart=args.mrestart):
-        """kind: 'jacobi' (Jacobi-PCG), 'learned' (FGMRES + network), 'learned-fcg'
-        (flexible PCG + network), 'identity' (FGMRES + untrained network)."""
+        """kind: 'jacobi' (Jacobi-PCG), 'mg' (PCG + one multigrid V-cycle),
+        'learned' (FGMRES + network), 'learned-fcg' (flexible PCG + network),
+        'identity' (FGMRES + untrained network)."""
         G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=side, ny=side, kx=1.0, ky=1.0)
-        if kind != "jacobi":
+        if kind not in ("jacobi", "mg"):
             (ident if kind == "identity" else net).attach(G)
-        ls = {"jacobi": "cg", "learned-fcg": "fcg"}.get(kind, "fgmres")
+        ls = {"jacobi": "cg", "mg": "cg", "learned-fcg": "fcg"}.get(kind, "fgmres")
+        prec = {"jacobi": True, "mg": "mg"}.get(kind, "learned")
         I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=20000 if ls == "cg" else args.imaxl,
-                          precond=True if kind == "jacobi" else "learned")
+                          precond=prec)
         if ls == "fgmres":
             pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
         if args.jtimes:
@@ -325,6 +327,7 @@ def implicit_workload(pk, args):
         return out, y
 
     jac, yj = run("jacobi")
+    mg, ym = run("mg")
     lrn, yl = run("learned")
     fcg, yf = run("learned-fcg")
     if args.no_untrained:
@@ -333,11 +336,15 @@ def implicit_workload(pk, args):
         idn, yi = run("identity")
     rel = float(np.sqrt(np.mean((yl - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
     relf = float(np.sqrt(np.mean((yf - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
+    relm = float(np.sqrt(np.mean((ym - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
     out = {"metric": "time-to-solution at fixed L2 err, learned-preconditioned FGMRES vs Jacobi-PCG",
            "config": f"mms-dirichlet {side}x{side} kx=ky=1 FP64, DIRK3 fixed dt=1e-3, LS reltol 1e-8",
            "steps_timed": n_steps, "gamma_dt_over_dx2": ratio,
            "jtimes": "analytic (stencil)" if args.jtimes else "difference quotient (as the reference)",
-           "jacobi_pcg": jac, "learned_fgmres": lrn, "learned_flexible_pcg": fcg, "untrained_fgmres": idn,
+           "jacobi_pcg": jac, "multigrid_pcg": mg, "learned_fgmres": lrn, "learned_flexible_pcg": fcg,
+           "untrained_fgmres": idn,
+           "speedup_multigrid_vs_jacobi_pcg": jac["s_per_step"] / mg["s_per_step"],
+           "rel_l2_diff_multigrid_vs_jacobi_pcg": relm,
            "speedup_learned_vs_jacobi_pcg": jac["s_per_step"] / lrn["s_per_step"],
            "speedup_learned_fcg_vs_jacobi_pcg": jac["s_per_step"] / fcg["s_per_step"],
            "rel_l2_diff_learned_fcg_vs_jacobi_pcg": relf,

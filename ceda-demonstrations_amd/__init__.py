@@ -92,6 +92,9 @@ _SIGS = {
     "diffusion": (i32, [d, vp, vp, vp]),
     "PSetup": (i32, [d, vp, vp, i32, C.POINTER(i32), d, vp]),
     "PSolve": (i32, [d, vp, vp, vp, vp, d, d, i32, vp]),
+    "MGPSetup": (i32, [d, vp, vp, i32, C.POINTER(i32), d, vp]),
+    "MGPSolve": (i32, [d, vp, vp, vp, vp, d, d, i32, vp]),
+    "DiffusionProblem_MGLevels": (i32, [vp]),
     "dom_eig": (i32, [d, vp, vp, dp, dp, vp, vp, vp, vp]),
     "LSRKStepCreateSTS": (vp, [vp, d, vp, vp]),
     "ARKStepCreate": (vp, [vp, vp, d, vp, vp]),
@@ -342,6 +345,13 @@ class Problem:
     def psolve(self, r, z):
         return lib().PSolve(0.0, None, None, r.h, z.h, 0.0, 0.0, 1, self.h)
 
+    def mg_psetup(self, gamma):
+        jcur = i32()
+        return lib().MGPSetup(0.0, None, None, 0, C.byref(jcur), gamma, self.h)
+
+    def mg_psolve(self, r, z):
+        return lib().MGPSolve(0.0, None, None, r.h, z.h, 0.0, 0.0, 1, self.h)
+
     def dom_eig(self):
         lr, li = d(), d()
         lib().dom_eig(0.0, None, None, C.byref(lr), C.byref(li), self.h, None, None, None)
@@ -478,6 +488,8 @@ class Integrator:
                 # network on the problem (DiffusionProblem_SetLearnedPrec), Jacobi-PCG fallback
                 L.ARKodeSetPreconditioner(self.h, fn_ptr("LearnedPSetup"), fn_ptr("LearnedPSolve"))
                 L.ARKodeSetLSFallbackPCG(self.h, 100000, fn_ptr("PSolve"), prob.h)
+            elif precond == "mg":
+                L.ARKodeSetPreconditioner(self.h, fn_ptr("MGPSetup"), fn_ptr("MGPSolve"))
             elif precond:
                 L.ARKodeSetPreconditioner(self.h, fn_ptr("PSetup"), fn_ptr("PSolve"))
         if not self.h:

@@ -575,6 +575,7 @@ void DiffusionProblem_Destroy(DiffusionProblem* P)
   if (!P) return;
   CEDA_HIP_CHECK(hipStreamSynchronize(P->exec->stream));
   N_VDestroy(P->diag);
+  ceda::mg_destroy(P->mg);
   double* ds[] = {P->d_fx, P->d_fy, P->d_px, P->d_py, P->d_sx, P->d_sy, P->d_cx, P->d_cy,
                   P->d_ax, P->d_by, P->d_tf};
   for (double* d : ds)

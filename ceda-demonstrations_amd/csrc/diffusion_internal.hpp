@@ -20,6 +20,7 @@ struct StencilCfg {
 };
 
 struct LearnedPrec;
+struct MGHierarchy;
 
 struct DiffusionProblem {
   DiffusionParams p;
@@ -41,10 +42,13 @@ struct DiffusionProblem {
   // stages per pass K, 0 = single-stage kernels only; rows; nt)
   StencilCfg cfg[3];
   LearnedPrec* learned = nullptr;  // network of LearnedPSolve (not owned)
+  MGHierarchy* mg = nullptr;       // multigrid hierarchy of MGPSetup/MGPSolve (owned)
   bool tuned = false;
 };
 
 namespace ceda {
+
+void mg_destroy(MGHierarchy* H);
 
 // One entry of the device-resident stage table of the fused STS path.  Every
 // kernel of a step reads its coefficients, time factors and operand pointers

@@ -92,6 +92,20 @@ int DiffusionProblem_StagePass(DiffusionProblem* prob, int K, const double* coef
 int Initial(sunrealtype t, N_Vector u, void* user_data);
 int Exact(sunrealtype t, N_Vector u, void* user_data);
 int laplacian(sunrealtype t, N_Vector u, N_Vector f, void* user_data);
+/* [ext] geometric-multigrid preconditioner, the GPU counterpart of the
+   reference's hypre PFMG option (preconditioner_hypre.cpp:51-160, settings
+   diffusion_2D.hpp:187-193: one V-cycle, zero initial guess, symmetric
+   red/black Gauss-Seidel, 2 pre + 2 post sweeps), same signatures and user
+   data as PSetup/PSolve.  Manufactured (Dirichlet) problems only: -1 for
+   the periodic one.  Vertex-centred coarsening n -> n/2, rediscretised coarse
+   operators, full weighting, bilinear prolongation; the cycle is symmetric,
+   so it preconditions PCG. */
+int MGPSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok, sunbooleantype* jcurPtr,
+             sunrealtype gamma, void* user_data);
+int MGPSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z, sunrealtype gamma,
+             sunrealtype delta, int lr, void* user_data);
+/* levels of the hierarchy built by MGPSetup (0 before the first setup) */
+int DiffusionProblem_MGLevels(const DiffusionProblem* prob);
 /* [ext] ARKLsJacTimesVecFn: J v of the RHS, exactly the 5-point operator of
    laplacian() (diffusion.cpp:9-209; the manufactured forcing does not depend
    on u), for ARKodeSetJacTimes; the reference uses arkLs's difference

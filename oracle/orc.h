@@ -125,6 +125,9 @@ double orc_vwrmsnorm(const double* x, const double* w, int64_t n);
 double orc_vdot(const double* x, const double* y, int64_t n);
 double orc_vmaxnorm(const double* x, int64_t n);
 
+/* geometric-multigrid V-cycle (orc_mg.c): z = M r for I - gamma J */
+int orc_mg_apply(const orc_problem* p, double gamma, const double* r, double* z);
+
 /* learned preconditioner network (orc_learned.c) */
 int orc_lp_num_params(int nl, const int* dil);
 double orc_lp_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,

@@ -615,9 +615,14 @@ static void atimes(orc_integrator* I, double t, double gamma, const double* v, d
   vlinsum(1.0, v, -gamma, I->jv, Av, I->n);
 }
 
+/* precond 1: Jacobi (preconditioner_jacobi.cpp); 2: one multigrid V-cycle
+   (orc_mg.c) for the gamma of the last setup */
 static void psolve(orc_integrator* I, const double* r, double* z)
 {
-  if (I->precond) {
+  if (I->precond == 2) {
+    orc_mg_apply(I->P, I->gammap, r, z);
+    I->npsolve++;
+  } else if (I->precond) {
     orc_psolve(I->P, r, z);
     I->npsolve++;
   } else vcopy(z, r, I->n);
@@ -668,7 +673,7 @@ static int newton(orc_integrator* I, double t, double gamma, int force_setup)
                      fabs(gamma / I->gammap - 1.0) > 0.2 ||
                      (I->msbp > 0 && I->nst >= I->nstlp + I->msbp);
     if (need) {
-      orc_psetup(I->P, gamma);
+      if (I->precond != 2) orc_psetup(I->P, gamma);
       I->npsetup++;
       I->gammap = gamma;
       I->nstlp = I->nst;

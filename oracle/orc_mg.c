@@ -1,0 +1,190 @@
+/* CPU oracle of the geometric-multigrid preconditioner (test infrastructure
+ * only: used by tests/ to check the GPU V-cycle; never by the product).
+ *
+ * The reference's alternative preconditioner is one hypre PFMG V-cycle with
+ * zero initial guess, symmetric red/black Gauss-Seidel, 2 pre and 2 post
+ * sweeps (preconditioner_hypre.cpp:51-102, 142-160; settings
+ * diffusion_2D.hpp:187-193).  hypre is absent, so this is the build's own
+ * V-cycle with those settings, restated serially:
+ *   level 0: the operator of the implicit system, A = I - gamma J with J the
+ *            5-point stencil of the RHS face tables fx, fy (diffusion.cpp:48-53);
+ *   coarsening: n -> n/2 vertex-centred (coarse node I = fine node 2I+1,
+ *            so for even n the last coarse node is the last fine node), until
+ *            a side would drop below 3; coarse face I sits on fine node 2I:
+ *            coefficient (f[2I] + f[min(2I+1, n)]) / 8 (mean of the fine faces
+ *            beside it, times h^2/H^2 = 1/4);
+ *   restriction: full weighting (1/16)[1 2 1; 2 4 2; 1 2 1] = P^T / 4;
+ *   prolongation: bilinear, coarse ghosts zero;
+ *   smoothing: red (i+j even) then black sweeps before, black then red after
+ *            (the adjoint), so the V-cycle is symmetric; the coarsest level
+ *            gets 4 such symmetric pairs.
+ * Expression orders are fixed here and in the GPU kernels (no FMA), so the
+ * GPU V-cycle equals this one bit for bit. */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "orc.h"
+
+#define MG_MAXL 24
+
+typedef struct {
+  int nl, nu1, nu2, nuc;
+  int64_t nx[MG_MAXL], ny[MG_MAXL];
+  double *fx[MG_MAXL], *fy[MG_MAXL];
+  double *x[MG_MAXL], *b[MG_MAXL], *r[MG_MAXL];
+  double gamma;
+} orc_mg;
+
+static void mg_free(orc_mg* M)
+{
+  for (int l = 0; l < M->nl; l++) {
+    free(M->fx[l]);
+    free(M->fy[l]);
+    free(M->x[l]);
+    free(M->b[l]);
+    free(M->r[l]);
+  }
+  free(M);
+}
+
+static orc_mg* mg_setup(const double* fx0, const double* fy0, int64_t nx, int64_t ny, double gamma)
+{
+  orc_mg* M = calloc(1, sizeof(orc_mg));
+  M->nu1 = 2;
+  M->nu2 = 2;
+  M->nuc = 4;
+  M->gamma = gamma;
+  int l = 0;
+  for (;;) {
+    M->nx[l] = nx;
+    M->ny[l] = ny;
+    M->fx[l] = malloc(sizeof(double) * (nx + 1));
+    M->fy[l] = malloc(sizeof(double) * (ny + 1));
+    if (l == 0) {
+      memcpy(M->fx[0], fx0, sizeof(double) * (nx + 1));
+      memcpy(M->fy[0], fy0, sizeof(double) * (ny + 1));
+    } else {
+      const double* px = M->fx[l - 1];
+      const double* py = M->fy[l - 1];
+      const int64_t fxn = M->nx[l - 1], fyn = M->ny[l - 1];
+      for (int64_t i = 0; i <= nx; i++)
+        M->fx[l][i] = (px[2 * i < fxn ? 2 * i : fxn] + px[2 * i + 1 < fxn ? 2 * i + 1 : fxn]) / 8.0;
+      for (int64_t j = 0; j <= ny; j++)
+        M->fy[l][j] = (py[2 * j < fyn ? 2 * j : fyn] + py[2 * j + 1 < fyn ? 2 * j + 1 : fyn]) / 8.0;
+    }
+    M->x[l] = calloc(nx * ny, sizeof(double));
+    M->b[l] = calloc(nx * ny, sizeof(double));
+    M->r[l] = calloc(nx * ny, sizeof(double));
+    l++;
+    const int64_t cx = nx / 2, cy = ny / 2;
+    if (cx < 3 || cy < 3 || l == MG_MAXL) break;
+    nx = cx;
+    ny = cy;
+  }
+  M->nl = l;
+  return M;
+}
+
+/* off-diagonal sum and diagonal of A at (i, j) of level l */
+static void mg_row(const orc_mg* M, int l, const double* x, int64_t i, int64_t j, double* off, double* d)
+{
+  const int64_t nx = M->nx[l], ny = M->ny[l];
+  const double* fx = M->fx[l];
+  const double* fy = M->fy[l];
+  const int64_t k = j * nx + i;
+  const double xw = i > 0 ? x[k - 1] : 0.0, xe = i < nx - 1 ? x[k + 1] : 0.0;
+  const double xs = j > 0 ? x[k - nx] : 0.0, xn = j < ny - 1 ? x[k + nx] : 0.0;
+  *off = ((fx[i] * xw + fx[i + 1] * xe) + fy[j] * xs) + fy[j + 1] * xn;
+  *d = 1.0 + M->gamma * ((fx[i] + fx[i + 1]) + (fy[j] + fy[j + 1]));
+}
+
+static void mg_sweep(orc_mg* M, int l, int color)
+{
+  const int64_t nx = M->nx[l], ny = M->ny[l];
+  double* x = M->x[l];
+  const double* b = M->b[l];
+  for (int64_t j = 0; j < ny; j++)
+    for (int64_t i = (j + color) & 1; i < nx; i += 2) {
+      double off, d;
+      mg_row(M, l, x, i, j, &off, &d);
+      x[j * nx + i] = (b[j * nx + i] + M->gamma * off) / d;
+    }
+}
+
+static void mg_vcycle(orc_mg* M, int l)
+{
+  const int64_t nx = M->nx[l], ny = M->ny[l];
+  memset(M->x[l], 0, sizeof(double) * nx * ny);
+  if (l == M->nl - 1) {
+    for (int s = 0; s < M->nuc; s++) {
+      mg_sweep(M, l, 0);
+      mg_sweep(M, l, 1);
+    }
+    for (int s = 0; s < M->nuc; s++) {
+      mg_sweep(M, l, 1);
+      mg_sweep(M, l, 0);
+    }
+    return;
+  }
+  for (int s = 0; s < M->nu1; s++) {
+    mg_sweep(M, l, 0);
+    mg_sweep(M, l, 1);
+  }
+  /* residual */
+  double* x = M->x[l];
+  double* r = M->r[l];
+  for (int64_t j = 0; j < ny; j++)
+    for (int64_t i = 0; i < nx; i++) {
+      double off, d;
+      mg_row(M, l, x, i, j, &off, &d);
+      double t = d * x[j * nx + i];
+      t = t - M->gamma * off;
+      r[j * nx + i] = M->b[l][j * nx + i] - t;
+    }
+  /* full-weighting restriction */
+  const int64_t cx = M->nx[l + 1], cy = M->ny[l + 1];
+  double* bc = M->b[l + 1];
+  for (int64_t J = 0; J < cy; J++)
+    for (int64_t I = 0; I < cx; I++) {
+      const int64_t i = 2 * I + 1, j = 2 * J + 1;
+#define R_(a, c) (((a) < nx && (c) < ny) ? r[(c) * nx + (a)] : 0.0)
+      double t = 4.0 * R_(i, j);
+      t = t + 2.0 * ((R_(i - 1, j) + R_(i + 1, j)) + (R_(i, j - 1) + R_(i, j + 1)));
+      t = t + ((R_(i - 1, j - 1) + R_(i + 1, j - 1)) + (R_(i - 1, j + 1) + R_(i + 1, j + 1)));
+#undef R_
+      bc[J * cx + I] = t * 0.0625;
+    }
+  mg_vcycle(M, l + 1);
+  /* bilinear prolongation, added */
+  const double* xc = M->x[l + 1];
+  for (int64_t j = 0; j < ny; j++)
+    for (int64_t i = 0; i < nx; i++) {
+#define XC(a, c) (((a) >= 0 && (a) < cx && (c) >= 0 && (c) < cy) ? xc[(c) * cx + (a)] : 0.0)
+      double v;
+      const int64_t I = i / 2, J = j / 2;  /* odd i: coarse (i-1)/2 = I; even i: I-1 and I */
+      if ((i & 1) && (j & 1)) v = XC(I, J);
+      else if (j & 1) v = 0.5 * (XC(I - 1, J) + XC(I, J));
+      else if (i & 1) v = 0.5 * (XC(I, J - 1) + XC(I, J));
+      else v = 0.25 * ((XC(I - 1, J - 1) + XC(I, J - 1)) + (XC(I - 1, J) + XC(I, J)));
+#undef XC
+      x[j * nx + i] = x[j * nx + i] + v;
+    }
+  for (int s = 0; s < M->nu2; s++) {
+    mg_sweep(M, l, 1);
+    mg_sweep(M, l, 0);
+  }
+}
+
+/* z = one V-cycle applied to r for A = I - gamma J of the problem's grid;
+   returns the number of levels */
+int orc_mg_apply(const orc_problem* p, double gamma, const double* r, double* z)
+{
+  orc_mg* M = mg_setup(p->fx, p->fy, p->nx, p->ny, gamma);
+  memcpy(M->b[0], r, sizeof(double) * p->nx * p->ny);
+  mg_vcycle(M, 0);
+  memcpy(z, M->x[0], sizeof(double) * p->nx * p->ny);
+  const int nl = M->nl;
+  mg_free(M);
+  return nl;
+}

@@ -139,6 +139,16 @@ class OracleProblem:
         lib().orc_psolve(C.byref(self.p), ptr(r), ptr(z))
         return z
 
+    def mg_apply(self, gamma, r):
+        """One V-cycle of the multigrid preconditioner (orc_mg.c)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros(self.n)
+        f = lib().orc_mg_apply
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        nl = f(C.cast(C.pointer(self.p), C.c_void_p), gamma, ptr(r), ptr(z))
+        return z, nl
+
     def dom_eig(self):
         lr, li = C.c_double(), C.c_double()
         lib().orc_dom_eig(C.byref(self.p), C.byref(lr), C.byref(li))
