@@ -115,12 +115,14 @@ def roofline(pk, ctx, G):
             "rhs_stencil_GBs": round(stencil_gbs, 1)}
 
 
-def kernels_only(pk, reps, sweep):
+def kernels_only(pk, reps, sweep, pin_pass=None):
     """Launch the hot-path kernels alone at the workload's size (the workload
     of the rocprofv3 counter runs): the autotuned multi-stage pass, fused
     STS stage and RHS stencil, and a plain copy (N_VScale by 1: 8 B read +
     8 B written per element) whose counters calibrate FETCH_SIZE/WRITE_SIZE.
-    With sweep, first time every candidate launch shape of the kernels."""
+    With sweep, first time every candidate launch shape of the kernels; with
+    pin_pass (K, rows, nt), launch the pass at that shape instead of the
+    autotuned one (counter runs at a fixed shape)."""
     ctx = pk.Context()
     G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
     L = pk.lib()
@@ -154,6 +156,9 @@ def kernels_only(pk, reps, sweep):
     L.DiffusionProblem_TimeKernel(G.h, 0, vs[0].h, None, None, None, vs[4].h, reps, C.byref(avg))
     res["tuned_rhs_GBs"] = round(16.0 * G.n / (avg.value * 1e-3) / 1e9, 1)
     res["autotuned_pass"] = G.get_config(2)
+    if pin_pass:
+        assert G.set_config(2, *pin_pass) == 0
+        res["pinned_pass"] = G.get_config(2)
     if G.get_config(2)[0] >= 2:
         L.DiffusionProblem_TimeKernel(G.h, 2, vs[0].h, vs[1].h, vs[2].h, vs[3].h, vs[4].h, reps,
                                       C.byref(avg))
@@ -388,6 +393,8 @@ def main():
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
                     help="only launch the hot-path kernels REPS times each (profiling workload)")
     ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
+    ap.add_argument("--pin-pass", default=None, metavar="K,ROWS,NT",
+                    help="with --kernels-only: launch the multi-stage pass at this shape")
     ap.add_argument("--workload", default="sts", choices=["sts", "batch", "implicit"],
                     help="sts: the headline 4096^2 RKL2 line; batch: the 256-problem batched "
                          "comparison; implicit: learned-preconditioned FGMRES vs Jacobi-PCG")
@@ -413,7 +420,8 @@ def main():
 
     pk = package()
     if args.kernels_only:
-        kernels_only(pk, args.kernels_only, args.sweep)
+        pin = tuple(int(x) for x in args.pin_pass.split(",")) if args.pin_pass else None
+        kernels_only(pk, args.kernels_only, args.sweep, pin)
         return
     if args.workload == "batch":
         batch_workload(pk, args.nprob, args.tout)

@@ -95,6 +95,8 @@ _SIGS = {
     "MGPSetup": (i32, [d, vp, vp, i32, C.POINTER(i32), d, vp]),
     "MGPSolve": (i32, [d, vp, vp, vp, vp, d, d, i32, vp]),
     "DiffusionProblem_MGLevels": (i32, [vp]),
+    "DiffusionProblem_SetMGOptions": (i32, [vp, i32, i32]),
+    "DiffusionProblem_MGNumCycles": (C.c_long, [vp]),
     "dom_eig": (i32, [d, vp, vp, dp, dp, vp, vp, vp, vp]),
     "LSRKStepCreateSTS": (vp, [vp, d, vp, vp]),
     "ARKStepCreate": (vp, [vp, vp, d, vp, vp]),
@@ -351,6 +353,13 @@ class Problem:
 
     def mg_psolve(self, r, z):
         return lib().MGPSolve(0.0, None, None, r.h, z.h, 0.0, 0.0, 1, self.h)
+
+    def mg_options(self, relax, nrelax):
+        """V-cycle smoother: relax type 2 (symmetric) or 3, nrelax sweep pairs."""
+        return lib().DiffusionProblem_SetMGOptions(self.h, relax, nrelax)
+
+    def mg_cycles(self):
+        return lib().DiffusionProblem_MGNumCycles(self.h)
 
     def dom_eig(self):
         lr, li = d(), d()

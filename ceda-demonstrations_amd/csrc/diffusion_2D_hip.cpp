@@ -5,8 +5,14 @@
 // UserOutput::parse_args diffusion_2D.cpp:651-677), the same sequence of
 // integrator calls (main.cpp:176-400), the output loop with
 // ||u||_rms = sqrt(u.u/nx/ny) (main.cpp:431-470, diffusion_2D.cpp:801), the
-// tight-tolerance reference run of --error (main.cpp:387-465) and the final
-// lines that runtests-diffusion2d.py:36-54 parses.  Added flags:
+// tight-tolerance reference run of --error (main.cpp:387-465), the one-step
+// mode of --onestep (main.cpp:407-414), the solution files of --output 2
+// (UserOutput::open/write, diffusion_2D.cpp:729-787, read by
+// plot_data_2d.py:271-586) and the final lines that
+// runtests-diffusion2d.py:36-54 parses.  The hypre build's --pfmg_relax and
+// --pfmg_nrelax (diffusion_2D.cpp:137-151) select the GPU multigrid
+// preconditioner in place of Jacobi (manufactured problem only) and print
+// "Total PFMG iterations" (main.cpp:489-492).  Added flags:
 //   --problem ceda|mms   manufactured Dirichlet problem of README.md:11-32
 //   --deefreq N --stagemax N --eigsafety X   LSRKStep settings
 //   --fastpath 0|1|2     0: N_Vector API path, 1: fused stage kernels,
@@ -42,6 +48,7 @@ int main(int argc, char** argv)
   double rtol = 1e-5, atol = 1e-10, hfixed = 0.0, epslin = 0.0, eigsafety = 1.01, ls_reltol = 0.0;
   int order = 2, controller = 0, maxsteps = 0, error = 0, linear = 1, precond = 1;
   int liniters = 20, msbp = 0, internaleig = 0, output = 1, nout = 20, fastpath = 0;
+  int onestep = 0, pfmg = 0, pfmg_relax = 2, pfmg_nrelax = 2;
   long deefreq = 25;
   int stagemax = 200;
   const char* write_final = nullptr;
@@ -68,6 +75,14 @@ int main(int argc, char** argv)
     else if (TAKE("--order")) order = atoi(v);
     else if (TAKE("--controller")) controller = atoi(v);
     else if (TAKE("--maxsteps")) maxsteps = atoi(v);
+    else if (TAKE("--onestep")) onestep = atoi(v);
+    else if (TAKE("--pfmg_relax")) {
+      pfmg = 1;
+      pfmg_relax = atoi(v);
+    } else if (TAKE("--pfmg_nrelax")) {
+      pfmg = 1;
+      pfmg_nrelax = atoi(v);
+    }
     else if (!strcmp(k, "--error")) error = 1;
     else if (!strcmp(k, "--nonlinear")) linear = 0;
     else if (TAKE("--ls")) ls = v;
@@ -103,6 +118,11 @@ int main(int argc, char** argv)
   DiffusionProblem* udata = DiffusionProblem_Create(&prm, ctx);
   if (!udata) {
     fprintf(stderr, "ERROR: problem setup\n");
+    return 1;
+  }
+  if (pfmg && (kind != DIFFUSION_MMS_DIRICHLET || DiffusionProblem_SetMGOptions(udata, pfmg_relax, pfmg_nrelax))) {
+    fprintf(stderr, "ERROR: the multigrid preconditioner needs --problem mms, --pfmg_relax 2|3 and "
+                    "--pfmg_nrelax 0..16\n");
     return 1;
   }
   int64_t nx, ny;
@@ -144,7 +164,8 @@ int main(int argc, char** argv)
     ARKodeSetOrder(arkode_mem, order);
     ARKodeSetLinearSolver(arkode_mem, LS, nullptr);
     if (precond) {
-      ARKodeSetPreconditioner(arkode_mem, PSetup, PSolve);
+      if (pfmg) ARKodeSetPreconditioner(arkode_mem, MGPSetup, MGPSolve);
+      else ARKodeSetPreconditioner(arkode_mem, PSetup, PSolve);
       ARKodeSetLSetupFrequency(arkode_mem, msbp);
     }
     ARKodeSetEpsLin(arkode_mem, epslin);
@@ -189,17 +210,52 @@ int main(int argc, char** argv)
     ARKodeSetMaxNumSteps(arkref_mem, 100000L * maxsteps);
   }
 
+  int stepmode = ARK_NORMAL;
+  if (onestep) {
+    nout = onestep;
+    stepmode = ARK_ONE_STEP;
+  }
   double t = 0.0, t2 = 0.0, dTout = prm.tf / nout, tout = dTout, errtot = 0.0, simtime = 0.0;
   auto rms = [&](N_Vector v) { return sqrt(N_VDotProd(v, v) / (double)nx / (double)ny); };
+  // --output 2: one file per rank (here rank 0), a header of "# key value"
+  // lines in the default stream format, then per output time a row
+  // "t u_0 u_1 ... " in %.15e (UserOutput::open/write)
+  FILE* uout = nullptr;
+  if (output == 2) {
+    uout = fopen("diffusion_2d_solution.00000.txt", "w");
+    if (!uout) {
+      fprintf(stderr, "ERROR: cannot open diffusion_2d_solution.00000.txt\n");
+      return 1;
+    }
+    // node coordinates of the stored nodes run from xl to xu (periodic: the
+    // domain bounds; manufactured: the first and last interior nodes)
+    const bool mms = (kind == DIFFUSION_MMS_DIRICHLET);
+    const double xl = mms ? prm.xl + dx : prm.xl, xu = mms ? prm.xu - dx : prm.xu;
+    const double yl = mms ? prm.yl + dy : prm.yl, yu = mms ? prm.yu - dy : prm.yu;
+    fprintf(uout, "# title Diffusion 2D\n# nvar 1\n# vars u\n# nt  %d\n", nout + 1);
+    fprintf(uout, "# nx  %ld\n# xl  %g\n# xu  %g\n", (long)nx, xl, xu);
+    fprintf(uout, "# ny  %ld\n# yl  %g\n# yu  %g\n", (long)ny, yl, yu);
+    fprintf(uout, "# px  1\n# py  1\n# np  1\n# is  0\n# ie  %ld\n# js  0\n# je  %ld\n", (long)(nx - 1),
+            (long)(ny - 1));
+  }
+  auto write_solution = [&](double tt) {
+    if (!uout) return;
+    N_VCopyFromDevice_HipNative(u);  // synchronises the vector's stream
+    const double* h = N_VGetArrayPointer(u);
+    fprintf(uout, "%.15e ", tt);
+    for (int64_t k = 0; k < nx * ny; k++) fprintf(uout, "%.15e ", h[k]);
+    fprintf(uout, "\n");
+  };
   if (output > 0) {
     printf("\n          t                     ||u||_rms      \n");
     printf(" ----------------------------------------------\n");
     printf("%22.15e%25.15e\n", t, rms(u));
   }
+  write_solution(t);
   for (int iout = 0; iout < nout; iout++) {
     const double t0 = wall();
     if (error) ARKodeSetStopTime(arkode_mem, tout);
-    int flag = ARKodeEvolve(arkode_mem, tout, u, &t, ARK_NORMAL);
+    int flag = ARKodeEvolve(arkode_mem, tout, u, &t, stepmode);
     (void)hipDeviceSynchronize();
     simtime += wall() - t0;
     if (flag < 0) {
@@ -217,10 +273,12 @@ int main(int argc, char** argv)
       if (e > errtot) errtot = e;
     }
     if (output > 0) printf("%22.15e%25.15e\n", t, rms(u));
+    write_solution(t);
     tout += dTout;
     if (tout > prm.tf) tout = prm.tf;
   }
   if (output > 0) printf(" ----------------------------------------------\n\n");
+  if (uout) fclose(uout);
   if (error) printf("Maximum relative error = %.16e\n", errtot);
   if (kind == DIFFUSION_MMS_DIRICHLET) {
     N_Vector ue = N_VClone(u);
@@ -236,6 +294,7 @@ int main(int argc, char** argv)
   printf("Total simulation time = %.6f\n\n", simtime);
   printf("Final integrator statistics:\n");
   ARKodePrintAllStats(arkode_mem, stdout, SUN_OUTPUTFORMAT_TABLE);
+  if (impl && precond && pfmg) printf("Total PFMG iterations        = %ld\n", DiffusionProblem_MGNumCycles(udata));
   if (write_final) {
     N_VCopyFromDevice_HipNative(u);
     FILE* fp = fopen(write_final, "wb");

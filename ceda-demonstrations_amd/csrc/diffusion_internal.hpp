@@ -43,6 +43,7 @@ struct DiffusionProblem {
   StencilCfg cfg[3];
   LearnedPrec* learned = nullptr;  // network of LearnedPSolve (not owned)
   MGHierarchy* mg = nullptr;       // multigrid hierarchy of MGPSetup/MGPSolve (owned)
+  int mg_relax = 2, mg_nrelax = 2;  // V-cycle smoother: PFMG relax type 2|3, sweeps
   bool tuned = false;
 };
 

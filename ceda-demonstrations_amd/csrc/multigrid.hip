@@ -2,8 +2,11 @@
 // V-cycle per application for A = I - gamma J (J the 5-point stencil of the
 // RHS face tables), with the settings of the reference's hypre PFMG
 // preconditioner (preconditioner_hypre.cpp:51-102, 142-160;
-// diffusion_2D.hpp:187-193): zero initial guess, symmetric red/black
-// Gauss-Seidel, 2 pre and 2 post sweeps, one cycle.
+// diffusion_2D.hpp:187-193): zero initial guess, one cycle, red/black
+// Gauss-Seidel smoothing, by default symmetric (PFMG relax type 2) with 2 pre
+// and 2 post sweeps; relax type 3 (nonsymmetric: red then black both before
+// and after the coarse correction) and the sweep count are the reference's
+// --pfmg_relax / --pfmg_nrelax options (diffusion_2D.cpp:137-151).
 //
 // Hierarchy: vertex-centred coarsening n -> n/2 (coarse node I on fine node
 // 2I+1) until a side would drop below 3; coarse operators rediscretised from
@@ -37,12 +40,14 @@ struct MGHierarchy {
   double* gamma = nullptr;  // device scalar
   double* h_gamma = nullptr;
   hipGraphExec_t graph = nullptr;
+  int relax = 2, nrelax = 2;  // the options the graph was captured with
+  long cycles = 0;            // V-cycles applied (MGPSolve calls)
 };
 
 namespace {
 
 constexpr int kMGMaxL = 24;
-constexpr int kPre = 2, kPost = 2, kCoarse = 4;
+constexpr int kCoarse = 4;
 
 unsigned grid_of(int64_t n)
 {
@@ -151,18 +156,21 @@ void vcycle(MGHierarchy* H, int l, hipStream_t s)
 {
   const MGLevel& L = H->L[l];
   hipLaunchKernelGGL(k_mg_fill0, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, L.nx * L.ny, L.x);
+  // post-smoothing order: black then red (the adjoint of the pre-smoother,
+  // relax type 2) or red then black again (relax type 3)
+  const int p0 = (H->relax == 2) ? 1 : 0;
   if (l == H->nl - 1) {
     for (int k = 0; k < kCoarse; k++) {
       sweep(H, l, 0, s);
       sweep(H, l, 1, s);
     }
     for (int k = 0; k < kCoarse; k++) {
-      sweep(H, l, 1, s);
-      sweep(H, l, 0, s);
+      sweep(H, l, p0, s);
+      sweep(H, l, 1 - p0, s);
     }
     return;
   }
-  for (int k = 0; k < kPre; k++) {
+  for (int k = 0; k < H->nrelax; k++) {
     sweep(H, l, 0, s);
     sweep(H, l, 1, s);
   }
@@ -174,9 +182,9 @@ void vcycle(MGHierarchy* H, int l, hipStream_t s)
   vcycle(H, l + 1, s);
   hipLaunchKernelGGL(k_mg_prolong, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, L.nx, L.ny, L.x, C.nx,
                      C.ny, C.x);
-  for (int k = 0; k < kPost; k++) {
-    sweep(H, l, 1, s);
-    sweep(H, l, 0, s);
+  for (int k = 0; k < H->nrelax; k++) {
+    sweep(H, l, p0, s);
+    sweep(H, l, 1 - p0, s);
   }
 }
 
@@ -245,7 +253,13 @@ int MGPSetup(sunrealtype, N_Vector, N_Vector, sunbooleantype, sunbooleantype* jc
   CEDA_HIP_CHECK(hipStreamSynchronize(s));  // the previous gamma may still be read
   *H->h_gamma = gamma;
   CEDA_HIP_CHECK(hipMemcpyAsync(H->gamma, H->h_gamma, sizeof(double), hipMemcpyHostToDevice, s));
+  if (H->graph && (H->relax != P->mg_relax || H->nrelax != P->mg_nrelax)) {
+    CEDA_HIP_CHECK(hipGraphExecDestroy(H->graph));
+    H->graph = nullptr;
+  }
   if (!H->graph) {
+    H->relax = P->mg_relax;
+    H->nrelax = P->mg_nrelax;
     hipGraph_t g;
     CEDA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     vcycle(H, 0, s);
@@ -269,9 +283,20 @@ int MGPSolve(sunrealtype, N_Vector, N_Vector, N_Vector r, N_Vector z, sunrealtyp
   CEDA_HIP_CHECK(hipMemcpyAsync(H->L[0].b, dptr(r), bytes, hipMemcpyDeviceToDevice, s));
   CEDA_HIP_CHECK(hipGraphLaunch(H->graph, s));
   CEDA_HIP_CHECK(hipMemcpyAsync(dptr(z), H->L[0].x, bytes, hipMemcpyDeviceToDevice, s));
+  H->cycles++;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int DiffusionProblem_MGLevels(const DiffusionProblem* P) { return (P && P->mg) ? P->mg->nl : 0; }
+
+int DiffusionProblem_SetMGOptions(DiffusionProblem* P, int relax, int nrelax)
+{
+  if (!P || (relax != 2 && relax != 3) || nrelax < 0 || nrelax > 16) return -1;
+  P->mg_relax = relax;
+  P->mg_nrelax = nrelax;
+  return 0;
+}
+
+long DiffusionProblem_MGNumCycles(const DiffusionProblem* P) { return (P && P->mg) ? P->mg->cycles : 0; }
 
 }  // extern "C"

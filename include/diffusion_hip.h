@@ -106,6 +106,16 @@ int MGPSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z, sunr
              sunrealtype delta, int lr, void* user_data);
 /* levels of the hierarchy built by MGPSetup (0 before the first setup) */
 int DiffusionProblem_MGLevels(const DiffusionProblem* prob);
+/* [ext] V-cycle smoother of MGPSetup/MGPSolve, the reference's --pfmg_relax
+   and --pfmg_nrelax (diffusion_2D.cpp:137-151, preconditioner_hypre.cpp:91-98):
+   relax 2 = symmetric red/black Gauss-Seidel (red, black before the coarse
+   correction; black, red after), 3 = nonsymmetric (red, black both times);
+   nrelax pre- and post-sweep pairs (0..16).  Default 2, 2.  Takes effect at
+   the next MGPSetup.  0, or -1 for other values. */
+int DiffusionProblem_SetMGOptions(DiffusionProblem* prob, int relax, int nrelax);
+/* V-cycles applied so far (MGPSolve calls): hypre's PFMG iteration count,
+   printed by main.cpp:489-492 as "Total PFMG iterations" */
+long DiffusionProblem_MGNumCycles(const DiffusionProblem* prob);
 /* [ext] ARKLsJacTimesVecFn: J v of the RHS, exactly the 5-point operator of
    laplacian() (diffusion.cpp:9-209; the manufactured forcing does not depend
    on u), for ARKodeSetJacTimes; the reference uses arkLs's difference

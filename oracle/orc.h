@@ -127,6 +127,8 @@ double orc_vmaxnorm(const double* x, int64_t n);
 
 /* geometric-multigrid V-cycle (orc_mg.c): z = M r for I - gamma J */
 int orc_mg_apply(const orc_problem* p, double gamma, const double* r, double* z);
+/* the same with smoother relax type 2 (symmetric) or 3 and nu sweep pairs */
+int orc_mg_apply_opts(const orc_problem* p, double gamma, int relax, int nu, const double* r, double* z);
 
 /* learned preconditioner network (orc_learned.c) */
 int orc_lp_num_params(int nl, const int* dil);

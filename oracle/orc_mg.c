@@ -15,9 +15,11 @@
  *            beside it, times h^2/H^2 = 1/4);
  *   restriction: full weighting (1/16)[1 2 1; 2 4 2; 1 2 1] = P^T / 4;
  *   prolongation: bilinear, coarse ghosts zero;
- *   smoothing: red (i+j even) then black sweeps before, black then red after
- *            (the adjoint), so the V-cycle is symmetric; the coarsest level
- *            gets 4 such symmetric pairs.
+ *   smoothing: nu pairs of red (i+j even) then black sweeps before; after,
+ *            nu pairs of black then red (the adjoint: relax type 2, the
+ *            V-cycle is symmetric) or red then black again (relax type 3,
+ *            diffusion_2D.cpp:137-151); the coarsest level gets 4 pairs in
+ *            each order.  Default relax 2, nu 2.
  * Expression orders are fixed here and in the GPU kernels (no FMA), so the
  * GPU V-cycle equals this one bit for bit. */
 #include <stdint.h>
@@ -29,7 +31,7 @@
 #define MG_MAXL 24
 
 typedef struct {
-  int nl, nu1, nu2, nuc;
+  int nl, nu1, nu2, nuc, relax;
   int64_t nx[MG_MAXL], ny[MG_MAXL];
   double *fx[MG_MAXL], *fy[MG_MAXL];
   double *x[MG_MAXL], *b[MG_MAXL], *r[MG_MAXL];
@@ -48,12 +50,14 @@ static void mg_free(orc_mg* M)
   free(M);
 }
 
-static orc_mg* mg_setup(const double* fx0, const double* fy0, int64_t nx, int64_t ny, double gamma)
+static orc_mg* mg_setup(const double* fx0, const double* fy0, int64_t nx, int64_t ny, double gamma,
+                        int relax, int nu)
 {
   orc_mg* M = calloc(1, sizeof(orc_mg));
-  M->nu1 = 2;
-  M->nu2 = 2;
+  M->nu1 = nu;
+  M->nu2 = nu;
   M->nuc = 4;
+  M->relax = relax;
   M->gamma = gamma;
   int l = 0;
   for (;;) {
@@ -122,8 +126,8 @@ static void mg_vcycle(orc_mg* M, int l)
       mg_sweep(M, l, 1);
     }
     for (int s = 0; s < M->nuc; s++) {
-      mg_sweep(M, l, 1);
-      mg_sweep(M, l, 0);
+      mg_sweep(M, l, M->relax == 2 ? 1 : 0);
+      mg_sweep(M, l, M->relax == 2 ? 0 : 1);
     }
     return;
   }
@@ -171,20 +175,25 @@ static void mg_vcycle(orc_mg* M, int l)
       x[j * nx + i] = x[j * nx + i] + v;
     }
   for (int s = 0; s < M->nu2; s++) {
-    mg_sweep(M, l, 1);
-    mg_sweep(M, l, 0);
+    mg_sweep(M, l, M->relax == 2 ? 1 : 0);
+    mg_sweep(M, l, M->relax == 2 ? 0 : 1);
   }
 }
 
-/* z = one V-cycle applied to r for A = I - gamma J of the problem's grid;
-   returns the number of levels */
-int orc_mg_apply(const orc_problem* p, double gamma, const double* r, double* z)
+/* z = one V-cycle (smoother relax type 2|3, nu sweep pairs) applied to r
+   for A = I - gamma J of the problem's grid; returns the number of levels */
+int orc_mg_apply_opts(const orc_problem* p, double gamma, int relax, int nu, const double* r, double* z)
 {
-  orc_mg* M = mg_setup(p->fx, p->fy, p->nx, p->ny, gamma);
+  orc_mg* M = mg_setup(p->fx, p->fy, p->nx, p->ny, gamma, relax, nu);
   memcpy(M->b[0], r, sizeof(double) * p->nx * p->ny);
   mg_vcycle(M, 0);
   memcpy(z, M->x[0], sizeof(double) * p->nx * p->ny);
   const int nl = M->nl;
   mg_free(M);
   return nl;
+}
+
+int orc_mg_apply(const orc_problem* p, double gamma, const double* r, double* z)
+{
+  return orc_mg_apply_opts(p, gamma, 2, 2, r, z);
 }

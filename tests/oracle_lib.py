@@ -139,14 +139,16 @@ class OracleProblem:
         lib().orc_psolve(C.byref(self.p), ptr(r), ptr(z))
         return z
 
-    def mg_apply(self, gamma, r):
-        """One V-cycle of the multigrid preconditioner (orc_mg.c)."""
+    def mg_apply(self, gamma, r, relax=2, nu=2):
+        """One V-cycle of the multigrid preconditioner (orc_mg.c): smoother
+        relax type 2 (symmetric red/black Gauss-Seidel) or 3, nu sweep pairs."""
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.zeros(self.n)
-        f = lib().orc_mg_apply
+        f = lib().orc_mg_apply_opts
         f.restype = C.c_int
-        f.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        nl = f(C.cast(C.pointer(self.p), C.c_void_p), gamma, ptr(r), ptr(z))
+        f.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double),
+                      C.POINTER(C.c_double)]
+        nl = f(C.cast(C.pointer(self.p), C.c_void_p), gamma, relax, nu, ptr(r), ptr(z))
         return z, nl
 
     def dom_eig(self):

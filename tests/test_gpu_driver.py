@@ -1,0 +1,149 @@
+"""The reference-style driver (ceda-demonstrations_amd/diffusion_2D_hip, the
+call sequence of diffusion_2D/main.cpp on the GPU), run as a program:
+
+- its stdout carries the lines runtests-diffusion2d.py:36-54 parses, read
+  here by the same token rules;
+- --output 2 writes diffusion_2d_solution.00000.txt in the layout of
+  UserOutput::open/write (diffusion_2D.cpp:729-787) that plot_data_2d.py
+  reads (header 271-360, rows 536-586): its first row is the initial state,
+  its last the final state, its times the output times;
+- --onestep N takes N single steps (main.cpp:407-414);
+- --error reports the maximum relative error against the tight-tolerance
+  reference run (main.cpp:387-465);
+- --pfmg_relax / --pfmg_nrelax (the hypre build's flags) switch the implicit
+  path to the multigrid preconditioner, reach the Jacobi-PCG solution and
+  print "Total PFMG iterations" (main.cpp:489-492)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from hiplib import PKG_DIR
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(PKG_DIR, "diffusion_2D_hip")
+
+
+def run(args, cwd, expect=0):
+    assert os.path.exists(EXE), f"driver not built: {EXE}"
+    print("driver:", " ".join(args), flush=True)
+    r = subprocess.run([EXE] + args, cwd=cwd, capture_output=True, text=True, timeout=180)
+    assert r.returncode == expect, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+# (tokens that must all be on the line, field index, key, accumulate) in the
+# order runtests-diffusion2d.py:36-54 tries them
+RULES = [(("Steps",), 2, "Steps", False), (("Error", "test"), 4, "Fails", False),
+         (("Maximum", "relative"), 4, "Accuracy", False), (("Explicit", "RHS"), 5, "FEvals", True),
+         (("Implicit", "RHS"), 5, "FEvals", True), (("LS", "RHS"), 5, "FEvals", True),
+         (("RHS", "fn"), 4, "FEvals", True), (("simulation",), 4, "Runtime", False)]
+
+
+def harness_stats(stdout):
+    stats = {"Steps": None, "Fails": None, "Accuracy": None, "FEvals": 0, "Runtime": None}
+    for line in stdout.split("\n"):
+        txt = line.split()
+        for tokens, idx, key, acc in RULES:
+            if all(t in txt for t in tokens):
+                val = float(txt[idx]) if key in ("Accuracy", "Runtime") else int(txt[idx])
+                stats[key] = stats[key] + val if acc else val
+                break
+    return stats
+
+
+def read_solution_file(path):
+    """Header '# key value' lines and the data rows, as plot_data_2d.py reads them."""
+    info = {}
+    with open(path) as f:
+        for line in f:
+            if not line.strip():
+                continue
+            if "#" not in line:
+                break
+            txt = line.split()
+            info[txt[1]] = " ".join(txt[2:])
+    return info, np.loadtxt(path, dtype=np.double, ndmin=2)
+
+
+def test_sts_stdout_and_solution_file(tmp_path):
+    nx, ny, tf, nout = 48, 40, 0.02, 4
+    out = run(["--problem", "mms", "--integrator", "rkl", "--nx", str(nx), "--ny", str(ny), "--kx", "1",
+               "--ky", "0.5", "--tf", str(tf), "--rtol", "1e-6", "--nout", str(nout), "--output", "2",
+               "--fastpath", "2", "--write_final", "final.bin"], tmp_path)
+    st = harness_stats(out)
+    assert st["Steps"] > 0 and st["Fails"] >= 0 and st["FEvals"] > st["Steps"] and st["Runtime"] >= 0.0
+    assert st["Accuracy"] is None
+    info, data = read_solution_file(tmp_path / "diffusion_2d_solution.00000.txt")
+    assert info["title"] == "Diffusion 2D" and info["vars"] == "u"
+    assert int(info["nvar"]) == 1 and int(info["nt"]) == nout + 1
+    assert int(info["nx"]) == nx and int(info["ny"]) == ny
+    assert (int(info["is"]), int(info["ie"]), int(info["js"]), int(info["je"])) == (0, nx - 1, 0, ny - 1)
+    dx, dy = 1.0 / (nx + 1), 1.0 / (ny + 1)  # interior nodes of the unit square
+    for k, v in (("xl", dx), ("xu", 1 - dx), ("yl", dy), ("yu", 1 - dy)):
+        assert abs(float(info[k]) - v) <= 1e-5 * max(1.0, abs(v)), k
+    assert data.shape == (nout + 1, 1 + nx * ny)
+    np.testing.assert_allclose(data[:, 0], np.linspace(0.0, tf, nout + 1), rtol=0, atol=1e-15)
+    P = O.OracleProblem(O.MMS, nx=nx, ny=ny, kx=1.0, ky=0.5)
+    np.testing.assert_allclose(data[0, 1:], P.initial(), rtol=1e-15, atol=0)
+    final = np.fromfile(tmp_path / "final.bin", dtype=np.float64)
+    np.testing.assert_allclose(data[-1, 1:], final, rtol=1e-15, atol=0)
+    # plot_data_2d.py's reshape: x varies first
+    np.testing.assert_allclose(data[-1, 1:].reshape(ny, nx), final.reshape(ny, nx), rtol=1e-15, atol=0)
+    # the state follows the manufactured solution (48x40 grid: spatial error
+    # O(dx^2) ~ 1e-4), and the driver's printed error metric is that error
+    err = np.sqrt(np.mean((final - P.exact(tf)) ** 2))
+    printed = float([ln for ln in out.split("\n") if ln.startswith("L2 error vs exact")][0].split()[-1])
+    assert err < 1e-3
+    assert abs(printed - err) <= 1e-6 * err
+
+
+def test_onestep_mode(tmp_path):
+    out = run(["--problem", "mms", "--integrator", "rkc", "--nx", "32", "--ny", "32", "--onestep", "7"],
+              tmp_path)
+    assert harness_stats(out)["Steps"] == 7
+    rows = [ln for ln in out.split("\n") if len(ln.split()) == 2 and ln.strip()[0].isdigit()]
+    assert len(rows) == 8  # the initial line and one per step
+
+
+def test_error_mode(tmp_path):
+    # the harness's own --maxsteps (runtests-diffusion2d.py:81): the default 0
+    # means 500 steps per output (SUNDIALS), too few for the rtol 1e-12 run
+    out = run(["--integrator", "rkc", "--nx", "32", "--ny", "32", "--tf", "0.1", "--nout", "2", "--error",
+               "--maxsteps", "100000"], tmp_path)
+    acc = harness_stats(out)["Accuracy"]
+    assert acc is not None and 0.0 < acc < 1e-2
+
+
+def dirk(tmp_path, name, extra):
+    d = tmp_path / name
+    d.mkdir()
+    out = run(["--problem", "mms", "--integrator", "dirk", "--order", "3", "--nx", "64", "--ny", "64",
+               "--fixedstep", "1e-3", "--tf", "0.005", "--nout", "1", "--ls_reltol", "1e-10",
+               "--liniters", "2000", "--write_final", "final.bin"] + extra, d)
+    return out, np.fromfile(d / "final.bin", dtype=np.float64)
+
+
+def ls_iters(out):
+    return int([ln for ln in out.split("\n") if ln.startswith("LS iters")][0].split()[-1])
+
+
+@pytest.mark.parametrize("relax,nrelax", [(3, 2), (2, 1)])
+def test_pfmg_flags_select_multigrid(tmp_path, relax, nrelax):
+    ref_out, ref = dirk(tmp_path, "jacobi", [])
+    assert "PFMG" not in ref_out
+    out, y = dirk(tmp_path, "mg", ["--pfmg_relax", str(relax), "--pfmg_nrelax", str(nrelax)])
+    its = int([ln for ln in out.split("\n") if ln.startswith("Total PFMG iterations")][0].split()[-1])
+    assert its > 0
+    assert ls_iters(out) * 3 < ls_iters(ref_out)
+    assert harness_stats(out)["Steps"] == harness_stats(ref_out)["Steps"] == 5
+    np.testing.assert_allclose(y, ref, rtol=1e-8, atol=1e-12)
+
+
+def test_pfmg_flags_rejected_where_unsupported(tmp_path):
+    run(["--integrator", "dirk", "--nx", "16", "--ny", "16", "--pfmg_relax", "3"], tmp_path, expect=1)
+    run(["--problem", "mms", "--integrator", "dirk", "--nx", "16", "--ny", "16", "--pfmg_relax", "1"],
+        tmp_path, expect=1)

@@ -42,6 +42,24 @@ def test_vcycle_bitwise(pk, ctx, nx, ny, kx, ky, gamma):
     np.testing.assert_array_equal(Z.get(), P.mg_apply(2 * gamma, r)[0])
 
 
+@pytest.mark.parametrize("relax,nu", [(2, 1), (2, 3), (3, 2), (3, 1), (2, 0)])
+def test_vcycle_options_bitwise(pk, ctx, relax, nu):
+    """--pfmg_relax / --pfmg_nrelax: every smoother option equals the oracle's,
+    switching options recaptures the cycle, and the cycle counter counts."""
+    nx, ny, gamma = 97, 80, 2e-2
+    G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=nx, ny=ny, kx=1.0, ky=0.5)
+    P = O.OracleProblem(O.MMS, nx=nx, ny=ny, kx=1.0, ky=0.5)
+    r = np.random.default_rng(relax * 10 + nu).standard_normal(P.n)
+    R, Z = G.vector(r), G.vector()
+    assert G.mg_psetup(gamma) == 0 and G.mg_psolve(R, Z) == 0
+    np.testing.assert_array_equal(Z.get(), P.mg_apply(gamma, r)[0])
+    assert G.mg_options(relax, nu) == 0
+    assert G.mg_psetup(gamma) == 0 and G.mg_psolve(R, Z) == 0
+    np.testing.assert_array_equal(Z.get(), P.mg_apply(gamma, r, relax, nu)[0])
+    assert G.mg_cycles() == 2
+    assert G.mg_options(1, 2) == -1 and G.mg_options(2, -1) == -1 and G.mg_options(3, 17) == -1
+
+
 def test_periodic_problem_is_rejected(pk, ctx):
     G = pk.Problem(ctx, pk.CEDA_PERIODIC, nx=16, ny=16)
     assert G.mg_psetup(1e-3) == -1

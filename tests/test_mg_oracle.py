@@ -56,3 +56,48 @@ def test_single_level_is_symmetric_gauss_seidel():
     for _ in range(4):
         sweep(black); sweep(red)
     np.testing.assert_allclose(z, x, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("nu,bound", [(1, 0.35), (3, 0.2)])
+def test_vcycle_sweep_count_option(nu, bound):
+    """--pfmg_nrelax: symmetric for any sweep count; fewer sweeps contract less."""
+    P = O.OracleProblem(O.MMS, nx=23, ny=19, kx=1.0, ky=1.0)
+    n, gamma = P.n, 2e-2
+    M = dense(lambda e: P.mg_apply(gamma, e, 2, nu)[0], n)
+    assert np.abs(M - M.T).max() <= 1e-12 * np.abs(M).max()
+    A = dense(lambda e: e - gamma * P.rhs(0.0, e) + gamma * P.rhs(0.0, np.zeros(n)), n)
+    rho = np.abs(np.linalg.eigvals(np.eye(n) - M @ A)).max()
+    assert rho < bound, rho
+    rho2 = np.abs(np.linalg.eigvals(np.eye(n) - dense(lambda e: P.mg_apply(gamma, e, 2, 2)[0], n) @ A)).max()
+    assert (rho > rho2) if nu == 1 else (rho < rho2)
+
+
+def test_vcycle_nonsymmetric_relax_type():
+    """--pfmg_relax 3 (red then black on both sides of the coarse correction):
+    no longer symmetric, still a contraction."""
+    P = O.OracleProblem(O.MMS, nx=23, ny=19, kx=1.0, ky=0.5)
+    n, gamma = P.n, 2e-2
+    M = dense(lambda e: P.mg_apply(gamma, e, 3, 2)[0], n)
+    assert np.abs(M - M.T).max() > 1e-6 * np.abs(M).max()
+    A = dense(lambda e: e - gamma * P.rhs(0.0, e) + gamma * P.rhs(0.0, np.zeros(n)), n)
+    assert np.abs(np.linalg.eigvals(np.eye(n) - M @ A)).max() < 0.4
+
+
+def test_single_level_nonsymmetric_is_red_black_gauss_seidel():
+    """Relax type 3 on a grid too small to coarsen: 8 red-black pairs."""
+    P = O.OracleProblem(O.MMS, nx=5, ny=6, kx=1.0, ky=1.0)
+    gamma = 0.01
+    r = np.random.default_rng(1).standard_normal(P.n)
+    z, nl = P.mg_apply(gamma, r, 3, 2)
+    assert nl == 1
+    n = P.n
+    A = dense(lambda e: e - gamma * P.rhs(0.0, e) + gamma * P.rhs(0.0, np.zeros(n)), n)
+    x = np.zeros(n)
+    red = [k for k in range(n) if ((k % 5) + (k // 5)) % 2 == 0]
+    black = [k for k in range(n) if k not in red]
+    def sweep(idx):
+        for k in idx:
+            x[k] = (r[k] - (A[k] @ x - A[k, k] * x[k])) / A[k, k]
+    for _ in range(8):
+        sweep(red); sweep(black)
+    np.testing.assert_allclose(z, x, rtol=1e-13, atol=1e-15)
