@@ -610,7 +610,7 @@ int DiffusionProblem_SetStencilConfig(DiffusionProblem* P, int which, int varian
 {
   if (!P || which < 0 || which > 2 || rows < 1 || rows > 1024 || (nt != 0 && nt != 1)) return -1;
   if (which < 2 && (variant < 0 || variant > 1 || rows > 256 || (variant == 0 && nt))) return -1;
-  if (which == 2 && (variant < 0 || variant > 4)) return -1;
+  if (which == 2 && (variant < 0 || variant > kMaxPassStages)) return -1;
   P->cfg[which].variant = variant;
   P->cfg[which].rows = rows;
   P->cfg[which].nt = nt;
@@ -635,10 +635,12 @@ namespace {
 // stage K -> ops[5]
 StageEntry* pass_table(double* const* ops, int K, hipStream_t s)
 {
-  StageEntry h[4];
+  // the kernel reads entries 0..K-1: the table holds the deepest pass
+  if (K < 1 || K > kMaxPassStages) return nullptr;
+  StageEntry h[kMaxPassStages];
   memset(h, 0, sizeof(h));
   const double hc[5] = {1.0e-9, -0.5, 0.25, 1.25, -1.0e-10};
-  for (int m = 0; m < 4; m++) {
+  for (int m = 0; m < kMaxPassStages; m++) {
     for (int k = 0; k < 5; k++) h[m].c[k] = hc[k];
     h[m].T2 = 1.0;
     h[m].yjm1 = ops[0];
@@ -721,7 +723,7 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
   // heights between 16 and 64 matter: the K-1 halo rows above and below a
   // tile are re-read (mostly from HBM), while tall tiles leave too few
   // wavefronts or a partial last round of them
-  for (int K : {2, 3, 4}) {
+  for (int K : {2, 3, 4, 5, 6}) {
     StageEntry* dtab = pass_table(buf, K, s);
     for (int rows : {8, 16, 24, 32, 40, 48, 64, 96, 128})
       for (int nt : {0, 1}) {
@@ -806,7 +808,7 @@ int DiffusionProblem_StagePass(DiffusionProblem* P, int K, const double* coef, N
                                N_Vector yjm2, N_Vector yn, N_Vector fn, N_Vector outkm1,
                                N_Vector outk)
 {
-  if (!P || K < 1 || K > 4 || !coef || !vec_ok(P, yjm1) || !vec_ok(P, yjm2) || !vec_ok(P, yn) ||
+  if (!P || K < 1 || K > kMaxPassStages || !coef || !vec_ok(P, yjm1) || !vec_ok(P, yjm2) || !vec_ok(P, yn) ||
       !vec_ok(P, fn) || !vec_ok(P, outk) || (K >= 2 && !vec_ok(P, outkm1)))
     return -1;
   const double* ins[4] = {dptr(yjm1), dptr(yjm2), dptr(yn), dptr(fn)};
@@ -816,7 +818,7 @@ int DiffusionProblem_StagePass(DiffusionProblem* P, int K, const double* coef, N
     for (double* o : outs)
       if (o && o == x) return -1;
   hipStream_t s = P->exec->stream;
-  StageEntry h[4];
+  StageEntry h[kMaxPassStages];  // K <= kMaxPassStages (checked above)
   memset(h, 0, sizeof(h));
   for (int m = 0; m < K; m++) {
     for (int k = 0; k < 5; k++) h[m].c[k] = coef[8 * m + k];

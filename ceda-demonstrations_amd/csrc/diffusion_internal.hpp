@@ -84,6 +84,7 @@ int pass_halo_lanes(int K, int cpl);
 
 // ---- batched super-time-stepping (many independent problems, one launch
 // per pass index) ----
+constexpr int kMaxPassStages = 6;  // stages per multi-stage pass (one problem)
 constexpr int kBatchMaxK = 3;  // stages per pass the batch tile geometry covers
 struct BatchDesc {             // per problem, static
   const double *fx, *fy, *sx, *sy, *cx, *cy;

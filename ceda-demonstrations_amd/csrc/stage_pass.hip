@@ -114,7 +114,7 @@ template <int CPL, bool PER, bool MMS, int K, int L, bool FUSED, bool NT>
 __device__ __forceinline__ void pass_wave(const PassGeo& g, const StageEntry* __restrict__ e,
                                           int strip, int rb)
 {
-  static_assert(K >= 1 && K <= 4, "1..4 stages per pass");
+  static_assert(K >= 1 && K <= kMaxPassStages, "1..6 stages per pass");
   static_assert(L * CPL >= K - 1, "halo lanes must cover the shrinking levels");
   static_assert(FUSED || K == 1, "the plain RHS is a single level");
   constexpr int OW = (kWave - 2 * L) * CPL;
@@ -361,7 +361,9 @@ void launch_pass_t(const PassGeo& g, int K, bool nt, const StageEntry* e, hipStr
     case 1: CEDA_PASS(1) break;
     case 2: CEDA_PASS(2) break;
     case 3: CEDA_PASS(3) break;
-    default: CEDA_PASS(4) break;
+    case 4: CEDA_PASS(4) break;
+    case 5: CEDA_PASS(5) break;
+    default: CEDA_PASS(6) break;
   }
 #undef CEDA_PASS
 }
@@ -507,6 +509,9 @@ int batch_halo_lanes(int cpl) { return cpl == 2 ? batch_halo<2>() : batch_halo<1
 
 void launch_pass(DiffusionProblem* P, int K, int rows, int nt, const StageEntry* e, hipStream_t s)
 {
+  // the kernel reads e[0..K-1]; callers validate K, this keeps a bad one
+  // from reaching the device
+  if (K < 1 || K > kMaxPassStages || !e || rows < 1) return;
   const int cpl = (P->p.nx % 2 == 0) ? 2 : 1;
   const PassGeo g = geo_of(P, cpl, pass_halo_lanes(K, cpl), rows);
   const bool per = (P->p.kind == DIFFUSION_CEDA_PERIODIC);

@@ -56,7 +56,7 @@ int DiffusionProblem_SetStencilRows(DiffusionProblem* prob, int rows);
    variant 0 = row-interleaved tiles (rows per wavefront), variant 1 =
    y-marching (rows = consecutive rows per wavefront, nt = non-temporal
    streamed operands).  which = 2 is the multi-stage pass of the fused path:
-   variant = stages per pass K (2..4; 0 = one kernel per stage), rows = output
+   variant = stages per pass K (2..6; 0 = one kernel per stage), rows = output
    rows per wavefront tile (1..1024), nt as above.  Every shape computes
    bitwise-identical results. */
 int DiffusionProblem_SetStencilConfig(DiffusionProblem* prob, int which, int variant, int rows,
@@ -78,7 +78,7 @@ int DiffusionProblem_Autotune(DiffusionProblem* prob, int reps);
 int DiffusionProblem_TimeKernel(DiffusionProblem* prob, int which, N_Vector u, N_Vector a,
                                 N_Vector b, N_Vector c, N_Vector out, int reps,
                                 double* avg_ms);
-/* [ext] K consecutive STS stages (K = 1..4) in one pass of the multi-stage
+/* [ext] K consecutive STS stages (K = 1..6) in one pass of the multi-stage
    kernel, with the configured rows/nt of which = 2: stage m (0-based) uses
    coef[8m .. 8m+4] as {c0..c4} and coef[8m+5], coef[8m+6] as the MMS time
    factors (T1, T2) of its RHS; reads yjm1 = Y_{j-1}, yjm2 = Y_{j-2}, yn, fn;

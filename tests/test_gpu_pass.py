@@ -57,7 +57,7 @@ SHAPES = [(0, 300, 77), (0, 129, 40), (1, 256, 130), (1, 77, 65), (1, 2, 2), (0,
           (1, 5, 3), (0, 130, 7), (1, 500, 9), (0, 64, 1), (1, 1024, 300)]
 
 
-@pytest.mark.parametrize("K", [1, 2, 3, 4])
+@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("kind,nx,ny", SHAPES)
 def test_stage_pass_bitwise(pk, ctx, K, kind, nx, ny):
     kw = dict(nx=nx, ny=ny, kx=1.3, ky=0.6, inhomogeneous=1 if kind == 0 else 0)
@@ -90,7 +90,7 @@ def test_stage_pass_rejects_aliasing(pk, ctx):
     coef = np.zeros((3, 8))
     assert G.stage_pass(3, coef, V[0], V[1], V[2], V[3], V[0], V[4]) == -1
     assert G.stage_pass(3, coef, V[0], V[1], V[2], V[3], V[4], V[4]) == -1
-    assert G.stage_pass(5, coef, V[0], V[1], V[2], V[3], V[4], V[1]) == -1
+    assert G.stage_pass(7, coef, V[0], V[1], V[2], V[3], V[4], V[1]) == -1
 
 
 def fused_run(pk, ctx, kind, method, kw, cfg, graphs, tout, internal):
@@ -117,7 +117,8 @@ def test_fused_path_with_passes_is_bitwise(pk, ctx, method, kind, nx, ny, intern
     kw = dict(nx=nx, ny=ny, kx=1.0, ky=0.5)
     base = fused_run(pk, ctx, kind, method, kw, (0, 32, 0), 1, 0.05, internal)
     assert base[0] == 0
-    for cfg, graphs in [((2, 16, 0), 1), ((3, 8, 1), 1), ((4, 32, 0), 0), ((3, 200, 0), 1)]:
+    for cfg, graphs in [((2, 16, 0), 1), ((3, 8, 1), 1), ((4, 32, 0), 0), ((3, 200, 0), 1), ((5, 16, 1), 1),
+                        ((6, 24, 0), 1)]:
         got = fused_run(pk, ctx, kind, method, kw, cfg, graphs, 0.05, internal)
         assert got[0] == 0 and got[1] == base[1]
         for f in ("nst", "attempts", "nfe", "netf", "stage_max"):
