@@ -364,6 +364,166 @@ def implicit_workload(pk, args):
     print(json.dumps(out))
 
 
+def c5_problems(n=64, seed=20260102):
+    """The batched implicit configuration: square grids with side uniform in
+    [256, 1024], kx and ky log-uniform in [0.1, 10]."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        side = int(rng.integers(256, 1025))
+        kx, ky = np.exp(rng.uniform(np.log(0.1), np.log(10.0), 2))
+        out.append(dict(nx=side, ny=side, kx=float(kx), ky=float(ky)))
+    return out
+
+
+def c5_workload(pk, args):
+    """Many implicit problems at once: 64 manufactured problems of different
+    sizes and coefficients, each ARKStep DIRK3 with fixed dt = 1e-3 and linear
+    solves to relative residual 1e-8 (equal accuracy for every method),
+    integrated to t = steps*dt.  The problems run as independent integrators
+    driven concurrently by `--c5-workers` host threads, each with its own
+    SUNContext (HIP stream); a worker integrates its problems one after
+    another.  Methods: Jacobi-PCG (the reference's configuration), FGMRES
+    with one learned network shared by every problem (trained once on the GPU
+    at the batch's median gamma*dt*k/dx^2, outside the timed region; its
+    output scaled per problem by the inverse Jacobi diagonal) and PCG with the
+    multigrid V-cycle.  Time-to-solution is the wall time of the whole batch,
+    problem setup included; Jacobi-PCG is also run by a single worker."""
+    import threading
+    import numpy as np
+
+    probs = c5_problems(args.c5_n, 20260102)
+    dt, steps = 1e-3, args.c5_steps
+    gamma_a = 1767732205903.0 / 4055673282236.0
+    ratios = [gamma_a * dt * 0.5 * (p["kx"] + p["ky"]) * (p["nx"] + 1) ** 2 for p in probs]
+    cells = sum(p["nx"] * p["ny"] for p in probs)
+    log(f"c5: {len(probs)} problems, {cells} cells, sides {min(p['nx'] for p in probs)}..."
+        f"{max(p['nx'] for p in probs)}, gamma*dt*k/dx^2 {min(ratios):.0f}..{max(ratios):.0f}, {steps} steps")
+    shared = None
+    train = {}
+    if "learned" in args.c5_methods:
+        ctx = pk.Context()
+        ratio = float(np.median(ratios))
+        t0 = time.perf_counter()
+        R, E, iters, rc = pk.training_set(ctx, args.nsamp, args.train_n, ratio, reltol=1e-12, seed=1234)
+        assert rc == 0
+        net = pk.LearnedPrec(ctx, (1, 1, 1, 1))
+        net.init_identity(8e-4, 0)
+        net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
+        hist = net.train(args.nsamp, args.train_n, ratio, R, E, args.train_steps, args.train_batch,
+                         args.train_lr, 0)
+        hip_sync()
+        shared = net.params()
+        train = dict(ratio=ratio, pairs=args.nsamp, grid=args.train_n, steps=args.train_steps,
+                     seconds=time.perf_counter() - t0, loss_first20=float(hist[:20].mean()),
+                     loss_last50=float(hist[-50:].mean()))
+        log(f"shared network trained at ratio {ratio:.0f}: {train}")
+        ctx.close()
+
+    def solve_one(ctx, net, k, kind):
+        p = probs[k]
+        G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=p["nx"], ny=p["ny"], kx=p["kx"], ky=p["ky"])
+        ls, prec, maxl = "cg", True, 20000
+        if kind == "mg":
+            prec = "mg"
+        elif kind == "learned":
+            dx2 = 1.0 / (p["nx"] + 1) ** 2
+            net.set_output_scale(1.0 / (1.0 + 2.0 * gamma_a * dt * (p["kx"] + p["ky"]) / dx2))
+            net.attach(G)
+            ls, prec, maxl = "fgmres", "learned", args.imaxl
+        I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=maxl, precond=prec)
+        if ls == "fgmres":
+            pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, args.mrestart)
+        I.analytic_jtimes()
+        I.ARKodeSetOrder(3)
+        I.ARKodeSetFixedStep(dt)
+        I.ARKodeSetLSRelativeTolerance(1e-8)
+        I.ARKodeSetMaxNumSteps(1000000)
+        flag, t = I.evolve(dt * steps)
+        assert flag == 0, (kind, k, flag)
+        st = I.stats()
+        y = I.y.get()
+        ex = G.exact(t).get()
+        rec = dict(lin_iters=int(st["nli"]), fallbacks=I.fallbacks(),
+                   l2_err=float(np.sqrt(np.mean((y - ex) ** 2))))
+        I.free()
+        G.destroy()
+        return rec
+
+    def run(kind, workers):
+        todo = list(range(len(probs)))
+        lock = threading.Lock()
+        recs = [None] * len(probs)
+        errors = []
+
+        def worker():
+            ctx = pk.Context()
+            net = None
+            if kind == "learned":
+                net = pk.LearnedPrec(ctx, (1, 1, 1, 1))
+                net.set_params(shared)
+            try:
+                while True:
+                    with lock:
+                        if not todo or errors:
+                            break
+                        k = todo.pop(0)
+                    recs[k] = solve_one(ctx, net, k, kind)
+                    with lock:
+                        done = sum(r is not None for r in recs)
+                    if done % 8 == 0:
+                        log(f"  {kind} x{workers}: {done}/{len(probs)} problems")
+            except Exception as e:  # reported after the join
+                with lock:
+                    errors.append(repr(e))
+            finally:
+                ctx.close()
+
+        hip_sync()
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=worker) for _ in range(workers)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        hip_sync()
+        el = time.perf_counter() - t0
+        assert not errors, errors
+        errs = np.array([r["l2_err"] for r in recs])
+        its = np.array([r["lin_iters"] for r in recs])
+        out = dict(seconds=el, workers=workers, solves_per_s=len(probs) / el,
+                   krylov_iters_total=int(its.sum()), krylov_iters_max=int(its.max()),
+                   fallbacks=int(sum(r["fallbacks"] for r in recs)),
+                   l2_err_max=float(errs.max()), l2_err_median=float(np.median(errs)))
+        log(f"{kind} x{workers}: {out}")
+        return out, errs
+
+    res = {}
+    errs = {}
+    for kind in args.c5_methods.split(","):
+        res[kind], errs[kind] = run(kind, args.c5_workers)
+    if "jacobi" in res:
+        res["jacobi_one_worker"], _ = run("jacobi", 1)
+    out = {"metric": "time-to-solution at fixed L2 err, batched implicit: shared learned preconditioner "
+                     "(FGMRES) and multigrid (PCG) vs Jacobi-PCG",
+           "config": {"workload": f"{len(probs)} mms-dirichlet problems, side U[256,1024], kx,ky logU[0.1,10] "
+                                  f"(seed 20260102), {cells} cells, DIRK3 fixed dt=1e-3, {steps} steps, "
+                                  "LS reltol 1e-8, analytic J v",
+                      "concurrency": f"{args.c5_workers} host threads, one SUNContext/stream each"},
+           "results": res, "training": train}
+    if "jacobi" in res:
+        for kind in res:
+            if kind != "jacobi":
+                out[f"speedup_{kind}_vs_jacobi"] = res["jacobi"]["seconds"] / res[kind]["seconds"]
+        for kind in errs:
+            if kind != "jacobi":
+                out[f"max_rel_l2_err_change_{kind}_vs_jacobi"] = float(
+                    np.max(np.abs(errs[kind] - errs["jacobi"]) / errs["jacobi"]))
+    print(json.dumps(out))
+
+
 def cpu_baseline():
     """One adaptive RKL2 step of the same workload on the CPU oracle (serial)."""
     import oracle_lib as O
@@ -395,7 +555,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
     ap.add_argument("--pin-pass", default=None, metavar="K,ROWS,NT",
                     help="with --kernels-only: launch the multi-stage pass at this shape")
-    ap.add_argument("--workload", default="sts", choices=["sts", "batch", "implicit"],
+    ap.add_argument("--workload", default="sts", choices=["sts", "batch", "implicit", "c5"],
                     help="sts: the headline 4096^2 RKL2 line; batch: the 256-problem batched "
                          "comparison; implicit: learned-preconditioned FGMRES vs Jacobi-PCG")
     ap.add_argument("--iside", type=int, default=2048)
@@ -410,6 +570,10 @@ def main():
     ap.add_argument("--mrestart", type=int, default=30)
     ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")
     ap.add_argument("--nprob", type=int, default=256)
+    ap.add_argument("--c5-n", type=int, default=64, help="c5: number of problems")
+    ap.add_argument("--c5-steps", type=int, default=3, help="c5: DIRK3 steps per problem")
+    ap.add_argument("--c5-workers", type=int, default=8, help="c5: concurrent host threads")
+    ap.add_argument("--c5-methods", default="jacobi,mg,learned")
     ap.add_argument("--tout", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -428,6 +592,9 @@ def main():
         return
     if args.workload == "implicit":
         implicit_workload(pk, args)
+        return
+    if args.workload == "c5":
+        c5_workload(pk, args)
         return
     ctx = pk.Context()
     G, I = make_integrator(pk, ctx, args.fastpath)

@@ -448,14 +448,14 @@ void ensure_table(ArkMem* A, int need)
   if (need <= A->tab_cap) return;
   if (A->d_tab) {
     CEDA_HIP_CHECK(hipStreamSynchronize(A->fast->exec->stream));
-    CEDA_HIP_CHECK(hipFree(A->d_tab));
-    CEDA_HIP_CHECK(hipHostFree(A->h_tab));
-    for (auto& g : A->graphs) CEDA_HIP_CHECK(hipGraphExecDestroy(g.second));
+    CEDA_HIP_CHECK(ceda::dev_free(A->d_tab));
+    CEDA_HIP_CHECK(ceda::host_free(A->h_tab));
+    for (auto& g : A->graphs) CEDA_HIP_CHECK(ceda::graph_exec_destroy(g.second));
     A->graphs.clear();
   }
   A->tab_cap = need;
-  CEDA_HIP_CHECK(hipMalloc(&A->d_tab, sizeof(StageEntry) * need));
-  CEDA_HIP_CHECK(hipHostMalloc(&A->h_tab, sizeof(StageEntry) * need, hipHostMallocDefault));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&A->d_tab, sizeof(StageEntry) * need));
+  CEDA_HIP_CHECK(ceda::host_malloc(&A->h_tab, sizeof(StageEntry) * need, hipHostMallocDefault));
   memset(A->h_tab, 0, sizeof(StageEntry) * need);
 }
 
@@ -554,12 +554,15 @@ int stages_fused(ArkMem* A, const StagePlan& SP)
     auto it = A->graphs.find(key);
     if (it == A->graphs.end()) {
       hipGraph_t g;
-      CEDA_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      issue_stage_kernels(A, s, pc, st);
-      CEDA_HIP_CHECK(hipStreamEndCapture(st, &g));
       hipGraphExec_t ge;
-      CEDA_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-      CEDA_HIP_CHECK(hipGraphDestroy(g));
+      {  // no device-wide synchronisation from other threads meanwhile
+        std::lock_guard<std::recursive_mutex> lk(ceda::device_sync_mutex());
+        CEDA_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        issue_stage_kernels(A, s, pc, st);
+        CEDA_HIP_CHECK(hipStreamEndCapture(st, &g));
+        CEDA_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        CEDA_HIP_CHECK(hipGraphDestroy(g));
+      }
       it = A->graphs.emplace(key, ge).first;
     }
     CEDA_HIP_CHECK(hipGraphLaunch(it->second, st));
@@ -1069,10 +1072,10 @@ void ARKodeFree(void** mem)
     if (v) N_VDestroy(v);
   if (A->d_tab) {
     CEDA_HIP_CHECK(hipStreamSynchronize(A->fast->exec->stream));
-    CEDA_HIP_CHECK(hipFree(A->d_tab));
-    CEDA_HIP_CHECK(hipHostFree(A->h_tab));
+    CEDA_HIP_CHECK(ceda::dev_free(A->d_tab));
+    CEDA_HIP_CHECK(ceda::host_free(A->h_tab));
   }
-  for (auto& g : A->graphs) CEDA_HIP_CHECK(hipGraphExecDestroy(g.second));
+  for (auto& g : A->graphs) CEDA_HIP_CHECK(ceda::graph_exec_destroy(g.second));
   if (A->tab_ev) CEDA_HIP_CHECK(hipEventDestroy(A->tab_ev));
   if (A->fbLS) SUNLinSolFree(A->fbLS);
   delete A;

@@ -76,22 +76,22 @@ namespace {
 template <class T>
 void dev_alloc(T** d, size_t n)
 {
-  CEDA_HIP_CHECK(hipMalloc(d, sizeof(T) * (n ? n : 1)));
+  CEDA_HIP_CHECK(ceda::dev_malloc(d, sizeof(T) * (n ? n : 1)));
 }
 template <class T>
 void host_alloc(T** h, size_t n)
 {
-  CEDA_HIP_CHECK(hipHostMalloc(h, sizeof(T) * (n ? n : 1), hipHostMallocDefault));
+  CEDA_HIP_CHECK(ceda::host_malloc(h, sizeof(T) * (n ? n : 1), hipHostMallocDefault));
 }
 
 void free_staging(DiffusionBatch* B)
 {
   void* ds[] = {B->d_ent, B->d_tiles, B->d_err, B->d_s, B->d_tb, B->d_tc, B->d_part, B->d_dsq};
   for (void* p : ds)
-    if (p) CEDA_HIP_CHECK(hipFree(p));
+    if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
   void* hs[] = {B->h_ent, B->h_tiles, B->h_err, B->h_s, B->h_tb, B->h_tc, B->h_dsq};
   for (void* p : hs)
-    if (p) CEDA_HIP_CHECK(hipHostFree(p));
+    if (p) CEDA_HIP_CHECK(ceda::host_free(p));
   B->d_ent = B->h_ent = nullptr;
   B->d_tiles = B->h_tiles = nullptr;
   B->d_err = B->h_err = nullptr;
@@ -152,7 +152,7 @@ void upload_desc(DiffusionBatch* B)
                      P->p.kind == DIFFUSION_CEDA_PERIODIC ? 1 : 0, b.cpl, b.rows, b.strips};
   }
   if (!B->d_desc) dev_alloc(&B->d_desc, h.size());
-  CEDA_HIP_CHECK(hipMemcpy(B->d_desc, h.data(), sizeof(BatchDesc) * h.size(), hipMemcpyHostToDevice));
+  CEDA_HIP_CHECK(ceda::copy_sync(B->d_desc, h.data(), sizeof(BatchDesc) * h.size(), hipMemcpyHostToDevice));
 }
 
 // the tile autotuner: for every size class (nx, ny) time a full-length pass
@@ -381,7 +381,7 @@ void DiffusionBatch_Destroy(DiffusionBatch* B)
   if (!B) return;
   CEDA_HIP_CHECK(hipStreamSynchronize(B->ex->stream));
   free_staging(B);
-  if (B->d_desc) CEDA_HIP_CHECK(hipFree(B->d_desc));
+  if (B->d_desc) CEDA_HIP_CHECK(ceda::dev_free(B->d_desc));
   for (BProb& b : B->probs) {
     if (b.A) {
       void* m = b.A;

@@ -394,9 +394,9 @@ double coeff(double k, int inhomogeneous, double s)
 double* upload(const std::vector<double>& h)
 {
   double* d = nullptr;
-  CEDA_HIP_CHECK(hipMalloc(&d, sizeof(double) * (h.empty() ? 1 : h.size())));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&d, sizeof(double) * (h.empty() ? 1 : h.size())));
   if (!h.empty())
-    CEDA_HIP_CHECK(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    CEDA_HIP_CHECK(ceda::copy_sync(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   return d;
 }
 
@@ -560,8 +560,8 @@ DiffusionProblem* DiffusionProblem_Create(const DiffusionParams* prm, SUNContext
   P->d_sx = upload(P->sx); P->d_sy = upload(P->sy);
   P->d_cx = upload(P->cx); P->d_cy = upload(P->cy);
   P->d_ax = upload(P->ax); P->d_by = upload(P->by);
-  CEDA_HIP_CHECK(hipMalloc(&P->d_tf, sizeof(double) * 8));
-  CEDA_HIP_CHECK(hipHostMalloc(&P->h_tf, sizeof(double) * 8, hipHostMallocDefault));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&P->d_tf, sizeof(double) * 8));
+  CEDA_HIP_CHECK(ceda::host_malloc(&P->h_tf, sizeof(double) * 8, hipHostMallocDefault));
   P->diag = N_VNew_HipNative(P->n, ctx);
   N_VConst(1.0, P->diag);
   P->cfg[0] = StencilCfg{0, 8, 0};
@@ -579,8 +579,8 @@ void DiffusionProblem_Destroy(DiffusionProblem* P)
   double* ds[] = {P->d_fx, P->d_fy, P->d_px, P->d_py, P->d_sx, P->d_sy, P->d_cx, P->d_cy,
                   P->d_ax, P->d_by, P->d_tf};
   for (double* d : ds)
-    if (d) CEDA_HIP_CHECK(hipFree(d));
-  if (P->h_tf) CEDA_HIP_CHECK(hipHostFree(P->h_tf));
+    if (d) CEDA_HIP_CHECK(ceda::dev_free(d));
+  if (P->h_tf) CEDA_HIP_CHECK(ceda::host_free(P->h_tf));
   delete P;
 }
 
@@ -651,7 +651,7 @@ StageEntry* pass_table(double* const* ops, int K, hipStream_t s)
   }
   if (K >= 2) h[K - 2].out = ops[4];
   StageEntry* d = nullptr;
-  CEDA_HIP_CHECK(hipMalloc(&d, sizeof(h)));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&d, sizeof(h)));
   CEDA_HIP_CHECK(hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, s));
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
   return d;
@@ -676,12 +676,12 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
   hipStream_t s = P->exec->stream;
   double* buf[6];
   for (int k = 0; k < 6; k++) {
-    CEDA_HIP_CHECK(hipMalloc(&buf[k], sizeof(double) * P->n));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&buf[k], sizeof(double) * P->n));
     CEDA_HIP_CHECK(hipMemsetAsync(buf[k], 0, sizeof(double) * P->n, s));
   }
   double* coef = nullptr;
   const double hc[8] = {1.0e-9, -0.5, 0.25, 1.25, -1.0e-10, 0.0, 1.0, 0.0};
-  CEDA_HIP_CHECK(hipMalloc(&coef, sizeof(hc)));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&coef, sizeof(hc)));
   CEDA_HIP_CHECK(hipMemcpyAsync(coef, hc, sizeof(hc), hipMemcpyHostToDevice, s));
   hipEvent_t e0, e1;
   CEDA_HIP_CHECK(hipEventCreate(&e0));
@@ -741,14 +741,14 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
         }
       }
     CEDA_HIP_CHECK(hipStreamSynchronize(s));
-    CEDA_HIP_CHECK(hipFree(dtab));
+    CEDA_HIP_CHECK(ceda::dev_free(dtab));
   }
   P->cfg[2] = bestp;
   CEDA_HIP_CHECK(hipEventDestroy(e0));
   CEDA_HIP_CHECK(hipEventDestroy(e1));
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
-  for (int k = 0; k < 6; k++) CEDA_HIP_CHECK(hipFree(buf[k]));
-  CEDA_HIP_CHECK(hipFree(coef));
+  for (int k = 0; k < 6; k++) CEDA_HIP_CHECK(ceda::dev_free(buf[k]));
+  CEDA_HIP_CHECK(ceda::dev_free(coef));
   P->tuned = true;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[key] = {P->cfg[0], P->cfg[1], P->cfg[2]};
@@ -766,8 +766,8 @@ int DiffusionProblem_TimeKernel(DiffusionProblem* P, int which, N_Vector u, N_Ve
   // stage coefficients of a mid-loop RKL2 stage; the values do not change
   // the traffic, only keep the result finite
   const double hc[8] = {1.0e-9, -0.5, 0.25, 1.25, -1.0e-10, 0.0, 1.0, 0.0};
-  CEDA_HIP_CHECK(hipMalloc(&coef, sizeof(hc)));
-  CEDA_HIP_CHECK(hipMemcpy(coef, hc, sizeof(hc), hipMemcpyHostToDevice));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&coef, sizeof(hc)));
+  CEDA_HIP_CHECK(ceda::copy_sync(coef, hc, sizeof(hc), hipMemcpyHostToDevice));
   hipEvent_t e0, e1;
   CEDA_HIP_CHECK(hipEventCreate(&e0));
   CEDA_HIP_CHECK(hipEventCreate(&e1));
@@ -775,8 +775,8 @@ int DiffusionProblem_TimeKernel(DiffusionProblem* P, int which, N_Vector u, N_Ve
   double* mid = nullptr;
   StageEntry* dtab = nullptr;
   if (which == 2) {
-    CEDA_HIP_CHECK(hipMalloc(&mid, sizeof(double) * P->n));
-    CEDA_HIP_CHECK(hipMemset(mid, 0, sizeof(double) * P->n));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&mid, sizeof(double) * P->n));
+    CEDA_HIP_CHECK(ceda::set_sync(mid, 0, sizeof(double) * P->n));
     double* ops[6] = {dptr(u), dptr(a), dptr(b), dptr(c), mid, dptr(out)};
     dtab = pass_table(ops, P->cfg[2].variant, s);
   }
@@ -795,12 +795,12 @@ int DiffusionProblem_TimeKernel(DiffusionProblem* P, int which, N_Vector u, N_Ve
   *avg_ms = (double)ms / reps;
   if (mid) {
     CEDA_HIP_CHECK(hipStreamSynchronize(s));
-    CEDA_HIP_CHECK(hipFree(mid));
-    CEDA_HIP_CHECK(hipFree(dtab));
+    CEDA_HIP_CHECK(ceda::dev_free(mid));
+    CEDA_HIP_CHECK(ceda::dev_free(dtab));
   }
   CEDA_HIP_CHECK(hipEventDestroy(e0));
   CEDA_HIP_CHECK(hipEventDestroy(e1));
-  CEDA_HIP_CHECK(hipFree(coef));
+  CEDA_HIP_CHECK(ceda::dev_free(coef));
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -832,12 +832,12 @@ int DiffusionProblem_StagePass(DiffusionProblem* P, int K, const double* coef, N
   }
   if (K >= 2) h[K - 2].out = outs[0];
   StageEntry* d = nullptr;
-  CEDA_HIP_CHECK(hipMalloc(&d, sizeof(h)));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&d, sizeof(h)));
   CEDA_HIP_CHECK(hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, s));
   launch_pass(P, K, P->cfg[2].rows, P->cfg[2].nt, d, s);
   CEDA_HIP_CHECK(hipGetLastError());
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
-  CEDA_HIP_CHECK(hipFree(d));
+  CEDA_HIP_CHECK(ceda::dev_free(d));
   return 0;
 }
 

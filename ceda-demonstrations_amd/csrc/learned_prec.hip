@@ -591,37 +591,37 @@ void lp_workspace(LearnedPrec* L, int nb, int64_t npix)
     CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
     for (int l = 0; l < kMaxL; l++)
       if (L->act[l]) {
-        CEDA_HIP_CHECK(hipFree(L->act[l]));
+        CEDA_HIP_CHECK(ceda::dev_free(L->act[l]));
         L->act[l] = nullptr;
       }
     void* ps[] = {L->gA, L->gB, L->u, L->dd, L->ad, L->gy};
     for (void* p : ps)
-      if (p) CEDA_HIP_CHECK(hipFree(p));
-    for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(hipMalloc(&L->act[l], sizeof(unsigned short) * kC * tot));
-    CEDA_HIP_CHECK(hipMalloc(&L->gA, sizeof(unsigned short) * kC * tot));
-    CEDA_HIP_CHECK(hipMalloc(&L->gB, sizeof(unsigned short) * kC * tot));
-    CEDA_HIP_CHECK(hipMalloc(&L->u, sizeof(double) * tot));
-    CEDA_HIP_CHECK(hipMalloc(&L->dd, sizeof(double) * tot));
-    CEDA_HIP_CHECK(hipMalloc(&L->ad, sizeof(double) * tot));
-    CEDA_HIP_CHECK(hipMalloc(&L->gy, sizeof(double) * tot));
+      if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
+    for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(ceda::dev_malloc(&L->act[l], sizeof(unsigned short) * kC * tot));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gA, sizeof(unsigned short) * kC * tot));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gB, sizeof(unsigned short) * kC * tot));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->u, sizeof(double) * tot));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->dd, sizeof(double) * tot));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->ad, sizeof(double) * tot));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gy, sizeof(double) * tot));
     L->cap_pix = tot;
   }
   if (nb > L->cap_nb || nb * P * 2 > L->part_cap) {
     CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
     void* ps[] = {L->sc, L->isc, L->part, L->lossb, L->coef};
     for (void* p : ps)
-      if (p) CEDA_HIP_CHECK(hipFree(p));
+      if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
     const int cnb = std::max(nb, L->cap_nb);
     const int pc = std::max(cnb * 64 * 2, L->part_cap);
-    CEDA_HIP_CHECK(hipMalloc(&L->sc, sizeof(double) * cnb));
-    CEDA_HIP_CHECK(hipMalloc(&L->isc, sizeof(double) * cnb));
-    CEDA_HIP_CHECK(hipMalloc(&L->part, sizeof(double) * pc));
-    CEDA_HIP_CHECK(hipMalloc(&L->lossb, sizeof(double) * cnb));
-    CEDA_HIP_CHECK(hipMalloc(&L->coef, sizeof(double) * cnb));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->sc, sizeof(double) * cnb));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->isc, sizeof(double) * cnb));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->part, sizeof(double) * pc));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->lossb, sizeof(double) * cnb));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->coef, sizeof(double) * cnb));
     L->cap_nb = cnb;
     L->part_cap = pc;
   }
-  if (!L->wpart) CEDA_HIP_CHECK(hipMalloc(&L->wpart, sizeof(float) * kPW * kWGradWaves));
+  if (!L->wpart) CEDA_HIP_CHECK(ceda::dev_malloc(&L->wpart, sizeof(float) * kPW * kWGradWaves));
 }
 
 int rms_parts(int64_t npix) { return (int)std::min<int64_t>(64, (npix + 4095) / 4096); }
@@ -769,17 +769,17 @@ LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx)
     off += L->cout[l];
   }
   L->np = off;
-  CEDA_HIP_CHECK(hipMalloc(&L->d_p, sizeof(float) * off));
-  CEDA_HIP_CHECK(hipMalloc(&L->d_g, sizeof(float) * off));
-  CEDA_HIP_CHECK(hipMalloc(&L->d_m, sizeof(float) * off));
-  CEDA_HIP_CHECK(hipMalloc(&L->d_v, sizeof(float) * off));
-  CEDA_HIP_CHECK(hipMemset(L->d_p, 0, sizeof(float) * off));
-  CEDA_HIP_CHECK(hipMemset(L->d_m, 0, sizeof(float) * off));
-  CEDA_HIP_CHECK(hipMemset(L->d_v, 0, sizeof(float) * off));
-  CEDA_HIP_CHECK(hipMalloc(&L->d_afw, sizeof(unsigned short) * 9 * 256 * nlayer));
-  CEDA_HIP_CHECK(hipMalloc(&L->d_abw, sizeof(unsigned short) * 9 * 256 * nlayer));
-  CEDA_HIP_CHECK(hipMemset(L->d_afw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
-  CEDA_HIP_CHECK(hipMemset(L->d_abw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_p, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_g, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_m, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_v, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_p, 0, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_m, 0, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_v, 0, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_afw, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_abw, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_afw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_abw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
   lp_pack(L);
   CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
   return L;
@@ -792,9 +792,9 @@ void LearnedPrec_Destroy(LearnedPrec* L)
   void* ps[] = {L->d_p, L->d_g, L->d_m, L->d_v, L->d_afw, L->d_abw, L->gA, L->gB, L->u, L->dd,
                 L->ad, L->gy, L->sc, L->isc, L->part, L->lossb, L->coef, L->wpart};
   for (void* p : ps)
-    if (p) CEDA_HIP_CHECK(hipFree(p));
+    if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
   for (int l = 0; l < kMaxL; l++)
-    if (L->act[l]) CEDA_HIP_CHECK(hipFree(L->act[l]));
+    if (L->act[l]) CEDA_HIP_CHECK(ceda::dev_free(L->act[l]));
   delete L;
 }
 
@@ -870,13 +870,13 @@ int LearnedPrec_LossGrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, c
 {
   if (!L || nb < 1 || nx < 1 || ny < 1 || !r || !e || !loss) return -1;
   double* dl = nullptr;
-  CEDA_HIP_CHECK(hipMalloc(&dl, sizeof(double)));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&dl, sizeof(double)));
   lp_lossgrad(L, nb, nx, ny, ratio, r, e, dl);
   CEDA_HIP_CHECK(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, L->ex->stream));
   if (grad)
     CEDA_HIP_CHECK(hipMemcpyAsync(grad, L->d_g, sizeof(float) * L->np, hipMemcpyDeviceToHost, L->ex->stream));
   CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
-  CEDA_HIP_CHECK(hipFree(dl));
+  CEDA_HIP_CHECK(ceda::dev_free(dl));
   return 0;
 }
 
@@ -891,11 +891,11 @@ int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, u
     for (int k = 0; k < n; k++) S[(size_t)x * n + k] = sin(M_PI * (double)(k + 1) * (double)(x + 1) / (double)(n + 1));
   double *dS = nullptr, *tmp = nullptr, *p = nullptr, *ap = nullptr;
   CGState* st = nullptr;
-  CEDA_HIP_CHECK(hipMalloc(&dS, sizeof(double) * n * n));
-  CEDA_HIP_CHECK(hipMalloc(&tmp, sizeof(double) * tot));
-  CEDA_HIP_CHECK(hipMalloc(&p, sizeof(double) * tot));
-  CEDA_HIP_CHECK(hipMalloc(&ap, sizeof(double) * tot));
-  CEDA_HIP_CHECK(hipMalloc(&st, sizeof(CGState) * nsamp));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&dS, sizeof(double) * n * n));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&tmp, sizeof(double) * tot));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&p, sizeof(double) * tot));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&ap, sizeof(double) * tot));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&st, sizeof(CGState) * nsamp));
   CEDA_HIP_CHECK(hipMemcpyAsync(dS, S.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
   const unsigned gg = grid_for(tot);
   // fields: even samples smooth sine sums, odd samples (I - ratio L) applied to
@@ -929,7 +929,7 @@ int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, u
   CEDA_HIP_CHECK(hipGetLastError());
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
   void* ps[] = {dS, tmp, p, ap, st};
-  for (void* q : ps) CEDA_HIP_CHECK(hipFree(q));
+  for (void* q : ps) CEDA_HIP_CHECK(ceda::dev_free(q));
   return all ? 0 : 1;
 }
 
@@ -941,10 +941,10 @@ int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const doub
   const int64_t npix = (int64_t)n * n;
   double *rb = nullptr, *eb = nullptr, *dloss = nullptr;
   int* didx = nullptr;
-  CEDA_HIP_CHECK(hipMalloc(&rb, sizeof(double) * npix * batch));
-  CEDA_HIP_CHECK(hipMalloc(&eb, sizeof(double) * npix * batch));
-  CEDA_HIP_CHECK(hipMalloc(&dloss, sizeof(double) * steps));
-  CEDA_HIP_CHECK(hipMalloc(&didx, sizeof(int) * batch * steps));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&rb, sizeof(double) * npix * batch));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&eb, sizeof(double) * npix * batch));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&dloss, sizeof(double) * steps));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&didx, sizeof(int) * batch * steps));
   std::vector<int> idx((size_t)batch * steps);
   std::mt19937_64 rng(seed);
   std::uniform_int_distribution<int> U(0, nsamp - 1);
@@ -961,7 +961,7 @@ int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const doub
     CEDA_HIP_CHECK(hipMemcpyAsync(loss_hist, dloss, sizeof(double) * steps, hipMemcpyDeviceToHost, s));
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
   void* ps[] = {rb, eb, dloss, didx};
-  for (void* q : ps) CEDA_HIP_CHECK(hipFree(q));
+  for (void* q : ps) CEDA_HIP_CHECK(ceda::dev_free(q));
   return 0;
 }
 

@@ -198,13 +198,13 @@ MGHierarchy* build(DiffusionProblem* P)
     MGLevel& L = H->L[l];
     L.nx = nx;
     L.ny = ny;
-    CEDA_HIP_CHECK(hipMalloc(&L.fx, sizeof(double) * (nx + 1)));
-    CEDA_HIP_CHECK(hipMalloc(&L.fy, sizeof(double) * (ny + 1)));
-    CEDA_HIP_CHECK(hipMemcpy(L.fx, fx.data(), sizeof(double) * (nx + 1), hipMemcpyHostToDevice));
-    CEDA_HIP_CHECK(hipMemcpy(L.fy, fy.data(), sizeof(double) * (ny + 1), hipMemcpyHostToDevice));
-    CEDA_HIP_CHECK(hipMalloc(&L.x, sizeof(double) * nx * ny));
-    CEDA_HIP_CHECK(hipMalloc(&L.b, sizeof(double) * nx * ny));
-    CEDA_HIP_CHECK(hipMalloc(&L.r, sizeof(double) * nx * ny));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L.fx, sizeof(double) * (nx + 1)));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L.fy, sizeof(double) * (ny + 1)));
+    CEDA_HIP_CHECK(ceda::copy_sync(L.fx, fx.data(), sizeof(double) * (nx + 1), hipMemcpyHostToDevice));
+    CEDA_HIP_CHECK(ceda::copy_sync(L.fy, fy.data(), sizeof(double) * (ny + 1), hipMemcpyHostToDevice));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L.x, sizeof(double) * nx * ny));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L.b, sizeof(double) * nx * ny));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L.r, sizeof(double) * nx * ny));
     l++;
     const int64_t cx = nx / 2, cy = ny / 2;
     if (cx < 3 || cy < 3 || l == kMGMaxL) break;
@@ -217,8 +217,8 @@ MGHierarchy* build(DiffusionProblem* P)
     ny = cy;
   }
   H->nl = l;
-  CEDA_HIP_CHECK(hipMalloc(&H->gamma, sizeof(double)));
-  CEDA_HIP_CHECK(hipHostMalloc(&H->h_gamma, sizeof(double), hipHostMallocDefault));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&H->gamma, sizeof(double)));
+  CEDA_HIP_CHECK(ceda::host_malloc(&H->h_gamma, sizeof(double), hipHostMallocDefault));
   return H;
 }
 
@@ -231,11 +231,11 @@ void mg_destroy(MGHierarchy* H)
   for (int l = 0; l < H->nl; l++) {
     double* ps[] = {H->L[l].fx, H->L[l].fy, H->L[l].x, H->L[l].b, H->L[l].r};
     for (double* p : ps)
-      if (p) CEDA_HIP_CHECK(hipFree(p));
+      if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
   }
-  if (H->gamma) CEDA_HIP_CHECK(hipFree(H->gamma));
-  if (H->h_gamma) CEDA_HIP_CHECK(hipHostFree(H->h_gamma));
-  if (H->graph) CEDA_HIP_CHECK(hipGraphExecDestroy(H->graph));
+  if (H->gamma) CEDA_HIP_CHECK(ceda::dev_free(H->gamma));
+  if (H->h_gamma) CEDA_HIP_CHECK(ceda::host_free(H->h_gamma));
+  if (H->graph) CEDA_HIP_CHECK(ceda::graph_exec_destroy(H->graph));
   delete H;
 }
 }  // namespace ceda
@@ -254,13 +254,15 @@ int MGPSetup(sunrealtype, N_Vector, N_Vector, sunbooleantype, sunbooleantype* jc
   *H->h_gamma = gamma;
   CEDA_HIP_CHECK(hipMemcpyAsync(H->gamma, H->h_gamma, sizeof(double), hipMemcpyHostToDevice, s));
   if (H->graph && (H->relax != P->mg_relax || H->nrelax != P->mg_nrelax)) {
-    CEDA_HIP_CHECK(hipGraphExecDestroy(H->graph));
+    CEDA_HIP_CHECK(ceda::graph_exec_destroy(H->graph));
     H->graph = nullptr;
   }
   if (!H->graph) {
     H->relax = P->mg_relax;
     H->nrelax = P->mg_nrelax;
     hipGraph_t g;
+    // no device-wide synchronisation from other threads meanwhile
+    std::lock_guard<std::recursive_mutex> lk(ceda::device_sync_mutex());
     CEDA_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     vcycle(H, 0, s);
     CEDA_HIP_CHECK(hipStreamEndCapture(s, &g));

@@ -612,7 +612,7 @@ N_Vector N_VNew_HipNative(sunindextype length, SUNContext sunctx)
   N_Vector v = N_VNewEmpty_HipNative(length, sunctx);
   if (!v) return nullptr;
   NVContent* c = content(v);
-  if (length > 0) CEDA_HIP_CHECK(hipMalloc(&c->dev, sizeof(double) * (size_t)length));
+  if (length > 0) CEDA_HIP_CHECK(ceda::dev_malloc(&c->dev, sizeof(double) * (size_t)length));
   c->own_data = true;
   return v;
 }
@@ -649,7 +649,7 @@ N_Vector N_VClone_HipNative(N_Vector w)
   N_Vector v = N_VCloneEmpty_HipNative(w);
   if (!v) return nullptr;
   NVContent* c = content(v);
-  if (c->length > 0) CEDA_HIP_CHECK(hipMalloc(&c->dev, sizeof(double) * (size_t)c->length));
+  if (c->length > 0) CEDA_HIP_CHECK(ceda::dev_malloc(&c->dev, sizeof(double) * (size_t)c->length));
   c->own_data = true;
   return v;
 }
@@ -661,9 +661,9 @@ void N_VDestroy_HipNative(N_Vector v)
   if (c) {
     if (c->own_data && c->dev) {
       CEDA_HIP_CHECK(hipStreamSynchronize(c->exec->stream));
-      CEDA_HIP_CHECK(hipFree(c->dev));
+      CEDA_HIP_CHECK(ceda::dev_free(c->dev));
     }
-    if (c->host) CEDA_HIP_CHECK(hipHostFree(c->host));
+    if (c->host) CEDA_HIP_CHECK(ceda::host_free(c->host));
     delete c;
     v->content = nullptr;
   }
@@ -680,7 +680,7 @@ sunrealtype* N_VGetArrayPointer_HipNative(N_Vector v)
 {
   NVContent* c = content(v);
   if (!c->host && c->length > 0)
-    CEDA_HIP_CHECK(hipHostMalloc(&c->host, sizeof(double) * (size_t)c->length, hipHostMallocDefault));
+    CEDA_HIP_CHECK(ceda::host_malloc(&c->host, sizeof(double) * (size_t)c->length, hipHostMallocDefault));
   return c->host;
 }
 
@@ -689,7 +689,7 @@ sunrealtype* N_VGetDeviceArrayPointer_HipNative(N_Vector v) { return dptr(v); }
 void N_VSetArrayPointer_HipNative(sunrealtype* d_data, N_Vector v)
 {
   NVContent* c = content(v);
-  if (c->own_data && c->dev) CEDA_HIP_CHECK(hipFree(c->dev));
+  if (c->own_data && c->dev) CEDA_HIP_CHECK(ceda::dev_free(c->dev));
   c->dev = d_data;
   c->own_data = false;
 }

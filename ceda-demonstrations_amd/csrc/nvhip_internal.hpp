@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 
 #include "ceda_sundials.h"
 #include "nvector_hipnative.h"
@@ -22,6 +23,51 @@
   } while (0)
 
 namespace ceda {
+
+// HIP rejects a call that may synchronise the whole device (an allocation or
+// free, a synchronous copy or set on the null stream, a stream destruction)
+// while any thread's stream is being captured into a graph
+// (hipErrorStreamCaptureImplicit).  Every graph capture of this library and
+// every such call hold this lock, so that integrators on separate contexts
+// can be driven from separate host threads.
+std::recursive_mutex& device_sync_mutex();
+template <class T>
+inline hipError_t dev_malloc(T** p, size_t bytes)
+{
+  std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
+  return hipMalloc(p, bytes);
+}
+inline hipError_t dev_free(void* p)
+{
+  std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
+  return hipFree(p);
+}
+template <class T>
+inline hipError_t host_malloc(T** p, size_t bytes, unsigned flags)
+{
+  std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
+  return hipHostMalloc(p, bytes, flags);
+}
+inline hipError_t host_free(void* p)
+{
+  std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
+  return hipHostFree(p);
+}
+inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+  std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
+  return hipMemcpy(dst, src, bytes, kind);
+}
+inline hipError_t set_sync(void* dst, int value, size_t bytes)
+{
+  std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
+  return hipMemset(dst, value, bytes);
+}
+inline hipError_t graph_exec_destroy(hipGraphExec_t ge)
+{
+  std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
+  return hipGraphExecDestroy(ge);
+}
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wavefronts)
 constexpr int kWave = 64;            // gfx950 wavefront width

@@ -19,6 +19,11 @@ struct SUNContext_ {
 
 namespace ceda {
 Exec* context_exec(SUNContext ctx) { return &ctx->exec; }
+std::recursive_mutex& device_sync_mutex()
+{
+  static std::recursive_mutex mu;
+  return mu;
+}
 }  // namespace ceda
 
 extern "C" {
@@ -27,11 +32,14 @@ SUNErrCode SUNContext_Create(SUNComm, SUNContext* ctx)
 {
   if (!ctx) return -1;
   SUNContext c = new SUNContext_();
-  CEDA_HIP_CHECK(hipStreamCreateWithFlags(&c->exec.stream, hipStreamNonBlocking));
-  CEDA_HIP_CHECK(hipMalloc(&c->exec.d_partials,
+  {
+    std::lock_guard<std::recursive_mutex> g(ceda::device_sync_mutex());
+    CEDA_HIP_CHECK(hipStreamCreateWithFlags(&c->exec.stream, hipStreamNonBlocking));
+  }
+  CEDA_HIP_CHECK(ceda::dev_malloc(&c->exec.d_partials,
                            sizeof(double) * ceda::kMaxReduceBlocks * ceda::kMaxAcc));
-  CEDA_HIP_CHECK(hipMalloc(&c->exec.d_result, sizeof(double) * ceda::kMaxAcc));
-  CEDA_HIP_CHECK(hipHostMalloc(&c->exec.h_result, sizeof(double) * ceda::kMaxAcc,
+  CEDA_HIP_CHECK(ceda::dev_malloc(&c->exec.d_result, sizeof(double) * ceda::kMaxAcc));
+  CEDA_HIP_CHECK(ceda::host_malloc(&c->exec.h_result, sizeof(double) * ceda::kMaxAcc,
                                hipHostMallocDefault));
   int dev = 0;
   CEDA_HIP_CHECK(hipGetDevice(&dev));
@@ -47,10 +55,13 @@ SUNErrCode SUNContext_Free(SUNContext* ctx)
   if (!ctx || !*ctx) return 0;
   SUNContext c = *ctx;
   CEDA_HIP_CHECK(hipStreamSynchronize(c->exec.stream));
-  CEDA_HIP_CHECK(hipFree(c->exec.d_partials));
-  CEDA_HIP_CHECK(hipFree(c->exec.d_result));
-  CEDA_HIP_CHECK(hipHostFree(c->exec.h_result));
-  CEDA_HIP_CHECK(hipStreamDestroy(c->exec.stream));
+  CEDA_HIP_CHECK(ceda::dev_free(c->exec.d_partials));
+  CEDA_HIP_CHECK(ceda::dev_free(c->exec.d_result));
+  CEDA_HIP_CHECK(ceda::host_free(c->exec.h_result));
+  {
+    std::lock_guard<std::recursive_mutex> g(ceda::device_sync_mutex());
+    CEDA_HIP_CHECK(hipStreamDestroy(c->exec.stream));
+  }
   delete c;
   *ctx = nullptr;
   return 0;
