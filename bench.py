@@ -48,15 +48,17 @@ def hip_sync():
     C.CDLL("libamdhip64.so").hipDeviceSynchronize()
 
 
-def read_traffic(name):
-    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+def read_traffic(name, shape):
+    """HBM bytes per launch of a kernel at one launch shape (K, rows, nt) from
+    its committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated), or None when that shape was not measured."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
+    key = "traffic K=%d rows=%d nt=%d " % shape
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
-            return float(line.split("=")[1])
+        if line.startswith(key):
+            return float(line.split("=")[-1])
     return None
 
 
@@ -95,14 +97,14 @@ def roofline(pk, ctx, G):
         bytes_alg = 48.0 * G.n
         kname = (f"k_pass<2,false,true,{K},{'true' if nt else 'false'}> rows={rows} "
                  f"({K} fused STS stages per launch)")
-        traffic = read_traffic(f"stage_pass_k{K}_4096_pmc.txt")
+        traffic = read_traffic("stage_pass_4096_pmc.txt", (K, rows, nt))
     else:
         launch_ms = stage_ms
         bytes_alg = 40.0 * G.n
         v, r, n1 = G.get_config(1)
         kname = (f"k_stencil<2,false,true,true> rows={r}" if v == 0 else
                  f"k_march<2,false,true,true,{'true' if n1 else 'false'}> rows={r}") + " (fused STS stage)"
-        traffic = read_traffic("fused_stage_4096_pmc.txt")
+        traffic = read_traffic("fused_stage_4096_pmc.txt", (1, r, n1)) if v == 1 else None
     for v in vs:
         v.destroy()
     achieved = bytes_alg / (launch_ms * 1e-3) / 1e9
