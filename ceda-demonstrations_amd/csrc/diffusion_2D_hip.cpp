@@ -11,7 +11,7 @@
 // plot_data_2d.py:271-586) and the final lines that
 // runtests-diffusion2d.py:36-54 parses.  The hypre build's --pfmg_relax and
 // --pfmg_nrelax (diffusion_2D.cpp:137-151) select the GPU multigrid
-// preconditioner in place of Jacobi (manufactured problem only) and print
+// preconditioner in place of Jacobi (periodic grids: even sides) and print
 // "Total PFMG iterations" (main.cpp:489-492).  Added flags:
 //   --problem ceda|mms   manufactured Dirichlet problem of README.md:11-32
 //   --deefreq N --stagemax N --eigsafety X   LSRKStep settings
@@ -120,9 +120,11 @@ int main(int argc, char** argv)
     fprintf(stderr, "ERROR: problem setup\n");
     return 1;
   }
-  if (pfmg && (kind != DIFFUSION_MMS_DIRICHLET || DiffusionProblem_SetMGOptions(udata, pfmg_relax, pfmg_nrelax))) {
-    fprintf(stderr, "ERROR: the multigrid preconditioner needs --problem mms, --pfmg_relax 2|3 and "
-                    "--pfmg_nrelax 0..16\n");
+  const bool odd = (prm.nx % 2 != 0) || (prm.ny % 2 != 0);
+  if (pfmg && ((kind == DIFFUSION_CEDA_PERIODIC && odd) ||
+               DiffusionProblem_SetMGOptions(udata, pfmg_relax, pfmg_nrelax))) {
+    fprintf(stderr, "ERROR: the multigrid preconditioner needs even --nx and --ny for the periodic "
+                    "problem, --pfmg_relax 2|3 and --pfmg_nrelax 0..16\n");
     return 1;
   }
   int64_t nx, ny;

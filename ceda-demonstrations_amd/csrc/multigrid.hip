@@ -8,9 +8,15 @@
 // and after the coarse correction) and the sweep count are the reference's
 // --pfmg_relax / --pfmg_nrelax options (diffusion_2D.cpp:137-151).
 //
-// Hierarchy: vertex-centred coarsening n -> n/2 (coarse node I on fine node
-// 2I+1) until a side would drop below 3; coarse operators rediscretised from
-// the fine face tables (coarse face I on fine node 2I: (f[2I] + f[2I+1]) / 8);
+// Hierarchy (Dirichlet, the manufactured problem): vertex-centred coarsening
+// n -> n/2 (coarse node I on fine node 2I+1) until a side would drop below 3;
+// coarse operators rediscretised from the fine face tables (coarse face I on
+// fine node 2I: (f[2I] + f[2I+1]) / 8).  Periodic (the reference's problem,
+// even sides only): coarse node I on fine node 2I, coarse face I spanning
+// fine faces 2I-1 and 2I ((f[2I-1] + f[2I]) / 8, f[-1] = f[n-1]), coarsening
+// while the coarse sides stay even and at least 4, every neighbour, transfer
+// and sweep wrapping around (red/black colours stay consistent across the
+// wrap because every side is even);
 // full-weighting restriction = P^T/4 of the bilinear prolongation; red then
 // black sweeps before the coarse correction, black then red after (the
 // adjoint), 4 symmetric pairs on the coarsest grid.  The cycle is therefore
@@ -37,6 +43,7 @@ struct MGLevel {
 struct MGHierarchy {
   int nl = 0;
   MGLevel L[24];
+  int per = 0;              // periodic grid
   double* gamma = nullptr;  // device scalar
   double* h_gamma = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -59,11 +66,21 @@ unsigned grid_of(int64_t n)
 
 __device__ inline void mg_row(int64_t nx, int64_t ny, const double* __restrict__ fx,
                               const double* __restrict__ fy, double g, const double* x, int64_t i,
-                              int64_t j, double& off, double& d)
+                              int64_t j, int per, double& off, double& d)
 {
   const int64_t k = j * nx + i;
-  const double xw = i > 0 ? x[k - 1] : 0.0, xe = i < nx - 1 ? x[k + 1] : 0.0;
-  const double xs = j > 0 ? x[k - nx] : 0.0, xn = j < ny - 1 ? x[k + nx] : 0.0;
+  double xw, xe, xs, xn;
+  if (per) {
+    xw = x[j * nx + (i > 0 ? i - 1 : nx - 1)];
+    xe = x[j * nx + (i < nx - 1 ? i + 1 : 0)];
+    xs = x[(j > 0 ? j - 1 : ny - 1) * nx + i];
+    xn = x[(j < ny - 1 ? j + 1 : 0) * nx + i];
+  } else {
+    xw = i > 0 ? x[k - 1] : 0.0;
+    xe = i < nx - 1 ? x[k + 1] : 0.0;
+    xs = j > 0 ? x[k - nx] : 0.0;
+    xn = j < ny - 1 ? x[k + nx] : 0.0;
+  }
   off = ((fx[i] * xw + fx[i + 1] * xe) + fy[j] * xs) + fy[j + 1] * xn;
   d = 1.0 + g * ((fx[i] + fx[i + 1]) + (fy[j] + fy[j + 1]));
 }
@@ -77,7 +94,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_fill0(int64_t n, double* __restri
 // one colour of a red/black Gauss-Seidel sweep (red: i + j even), in place
 __global__ __launch_bounds__(kBlock) void k_mg_sweep(int64_t nx, int64_t ny, const double* __restrict__ fx,
                                                      const double* __restrict__ fy, const double* __restrict__ gp,
-                                                     const double* __restrict__ b, double* x, int color)
+                                                     const double* __restrict__ b, double* x, int color,
+                                                     int per)
 {
   const double g = *gp;
   const int64_t half = (nx + 1) / 2;
@@ -87,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(int64_t nx, int64_t ny, con
     const int64_t i = 2 * (q - j * half) + ((j + color) & 1);
     if (i >= nx) continue;
     double off, d;
-    mg_row(nx, ny, fx, fy, g, x, i, j, off, d);
+    mg_row(nx, ny, fx, fy, g, x, i, j, per, off, d);
     x[j * nx + i] = (b[j * nx + i] + g * off) / d;
   }
 }
@@ -95,29 +113,33 @@ __global__ __launch_bounds__(kBlock) void k_mg_sweep(int64_t nx, int64_t ny, con
 __global__ __launch_bounds__(kBlock) void k_mg_resid(int64_t nx, int64_t ny, const double* __restrict__ fx,
                                                      const double* __restrict__ fy, const double* __restrict__ gp,
                                                      const double* __restrict__ b, const double* __restrict__ x,
-                                                     double* __restrict__ r)
+                                                     double* __restrict__ r, int per)
 {
   const double g = *gp;
   const int64_t n = nx * ny;
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
     const int64_t j = k / nx, i = k - j * nx;
     double off, d;
-    mg_row(nx, ny, fx, fy, g, x, i, j, off, d);
+    mg_row(nx, ny, fx, fy, g, x, i, j, per, off, d);
     double t = d * x[k];
     t = t - g * off;
     r[k] = b[k] - t;
   }
 }
 
-// full weighting onto coarse node (I, J) = fine node (2I+1, 2J+1)
+// full weighting onto coarse node (I, J) = fine node (2I+1, 2J+1), or
+// (2I, 2J) with wrap-around on a periodic grid
 __global__ __launch_bounds__(kBlock) void k_mg_restrict(int64_t nx, int64_t ny, const double* __restrict__ r,
-                                                        int64_t cx, int64_t cy, double* __restrict__ bc)
+                                                        int64_t cx, int64_t cy, double* __restrict__ bc, int per)
 {
   const int64_t n = cx * cy;
   for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock) {
     const int64_t J = q / cx, I = q - J * cx;
-    const int64_t i = 2 * I + 1, j = 2 * J + 1;
-    auto R = [&](int64_t a, int64_t c) -> double { return (a < nx && c < ny) ? r[c * nx + a] : 0.0; };
+    const int64_t i = per ? 2 * I : 2 * I + 1, j = per ? 2 * J : 2 * J + 1;
+    auto R = [&](int64_t a, int64_t c) -> double {
+      if (per) return r[((c + ny) % ny) * nx + ((a + nx) % nx)];
+      return (a < nx && c < ny) ? r[c * nx + a] : 0.0;
+    };
     double t = 4.0 * R(i, j);
     t = t + 2.0 * ((R(i - 1, j) + R(i + 1, j)) + (R(i, j - 1) + R(i, j + 1)));
     t = t + ((R(i - 1, j - 1) + R(i + 1, j - 1)) + (R(i - 1, j + 1) + R(i + 1, j + 1)));
@@ -125,22 +147,29 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(int64_t nx, int64_t ny, 
   }
 }
 
-// bilinear prolongation of the coarse correction, added to x
+// bilinear prolongation of the coarse correction, added to x: a fine index
+// coincides with coarse index I (Dirichlet: odd i, I = i/2; periodic: even
+// i, I = i/2) or lies between I and I+1 (Dirichlet: I = i/2 - 1, zero
+// outside; periodic: I = i/2, wrapping)
 __global__ __launch_bounds__(kBlock) void k_mg_prolong(int64_t nx, int64_t ny, double* __restrict__ x,
-                                                       int64_t cx, int64_t cy, const double* __restrict__ xc)
+                                                       int64_t cx, int64_t cy, const double* __restrict__ xc,
+                                                       int per)
 {
   const int64_t n = nx * ny;
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
     const int64_t j = k / nx, i = k - j * nx;
     auto XC = [&](int64_t a, int64_t c) -> double {
+      if (per) return xc[((c + cy) % cy) * cx + ((a + cx) % cx)];
       return (a >= 0 && a < cx && c >= 0 && c < cy) ? xc[c * cx + a] : 0.0;
     };
-    const int64_t I = i / 2, J = j / 2;
+    const bool cxo = per ? !(i & 1) : (i & 1), cyo = per ? !(j & 1) : (j & 1);
+    const int64_t I = (per || cxo) ? i / 2 : i / 2 - 1;
+    const int64_t J = (per || cyo) ? j / 2 : j / 2 - 1;
     double v;
-    if ((i & 1) && (j & 1)) v = XC(I, J);
-    else if (j & 1) v = 0.5 * (XC(I - 1, J) + XC(I, J));
-    else if (i & 1) v = 0.5 * (XC(I, J - 1) + XC(I, J));
-    else v = 0.25 * ((XC(I - 1, J - 1) + XC(I, J - 1)) + (XC(I - 1, J) + XC(I, J)));
+    if (cxo && cyo) v = XC(I, J);
+    else if (cyo) v = 0.5 * (XC(I, J) + XC(I + 1, J));
+    else if (cxo) v = 0.5 * (XC(I, J) + XC(I, J + 1));
+    else v = 0.25 * ((XC(I, J) + XC(I + 1, J)) + (XC(I, J + 1) + XC(I + 1, J + 1)));
     x[k] = x[k] + v;
   }
 }
@@ -149,7 +178,7 @@ void sweep(MGHierarchy* H, int l, int color, hipStream_t s)
 {
   const MGLevel& L = H->L[l];
   hipLaunchKernelGGL(k_mg_sweep, dim3(grid_of((L.nx + 1) / 2 * L.ny)), dim3(kBlock), 0, s, L.nx, L.ny,
-                     L.fx, L.fy, H->gamma, L.b, L.x, color);
+                     L.fx, L.fy, H->gamma, L.b, L.x, color, H->per);
 }
 
 void vcycle(MGHierarchy* H, int l, hipStream_t s)
@@ -176,12 +205,12 @@ void vcycle(MGHierarchy* H, int l, hipStream_t s)
   }
   const MGLevel& C = H->L[l + 1];
   hipLaunchKernelGGL(k_mg_resid, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, L.nx, L.ny, L.fx, L.fy,
-                     H->gamma, L.b, L.x, L.r);
+                     H->gamma, L.b, L.x, L.r, H->per);
   hipLaunchKernelGGL(k_mg_restrict, dim3(grid_of(C.nx * C.ny)), dim3(kBlock), 0, s, L.nx, L.ny, L.r, C.nx,
-                     C.ny, C.b);
+                     C.ny, C.b, H->per);
   vcycle(H, l + 1, s);
   hipLaunchKernelGGL(k_mg_prolong, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, L.nx, L.ny, L.x, C.nx,
-                     C.ny, C.x);
+                     C.ny, C.x, H->per);
   for (int k = 0; k < H->nrelax; k++) {
     sweep(H, l, p0, s);
     sweep(H, l, 1 - p0, s);
@@ -191,6 +220,8 @@ void vcycle(MGHierarchy* H, int l, hipStream_t s)
 MGHierarchy* build(DiffusionProblem* P)
 {
   MGHierarchy* H = new MGHierarchy();
+  const bool per = (P->p.kind == DIFFUSION_CEDA_PERIODIC);
+  H->per = per ? 1 : 0;
   std::vector<double> fx = P->fx, fy = P->fy;
   int64_t nx = P->p.nx, ny = P->p.ny;
   int l = 0;
@@ -207,10 +238,18 @@ MGHierarchy* build(DiffusionProblem* P)
     CEDA_HIP_CHECK(ceda::dev_malloc(&L.r, sizeof(double) * nx * ny));
     l++;
     const int64_t cx = nx / 2, cy = ny / 2;
-    if (cx < 3 || cy < 3 || l == kMGMaxL) break;
+    if (per ? (cx % 2 || cy % 2 || cx < 4 || cy < 4) : (cx < 3 || cy < 3)) break;
+    if (l == kMGMaxL) break;
     std::vector<double> cfx(cx + 1), cfy(cy + 1);
-    for (int64_t i = 0; i <= cx; i++) cfx[i] = (fx[2 * i < nx ? 2 * i : nx] + fx[2 * i + 1 < nx ? 2 * i + 1 : nx]) / 8.0;
-    for (int64_t j = 0; j <= cy; j++) cfy[j] = (fy[2 * j < ny ? 2 * j : ny] + fy[2 * j + 1 < ny ? 2 * j + 1 : ny]) / 8.0;
+    if (per) {
+      for (int64_t i = 0; i <= cx; i++) cfx[i] = (fx[i > 0 ? 2 * i - 1 : nx - 1] + fx[2 * i]) / 8.0;
+      for (int64_t j = 0; j <= cy; j++) cfy[j] = (fy[j > 0 ? 2 * j - 1 : ny - 1] + fy[2 * j]) / 8.0;
+    } else {
+      for (int64_t i = 0; i <= cx; i++)
+        cfx[i] = (fx[2 * i < nx ? 2 * i : nx] + fx[2 * i + 1 < nx ? 2 * i + 1 : nx]) / 8.0;
+      for (int64_t j = 0; j <= cy; j++)
+        cfy[j] = (fy[2 * j < ny ? 2 * j : ny] + fy[2 * j + 1 < ny ? 2 * j + 1 : ny]) / 8.0;
+    }
     fx.swap(cfx);
     fy.swap(cfy);
     nx = cx;
@@ -246,7 +285,8 @@ int MGPSetup(sunrealtype, N_Vector, N_Vector, sunbooleantype, sunbooleantype* jc
              void* user_data)
 {
   DiffusionProblem* P = static_cast<DiffusionProblem*>(user_data);
-  if (!P || P->p.kind != DIFFUSION_MMS_DIRICHLET) return -1;
+  // periodic grids need even sides (red/black colours consistent across the wrap)
+  if (!P || (P->p.kind == DIFFUSION_CEDA_PERIODIC && (P->p.nx % 2 || P->p.ny % 2))) return -1;
   hipStream_t s = P->exec->stream;
   if (!P->mg) P->mg = build(P);
   MGHierarchy* H = P->mg;

@@ -96,10 +96,12 @@ int laplacian(sunrealtype t, N_Vector u, N_Vector f, void* user_data);
    reference's hypre PFMG option (preconditioner_hypre.cpp:51-160, settings
    diffusion_2D.hpp:187-193: one V-cycle, zero initial guess, symmetric
    red/black Gauss-Seidel, 2 pre + 2 post sweeps), same signatures and user
-   data as PSetup/PSolve.  Manufactured (Dirichlet) problems only: -1 for
-   the periodic one.  Vertex-centred coarsening n -> n/2, rediscretised coarse
-   operators, full weighting, bilinear prolongation; the cycle is symmetric,
-   so it preconditions PCG. */
+   data as PSetup/PSolve.  Both problems: the manufactured Dirichlet one
+   (coarse node I = fine node 2I+1) and the reference's periodic one (even
+   sides, coarse node I = fine node 2I, everything wrapping; -1 for an odd
+   side).  Coarsening n -> n/2, rediscretised coarse operators, full
+   weighting, bilinear prolongation; with the default smoother the cycle is
+   symmetric for a symmetric operator, so it preconditions PCG. */
 int MGPSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok, sunbooleantype* jcurPtr,
              sunrealtype gamma, void* user_data);
 int MGPSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z, sunrealtype gamma,

@@ -11,8 +11,8 @@ call sequence of diffusion_2D/main.cpp on the GPU), run as a program:
 - --error reports the maximum relative error against the tight-tolerance
   reference run (main.cpp:387-465);
 - --pfmg_relax / --pfmg_nrelax (the hypre build's flags) switch the implicit
-  path to the multigrid preconditioner, reach the Jacobi-PCG solution and
-  print "Total PFMG iterations" (main.cpp:489-492)."""
+  path to the multigrid preconditioner on either problem, reach the
+  Jacobi-PCG solution and print "Total PFMG iterations" (main.cpp:489-492)."""
 import os
 import subprocess
 
@@ -118,10 +118,10 @@ def test_error_mode(tmp_path):
     assert acc is not None and 0.0 < acc < 1e-2
 
 
-def dirk(tmp_path, name, extra):
+def dirk(tmp_path, name, extra, problem="mms"):
     d = tmp_path / name
     d.mkdir()
-    out = run(["--problem", "mms", "--integrator", "dirk", "--order", "3", "--nx", "64", "--ny", "64",
+    out = run(["--problem", problem, "--integrator", "dirk", "--order", "3", "--nx", "64", "--ny", "64",
                "--fixedstep", "1e-3", "--tf", "0.005", "--nout", "1", "--ls_reltol", "1e-10",
                "--liniters", "2000", "--write_final", "final.bin"] + extra, d)
     return out, np.fromfile(d / "final.bin", dtype=np.float64)
@@ -131,11 +131,13 @@ def ls_iters(out):
     return int([ln for ln in out.split("\n") if ln.startswith("LS iters")][0].split()[-1])
 
 
-@pytest.mark.parametrize("relax,nrelax", [(3, 2), (2, 1)])
-def test_pfmg_flags_select_multigrid(tmp_path, relax, nrelax):
-    ref_out, ref = dirk(tmp_path, "jacobi", [])
+@pytest.mark.parametrize("relax,nrelax,problem", [(3, 2, "mms"), (2, 1, "mms"), (3, 2, "ceda")])
+def test_pfmg_flags_select_multigrid(tmp_path, relax, nrelax, problem):
+    """The hypre build's command lines (runtests-diffusion2d.py: DIRK with
+    --pfmg_relax 3 on the reference's periodic problem)."""
+    ref_out, ref = dirk(tmp_path, "jacobi", [], problem)
     assert "PFMG" not in ref_out
-    out, y = dirk(tmp_path, "mg", ["--pfmg_relax", str(relax), "--pfmg_nrelax", str(nrelax)])
+    out, y = dirk(tmp_path, "mg", ["--pfmg_relax", str(relax), "--pfmg_nrelax", str(nrelax)], problem)
     its = int([ln for ln in out.split("\n") if ln.startswith("Total PFMG iterations")][0].split()[-1])
     assert its > 0
     assert ls_iters(out) * 3 < ls_iters(ref_out)
@@ -144,6 +146,6 @@ def test_pfmg_flags_select_multigrid(tmp_path, relax, nrelax):
 
 
 def test_pfmg_flags_rejected_where_unsupported(tmp_path):
-    run(["--integrator", "dirk", "--nx", "16", "--ny", "16", "--pfmg_relax", "3"], tmp_path, expect=1)
+    run(["--integrator", "dirk", "--nx", "15", "--ny", "16", "--pfmg_relax", "3"], tmp_path, expect=1)
     run(["--problem", "mms", "--integrator", "dirk", "--nx", "16", "--ny", "16", "--pfmg_relax", "1"],
         tmp_path, expect=1)

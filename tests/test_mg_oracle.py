@@ -101,3 +101,59 @@ def test_single_level_nonsymmetric_is_red_black_gauss_seidel():
     for _ in range(8):
         sweep(red); sweep(black)
     np.testing.assert_allclose(z, x, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("nx,ny,kx,ky", [(24, 20, 1.0, 1.0), (32, 16, 1.0, 0.5), (16, 24, 2.0, 0.3)])
+@pytest.mark.parametrize("gamma", [1e-3, 2e-2])
+def test_periodic_vcycle_is_symmetric_and_contracts(nx, ny, kx, ky, gamma):
+    """The reference's periodic problem (wrap-around transfers and smoothing):
+    symmetric positive definite for the homogeneous (symmetric) operator and
+    a contraction of its error modes."""
+    P = O.OracleProblem(O.CEDA, nx=nx, ny=ny, kx=kx, ky=ky)
+    n = P.n
+    z, nl = P.mg_apply(gamma, np.ones(n))
+    assert nl >= 2
+    M = dense(lambda e: P.mg_apply(gamma, e)[0], n)
+    assert np.abs(M - M.T).max() <= 1e-12 * np.abs(M).max()
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0.0
+    A = dense(lambda e: e - gamma * P.rhs(0.0, e), n)
+    rho = np.abs(np.linalg.eigvals(np.eye(n) - M @ A)).max()
+    assert rho < (0.25 if kx == ky else 0.5), rho
+
+
+def test_periodic_inhomogeneous_vcycle_contracts():
+    """With --inhomogeneous the reference's periodic operator is not symmetric
+    (the wrap faces sit at different x); the cycle still contracts."""
+    P = O.OracleProblem(O.CEDA, nx=24, ny=20, kx=1.0, ky=1.0, inhomogeneous=1)
+    n, gamma = P.n, 1e-2
+    M = dense(lambda e: P.mg_apply(gamma, e)[0], n)
+    A = dense(lambda e: e - gamma * P.rhs(0.0, e), n)
+    assert np.abs(np.linalg.eigvals(np.eye(n) - M @ A)).max() < 0.3
+
+
+def test_periodic_single_level_is_symmetric_gauss_seidel():
+    """A periodic grid whose half side is odd is not coarsened: 4 red-black
+    pairs then 4 black-red pairs with wrap-around neighbours."""
+    P = O.OracleProblem(O.CEDA, nx=6, ny=10, kx=1.0, ky=0.7)
+    gamma = 0.05
+    r = np.random.default_rng(2).standard_normal(P.n)
+    z, nl = P.mg_apply(gamma, r)
+    assert nl == 1
+    n = P.n
+    A = dense(lambda e: e - gamma * P.rhs(0.0, e), n)
+    x = np.zeros(n)
+    red = [k for k in range(n) if ((k % 6) + (k // 6)) % 2 == 0]
+    black = [k for k in range(n) if k not in red]
+    def sweep(idx):
+        for k in idx:
+            x[k] = (r[k] - (A[k] @ x - A[k, k] * x[k])) / A[k, k]
+    for _ in range(4):
+        sweep(red); sweep(black)
+    for _ in range(4):
+        sweep(black); sweep(red)
+    np.testing.assert_allclose(z, x, rtol=1e-13, atol=1e-15)
+
+
+def test_periodic_odd_side_is_rejected():
+    P = O.OracleProblem(O.CEDA, nx=15, ny=16)
+    assert P.mg_apply(1e-2, np.ones(P.n))[1] == -1
