@@ -113,6 +113,9 @@ def roofline(pk, ctx, G):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch_algorithmic": bytes_alg, "launch_ms": round(launch_ms, 4),
             "stages_per_launch": max(K, 1),
+            # what K single-stage launches would have to stream (40 B/cell each)
+            # in the pass's time: the gain of temporal blocking, not HBM traffic
+            "single_stage_equivalent_GBs": round(40.0 * G.n * max(K, 1) / (launch_ms * 1e-3) / 1e9, 1),
             "single_stage_fused_ms": round(stage_ms, 4), "single_stage_fused_GBs": round(stage_gbs, 1),
             "rhs_stencil_GBs": round(stencil_gbs, 1)}
 
@@ -277,7 +280,8 @@ def implicit_workload(pk, args):
     hip_sync()
     t_data = time.perf_counter() - t0
     assert rc == 0
-    net = pk.LearnedPrec(ctx, (1, 1, 1, 1))
+    dil = tuple(int(x) for x in args.dil.split(","))
+    net = pk.LearnedPrec(ctx, dil)
     net.init_identity(8e-4, 0)
     net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
     t0 = time.perf_counter()
@@ -288,7 +292,7 @@ def implicit_workload(pk, args):
         f"{args.train_steps} Adam steps {t_train:.1f} s; loss {hist[:20].mean():.3f} -> {hist[-50:].mean():.3f}")
     R.destroy()
     E.destroy()
-    ident = pk.LearnedPrec(ctx, (1, 1, 1, 1))
+    ident = pk.LearnedPrec(ctx, dil)
     ident.init_identity(0.0, 0)
 
     def run(kind, mrestart=args.mrestart):
@@ -333,34 +337,35 @@ def implicit_workload(pk, args):
         log(f"{kind}: {out}")
         return out, y
 
-    jac, yj = run("jacobi")
-    mg, ym = run("mg")
-    lrn, yl = run("learned")
-    fcg, yf = run("learned-fcg")
-    if args.no_untrained:
-        idn = {"lin_iters": float("nan"), "s_per_step": float("nan")}
-    else:
-        idn, yi = run("identity")
-    rel = float(np.sqrt(np.mean((yl - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
-    relf = float(np.sqrt(np.mean((yf - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
-    relm = float(np.sqrt(np.mean((ym - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
+    methods = args.imethods.split(",")
+    if not args.no_untrained:
+        methods.append("identity")
+    res, ys = {}, {}
+    for kind in methods:
+        res[kind], ys[kind] = run(kind)
+    names = {"jacobi": "jacobi_pcg", "mg": "multigrid_pcg", "learned": "learned_fgmres",
+             "learned-fcg": "learned_flexible_pcg", "identity": "untrained_fgmres"}
     out = {"metric": "time-to-solution at fixed L2 err, learned-preconditioned FGMRES vs Jacobi-PCG",
            "config": f"mms-dirichlet {side}x{side} kx=ky=1 FP64, DIRK3 fixed dt=1e-3, LS reltol 1e-8",
-           "steps_timed": n_steps, "gamma_dt_over_dx2": ratio,
-           "jtimes": "analytic (stencil)" if args.jtimes else "difference quotient (as the reference)",
-           "jacobi_pcg": jac, "multigrid_pcg": mg, "learned_fgmres": lrn, "learned_flexible_pcg": fcg,
-           "untrained_fgmres": idn,
-           "speedup_multigrid_vs_jacobi_pcg": jac["s_per_step"] / mg["s_per_step"],
-           "rel_l2_diff_multigrid_vs_jacobi_pcg": relm,
-           "speedup_learned_vs_jacobi_pcg": jac["s_per_step"] / lrn["s_per_step"],
-           "speedup_learned_fcg_vs_jacobi_pcg": jac["s_per_step"] / fcg["s_per_step"],
-           "rel_l2_diff_learned_fcg_vs_jacobi_pcg": relf,
-           "krylov_iters_ratio_learned_vs_untrained": lrn["lin_iters"] / idn["lin_iters"],
-           "rel_l2_diff_learned_vs_jacobi_pcg": rel,
-           "training": {"pairs": args.nsamp, "grid": tn, "data_s": t_data, "train_s": t_train,
-                        "steps": args.train_steps, "batch": args.train_batch, "lr": args.train_lr,
-                        "loss_first20": float(hist[:20].mean()), "loss_last50": float(hist[-50:].mean())},
-           "fgmres_restart": args.mrestart}
+           "steps_timed": n_steps, "gamma_dt_over_dx2": ratio, "network_dilations": list(dil),
+           "jtimes": "analytic (stencil)" if args.jtimes else "difference quotient (as the reference)"}
+    for kind in methods:
+        out[names[kind]] = res[kind]
+    if "jacobi" in res:
+        yj = ys["jacobi"]
+        for kind in methods:
+            if kind == "jacobi":
+                continue
+            tag = names[kind].replace("_pcg", "").replace("_fgmres", "")
+            out[f"speedup_{names[kind]}_vs_jacobi_pcg"] = res["jacobi"]["s_per_step"] / res[kind]["s_per_step"]
+            out[f"rel_l2_diff_{names[kind]}_vs_jacobi_pcg"] = float(
+                np.sqrt(np.mean((ys[kind] - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
+    if "learned" in res and "identity" in res:
+        out["krylov_iters_ratio_learned_vs_untrained"] = res["learned"]["lin_iters"] / res["identity"]["lin_iters"]
+    out["training"] = {"pairs": args.nsamp, "grid": tn, "data_s": t_data, "train_s": t_train,
+                       "steps": args.train_steps, "batch": args.train_batch, "lr": args.train_lr,
+                       "loss_first20": float(hist[:20].mean()), "loss_last50": float(hist[-50:].mean())}
+    out["fgmres_restart"] = args.mrestart
     net.release()
     ident.release()
     print(json.dumps(out))
@@ -570,6 +575,9 @@ def main():
     ap.add_argument("--no-untrained", action="store_true", help="skip the untrained-network FGMRES run")
     ap.add_argument("--jtimes", action="store_true", help="analytic J v (stencil) instead of the difference quotient")
     ap.add_argument("--mrestart", type=int, default=30)
+    ap.add_argument("--dil", default="1,1,1,1", help="dilations of the network's 3x3 layers")
+    ap.add_argument("--imethods", default="jacobi,mg,learned,learned-fcg",
+                    help="implicit: methods to run (jacobi first)")
     ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--c5-n", type=int, default=64, help="c5: number of problems")

@@ -110,7 +110,12 @@ __device__ inline void gstore(gdouble* p, const double* v, int cpl)
 // strip and rb must be wavefront-uniform (scalar registers): the row indices
 // then stay scalar and the per-row coefficient tables come through the scalar
 // cache, so the only vector-memory waits are those of the row prefetch.
-template <int CPL, bool PER, bool MMS, int K, int L, bool FUSED, bool NT>
+//
+// SCALAR_E: e is wavefront-uniform (scalar registers).  The level's T1 is then
+// re-read into a scalar register every row step through an empty asm, so the
+// compiler cannot hoist the K x CPL products T1*sx out of the march: they
+// would hold 4K more VGPRs, which at K = 5 costs a wavefront per SIMD.
+template <int CPL, bool PER, bool MMS, int K, int L, bool FUSED, bool NT, bool SCALAR_E = false>
 __device__ __forceinline__ void pass_wave(const PassGeo& g, const StageEntry* __restrict__ e,
                                           int strip, int rb)
 {
@@ -263,8 +268,10 @@ __device__ __forceinline__ void pass_wave(const PassGeo& g, const StageEntry* __
           f[CPL - 1] = cell(De0, De1, Ds, Dn, C[CPL - 1], C[0], ev, S[CPL - 1], N[CPL - 1]);
         } else f[0] = cell(Dw0, De0, Ds, Dn, C[0], wv, ev, S[0], N[0]);
         if (MMS) {
-          f[0] = f[0] + forcing(T1[m - 1], T2[m - 1], sx0, cx0, syj, cyj);
-          if (CPL == 2) f[CPL - 1] = f[CPL - 1] + forcing(T1[m - 1], T2[m - 1], sx1, cx1, syj, cyj);
+          double t1 = T1[m - 1];
+          if (SCALAR_E) asm volatile("" : "+s"(t1));
+          f[0] = f[0] + forcing(t1, T2[m - 1], sx0, cx0, syj, cyj);
+          if (CPL == 2) f[CPL - 1] = f[CPL - 1] + forcing(t1, T2[m - 1], sx1, cx1, syj, cyj);
         }
         if (FUSED) {
           const double* Am = (m == 1) ? A1 : W[m >= 2 ? m - 2 : 0][0];
@@ -329,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassGeo g, const StageEnt
   const int64_t nrb = (g.ny + g.rows - 1) / g.rows;
   if (wt >= (int64_t)g.strips * nrb) return;
   const int strip = (int)(wt % g.strips), rb = (int)(wt / g.strips);
-  pass_wave<CPL, PER, MMS, K, L, true, NT>(g, e, strip, rb);
+  pass_wave<CPL, PER, MMS, K, L, true, NT, true>(g, e, strip, rb);
 }
 
 PassGeo geo_of(const DiffusionProblem* P, int cpl, int L, int rows)
