@@ -310,6 +310,8 @@ def implicit_workload(pk, args):
             pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
         if args.jtimes:
             I.analytic_jtimes()
+            if args.fused_pcg:
+                I.fused_pcg()
         I.ARKodeSetOrder(3)
         I.ARKodeSetFixedStep(dt)
         I.ARKodeSetLSRelativeTolerance(1e-8)
@@ -444,6 +446,8 @@ def c5_workload(pk, args):
         if ls == "fgmres":
             pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, args.mrestart)
         I.analytic_jtimes()
+        if args.fused_pcg:
+            I.fused_pcg()
         I.ARKodeSetOrder(3)
         I.ARKodeSetFixedStep(dt)
         I.ARKodeSetLSRelativeTolerance(1e-8)
@@ -576,6 +580,8 @@ def main():
     ap.add_argument("--jtimes", action="store_true", help="analytic J v (stencil) instead of the difference quotient")
     ap.add_argument("--mrestart", type=int, default=30)
     ap.add_argument("--dil", default="1,1,1,1", help="dilations of the network's 3x3 layers")
+    ap.add_argument("--fused-pcg", action="store_true",
+                    help="implicit/c5: PCG in three fused kernels per iteration (with the analytic J v)")
     ap.add_argument("--imethods", default="jacobi,mg,learned,learned-fcg",
                     help="implicit: methods to run (jacobi first)")
     ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")

@@ -110,6 +110,7 @@ _SIGS = {
     "LSRKStepSetNumDomEigEstInitPreprocessIters": (i32, [vp, i32]),
     "LSRKStepSetNumDomEigEstPreprocessIters": (i32, [vp, i32]),
     "LSRKStepSetDiffusionFastPath": (i32, [vp, vp, i32]),
+    "ARKodeSetDiffusionFastPCG": (i32, [vp, vp]),
     "LSRKStepGetSpectralRadius": (i32, [vp, dp]),
     "SUNDomEigEstimator_Power": (vp, [vp, C.c_long, d, vp]),
     "SUNDomEigEstimator_Destroy": (i32, [C.POINTER(vp)]),
@@ -533,6 +534,11 @@ class Integrator:
     def analytic_jtimes(self):
         """J v from the stencil (JTimes) instead of the difference quotient."""
         return lib().ARKodeSetJacTimes(self.h, None, fn_ptr("JTimes"))
+
+    def fused_pcg(self, on=True):
+        """PCG in three fused kernels per iteration (needs analytic_jtimes and
+        the Jacobi or multigrid preconditioner; otherwise the N_Vector PCG)."""
+        return lib().ARKodeSetDiffusionFastPCG(self.h, self.prob.h if on else None)
 
     def fallbacks(self):
         n = C.c_long()

@@ -77,6 +77,7 @@ struct ArkMem {
   int dee_init_pre = 0, dee_pre = 0;
   // fused path
   DiffusionProblem* fast = nullptr;
+  DiffusionProblem* fast_pcg = nullptr;  // fused PCG (ARKodeSetDiffusionFastPCG)
   int use_graphs = 0;
   StageEntry* h_tab = nullptr;
   StageEntry* d_tab = nullptr;
@@ -128,6 +129,10 @@ void lsrk_after(ArkMem* A, double dsm);
 // stage table of one attempt for passes of up to kp stages (kp = 1: one
 // kernel per stage)
 void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T, int kp);
+// PCG in three fused kernels per iteration (pcg_fused.hip); pmode 0 none,
+// 1 Jacobi (P->diag), 2 the preconditioner solve ps
+int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKLsPrecSolveFn ps, void* ud,
+              double t, double gamma, N_Vector b, N_Vector x);
 // stages per pass of a multi-stage configuration (1 when disabled)
 inline int pass_stages(const StencilCfg& c) { return c.variant >= 2 ? c.variant : 1; }
 void attempt_begin(ArkMem* A, double tout, AttemptState& S);

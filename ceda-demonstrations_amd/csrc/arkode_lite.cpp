@@ -745,6 +745,13 @@ int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double
 int pcg(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
 {
   const bool pre = A->psolve && A->LS->pretype != SUN_PREC_NONE;
+  // the fused kernels know J (the stencil), Jacobi and the multigrid cycle
+  // of the problem that is also the callbacks' user data
+  if (A->fast_pcg && A->jtimes == JTimes && A->LS->type == LS_PCG && A->ud == (void*)A->fast_pcg) {
+    const int pmode = !pre ? 0 : (A->psolve == PSolve ? 1 : (A->psolve == MGPSolve ? 2 : -1));
+    if (pmode >= 0)
+      return pcg_fused(A, A->LS, A->fast_pcg, pmode, A->psolve, A->ud, t, gamma, b, x);
+  }
   return pcg_with(A, A->LS, pre ? A->psolve : nullptr, A->ud, t, gamma, b, x, A->LS->type == LS_FPCG);
 }
 
@@ -1161,6 +1168,16 @@ int LSRKStepGetSpectralRadius(void* m, sunrealtype* rho)
 {
   if (!m) return ARK_MEM_NULL;
   *rho = AM(m)->rho;
+  return 0;
+}
+
+int ARKodeSetDiffusionFastPCG(void* m, DiffusionProblem* prob)
+{
+  if (!m) return ARK_MEM_NULL;
+  ArkMem* A = AM(m);
+  if (A->method != M_DIRK) return ARK_ILL_INPUT;
+  if (prob && prob->n != A->n) return ARK_ILL_INPUT;
+  A->fast_pcg = prob;
   return 0;
 }
 

@@ -105,6 +105,13 @@ int LSRKStepGetMaxNumStages(void* arkode_mem, int* stage_max);
 /* [ext] fused stencil+stage kernels for a DiffusionProblem RHS; bitwise equal
    to the N_Vector path; use_graphs replays each stage loop as a HIP graph */
 int LSRKStepSetDiffusionFastPath(void* arkode_mem, DiffusionProblem* prob, int use_graphs);
+/* [ext] PCG (SUNLinSol_PCG, main.cpp:305) in three fused kernels per
+   iteration for a DiffusionProblem that is also the user data, when J v is
+   the analytic JTimes and the preconditioner is PSolve, MGPSolve or none
+   (otherwise the N_Vector PCG runs).  Same algorithm and stopping test as
+   the N_Vector PCG, not bitwise equal to it (the dot products are summed in
+   another order).  prob = NULL switches it off. */
+int ARKodeSetDiffusionFastPCG(void* arkode_mem, DiffusionProblem* prob);
 
 SUNDomEigEstimator SUNDomEigEstimator_Power(N_Vector q, long max_iters, sunrealtype rel_tol,
                                             SUNContext sunctx);
