@@ -40,6 +40,13 @@ struct MGLevel {
   double *x = nullptr, *b = nullptr, *r = nullptr;
 };
 
+// a level as the kernels see it
+struct MGLevelD {
+  int64_t nx, ny;
+  const double *fx, *fy;
+  double *x, *b, *r;
+};
+
 struct MGHierarchy {
   int nl = 0;
   MGLevel L[24];
@@ -49,12 +56,16 @@ struct MGHierarchy {
   hipGraphExec_t graph = nullptr;
   int relax = 2, nrelax = 2;  // the options the graph was captured with
   long cycles = 0;            // V-cycles applied (MGPSolve calls)
+  int lc = 0;                 // first level of the single-workgroup coarse cycle
+  size_t lc_lds = 0;          // its LDS bytes
+  MGLevelD* d_levels = nullptr;
 };
 
 namespace {
 
 constexpr int kMGMaxL = 24;
 constexpr int kCoarse = 4;
+constexpr int64_t kMGCoarseCells = 4096;  // levels this small run in one workgroup
 
 unsigned grid_of(int64_t n)
 {
@@ -85,104 +96,201 @@ __device__ inline void mg_row(int64_t nx, int64_t ny, const double* __restrict__
   d = 1.0 + g * ((fx[i] + fx[i + 1]) + (fy[j] + fy[j + 1]));
 }
 
-__global__ __launch_bounds__(kBlock) void k_mg_fill0(int64_t n, double* __restrict__ x)
+// ---- per-point operations, shared by the per-level kernels and the
+// single-workgroup coarse cycle (same arithmetic, same order)
+
+// q-th point of one colour of a red/black Gauss-Seidel sweep (red: i + j
+// even), in place
+__device__ inline void pt_sweep(const MGLevelD& L, double g, int color, int per, int64_t q)
 {
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
-    x[k] = 0.0;
+  const int64_t half = (L.nx + 1) / 2;
+  const int64_t j = q / half;
+  const int64_t i = 2 * (q - j * half) + ((j + color) & 1);
+  if (i >= L.nx) return;
+  double off, d;
+  mg_row(L.nx, L.ny, L.fx, L.fy, g, L.x, i, j, per, off, d);
+  L.x[j * L.nx + i] = (L.b[j * L.nx + i] + g * off) / d;
 }
 
-// one colour of a red/black Gauss-Seidel sweep (red: i + j even), in place
-__global__ __launch_bounds__(kBlock) void k_mg_sweep(int64_t nx, int64_t ny, const double* __restrict__ fx,
-                                                     const double* __restrict__ fy, const double* __restrict__ gp,
-                                                     const double* __restrict__ b, double* x, int color,
+__device__ inline void pt_resid(const MGLevelD& L, double g, int per, int64_t k)
+{
+  const int64_t j = k / L.nx, i = k - j * L.nx;
+  double off, d;
+  mg_row(L.nx, L.ny, L.fx, L.fy, g, L.x, i, j, per, off, d);
+  double t = d * L.x[k];
+  t = t - g * off;
+  L.r[k] = L.b[k] - t;
+}
+
+// full weighting of F's residual onto coarse node q of C: (I, J) = fine node
+// (2I+1, 2J+1), or (2I, 2J) with wrap-around on a periodic grid
+__device__ inline void pt_restrict(const MGLevelD& F, const MGLevelD& C, int per, int64_t q)
+{
+  const int64_t nx = F.nx, ny = F.ny;
+  const double* __restrict__ r = F.r;
+  const int64_t J = q / C.nx, I = q - J * C.nx;
+  const int64_t i = per ? 2 * I : 2 * I + 1, j = per ? 2 * J : 2 * J + 1;
+  auto R = [&](int64_t a, int64_t c) -> double {
+    if (per) return r[((c + ny) % ny) * nx + ((a + nx) % nx)];
+    return (a < nx && c < ny) ? r[c * nx + a] : 0.0;
+  };
+  double t = 4.0 * R(i, j);
+  t = t + 2.0 * ((R(i - 1, j) + R(i + 1, j)) + (R(i, j - 1) + R(i, j + 1)));
+  t = t + ((R(i - 1, j - 1) + R(i + 1, j - 1)) + (R(i - 1, j + 1) + R(i + 1, j + 1)));
+  C.b[q] = t * 0.0625;
+}
+
+// bilinear prolongation of C's correction onto fine point k of F, added: a
+// fine index coincides with coarse index I (Dirichlet: odd i, I = i/2;
+// periodic: even i, I = i/2) or lies between I and I+1 (Dirichlet:
+// I = i/2 - 1, zero outside; periodic: I = i/2, wrapping)
+__device__ inline void pt_prolong(const MGLevelD& F, const MGLevelD& C, int per, int64_t k)
+{
+  const int64_t cx = C.nx, cy = C.ny;
+  const double* __restrict__ xc = C.x;
+  const int64_t j = k / F.nx, i = k - j * F.nx;
+  auto XC = [&](int64_t a, int64_t c) -> double {
+    if (per) return xc[((c + cy) % cy) * cx + ((a + cx) % cx)];
+    return (a >= 0 && a < cx && c >= 0 && c < cy) ? xc[c * cx + a] : 0.0;
+  };
+  const bool cxo = per ? !(i & 1) : (i & 1), cyo = per ? !(j & 1) : (j & 1);
+  const int64_t I = (per || cxo) ? i / 2 : i / 2 - 1;
+  const int64_t J = (per || cyo) ? j / 2 : j / 2 - 1;
+  double v;
+  if (cxo && cyo) v = XC(I, J);
+  else if (cyo) v = 0.5 * (XC(I, J) + XC(I + 1, J));
+  else if (cxo) v = 0.5 * (XC(I, J) + XC(I, J + 1));
+  else v = 0.25 * ((XC(I, J) + XC(I + 1, J)) + (XC(I, J + 1) + XC(I + 1, J + 1)));
+  F.x[k] = F.x[k] + v;
+}
+
+#define CEDA_MG_GRID_STRIDE(q, n) \
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < (n); q += (int64_t)gridDim.x * kBlock)
+
+__global__ __launch_bounds__(kBlock) void k_mg_fill0(int64_t n, double* __restrict__ x)
+{
+  CEDA_MG_GRID_STRIDE(k, n) x[k] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mg_sweep(const MGLevelD L, const double* __restrict__ gp, int color,
                                                      int per)
 {
   const double g = *gp;
-  const int64_t half = (nx + 1) / 2;
-  const int64_t n = half * ny;
-  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock) {
-    const int64_t j = q / half;
-    const int64_t i = 2 * (q - j * half) + ((j + color) & 1);
-    if (i >= nx) continue;
-    double off, d;
-    mg_row(nx, ny, fx, fy, g, x, i, j, per, off, d);
-    x[j * nx + i] = (b[j * nx + i] + g * off) / d;
-  }
+  CEDA_MG_GRID_STRIDE(q, (L.nx + 1) / 2 * L.ny) pt_sweep(L, g, color, per, q);
 }
 
-__global__ __launch_bounds__(kBlock) void k_mg_resid(int64_t nx, int64_t ny, const double* __restrict__ fx,
-                                                     const double* __restrict__ fy, const double* __restrict__ gp,
-                                                     const double* __restrict__ b, const double* __restrict__ x,
-                                                     double* __restrict__ r, int per)
+__global__ __launch_bounds__(kBlock) void k_mg_resid(const MGLevelD L, const double* __restrict__ gp, int per)
 {
   const double g = *gp;
-  const int64_t n = nx * ny;
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
-    const int64_t j = k / nx, i = k - j * nx;
-    double off, d;
-    mg_row(nx, ny, fx, fy, g, x, i, j, per, off, d);
-    double t = d * x[k];
-    t = t - g * off;
-    r[k] = b[k] - t;
-  }
+  CEDA_MG_GRID_STRIDE(k, L.nx * L.ny) pt_resid(L, g, per, k);
 }
 
-// full weighting onto coarse node (I, J) = fine node (2I+1, 2J+1), or
-// (2I, 2J) with wrap-around on a periodic grid
-__global__ __launch_bounds__(kBlock) void k_mg_restrict(int64_t nx, int64_t ny, const double* __restrict__ r,
-                                                        int64_t cx, int64_t cy, double* __restrict__ bc, int per)
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(const MGLevelD F, const MGLevelD C, int per)
 {
-  const int64_t n = cx * cy;
-  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < n; q += (int64_t)gridDim.x * kBlock) {
-    const int64_t J = q / cx, I = q - J * cx;
-    const int64_t i = per ? 2 * I : 2 * I + 1, j = per ? 2 * J : 2 * J + 1;
-    auto R = [&](int64_t a, int64_t c) -> double {
-      if (per) return r[((c + ny) % ny) * nx + ((a + nx) % nx)];
-      return (a < nx && c < ny) ? r[c * nx + a] : 0.0;
-    };
-    double t = 4.0 * R(i, j);
-    t = t + 2.0 * ((R(i - 1, j) + R(i + 1, j)) + (R(i, j - 1) + R(i, j + 1)));
-    t = t + ((R(i - 1, j - 1) + R(i + 1, j - 1)) + (R(i - 1, j + 1) + R(i + 1, j + 1)));
-    bc[q] = t * 0.0625;
-  }
+  CEDA_MG_GRID_STRIDE(q, C.nx * C.ny) pt_restrict(F, C, per, q);
 }
 
-// bilinear prolongation of the coarse correction, added to x: a fine index
-// coincides with coarse index I (Dirichlet: odd i, I = i/2; periodic: even
-// i, I = i/2) or lies between I and I+1 (Dirichlet: I = i/2 - 1, zero
-// outside; periodic: I = i/2, wrapping)
-__global__ __launch_bounds__(kBlock) void k_mg_prolong(int64_t nx, int64_t ny, double* __restrict__ x,
-                                                       int64_t cx, int64_t cy, const double* __restrict__ xc,
-                                                       int per)
+__global__ __launch_bounds__(kBlock) void k_mg_prolong(const MGLevelD F, const MGLevelD C, int per)
 {
-  const int64_t n = nx * ny;
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
-    const int64_t j = k / nx, i = k - j * nx;
-    auto XC = [&](int64_t a, int64_t c) -> double {
-      if (per) return xc[((c + cy) % cy) * cx + ((a + cx) % cx)];
-      return (a >= 0 && a < cx && c >= 0 && c < cy) ? xc[c * cx + a] : 0.0;
-    };
-    const bool cxo = per ? !(i & 1) : (i & 1), cyo = per ? !(j & 1) : (j & 1);
-    const int64_t I = (per || cxo) ? i / 2 : i / 2 - 1;
-    const int64_t J = (per || cyo) ? j / 2 : j / 2 - 1;
-    double v;
-    if (cxo && cyo) v = XC(I, J);
-    else if (cyo) v = 0.5 * (XC(I, J) + XC(I + 1, J));
-    else if (cxo) v = 0.5 * (XC(I, J) + XC(I, J + 1));
-    else v = 0.25 * ((XC(I, J) + XC(I + 1, J)) + (XC(I, J + 1) + XC(I + 1, J + 1)));
-    x[k] = x[k] + v;
-  }
+  CEDA_MG_GRID_STRIDE(k, F.nx * F.ny) pt_prolong(F, C, per, k);
 }
+
+// The V-cycle from level l0 down to the coarsest and back, in one workgroup:
+// the same operations in the same order as the per-level launches, separated
+// by workgroup barriers.  Every array of these levels (x, b, r and the face
+// tables) is staged in LDS, so each of the ~200 barrier-separated phases
+// waits for LDS rather than for an L2 round trip; b of level l0 comes in
+// from global memory (the restriction of the level above, or the
+// preconditioner's input) and x of level l0 goes back (for the prolongation
+// to the level above, or as the result).  Used for the trailing levels whose
+// arrays fit in a workgroup's LDS (mg_coarse_lds).
+constexpr int kCoarseThreads = 1024;
+__global__ __launch_bounds__(kCoarseThreads) void k_mg_coarse(const MGLevelD* __restrict__ Ls, int l0, int nl,
+                                                               const double* __restrict__ gp, int relax,
+                                                               int nrelax, int per)
+{
+  extern __shared__ double lds[];
+  const double g = *gp;
+  const int p0 = (relax == 2) ? 1 : 0;
+  auto each = [&](int64_t n, auto&& f) {
+    for (int64_t q = threadIdx.x; q < n; q += kCoarseThreads) f(q);
+    __syncthreads();
+  };
+  // descriptors of the LDS copies of levels l0..nl-1 (shared, not per lane)
+  __shared__ MGLevelD S[kMGMaxL];
+  if (threadIdx.x == 0) {
+    double* at = lds;
+    for (int l = l0; l < nl; l++) {
+      const MGLevelD G = Ls[l];
+      const int64_t n = G.nx * G.ny;
+      double* fx = at;
+      double* fy = fx + (G.nx + 1);
+      double* x = fy + (G.ny + 1);
+      double* b = x + n;
+      double* r = b + n;
+      at = r + n;
+      S[l] = MGLevelD{G.nx, G.ny, fx, fy, x, b, r};
+    }
+  }
+  __syncthreads();
+  for (int l = l0; l < nl; l++) {
+    double* fx = const_cast<double*>(S[l].fx);
+    double* fy = const_cast<double*>(S[l].fy);
+    for (int64_t q = threadIdx.x; q <= Ls[l].nx; q += kCoarseThreads) fx[q] = Ls[l].fx[q];
+    for (int64_t q = threadIdx.x; q <= Ls[l].ny; q += kCoarseThreads) fy[q] = Ls[l].fy[q];
+  }
+  each(S[l0].nx * S[l0].ny, [&](int64_t k) { S[l0].b[k] = Ls[l0].b[k]; });
+  auto smooth = [&](const MGLevelD& L, int c0, int pairs) {
+    for (int k = 0; k < pairs; k++) {
+      each((L.nx + 1) / 2 * L.ny, [&](int64_t q) { pt_sweep(L, g, c0, per, q); });
+      each((L.nx + 1) / 2 * L.ny, [&](int64_t q) { pt_sweep(L, g, 1 - c0, per, q); });
+    }
+  };
+  for (int l = l0; l < nl - 1; l++) {
+    const MGLevelD L = S[l], C = S[l + 1];
+    each(L.nx * L.ny, [&](int64_t k) { L.x[k] = 0.0; });
+    smooth(L, 0, nrelax);
+    each(L.nx * L.ny, [&](int64_t k) { pt_resid(L, g, per, k); });
+    each(C.nx * C.ny, [&](int64_t q) { pt_restrict(L, C, per, q); });
+  }
+  {
+    const MGLevelD L = S[nl - 1];
+    each(L.nx * L.ny, [&](int64_t k) { L.x[k] = 0.0; });
+    smooth(L, 0, kCoarse);
+    smooth(L, p0, kCoarse);
+  }
+  for (int l = nl - 2; l >= l0; l--) {
+    const MGLevelD F = S[l], C = S[l + 1];
+    each(F.nx * F.ny, [&](int64_t k) { pt_prolong(F, C, per, k); });
+    smooth(F, p0, nrelax);
+  }
+  each(S[l0].nx * S[l0].ny, [&](int64_t k) { Ls[l0].x[k] = S[l0].x[k]; });
+}
+
+// LDS bytes k_mg_coarse needs for levels l0..nl-1
+size_t mg_coarse_lds(const MGLevel* L, int l0, int nl)
+{
+  size_t words = 0;
+  for (int l = l0; l < nl; l++) words += (size_t)(L[l].nx + 1) + (size_t)(L[l].ny + 1) + 3 * (size_t)(L[l].nx * L[l].ny);
+  return words * sizeof(double);
+}
+
+MGLevelD dev_level(const MGLevel& L) { return MGLevelD{L.nx, L.ny, L.fx, L.fy, L.x, L.b, L.r}; }
 
 void sweep(MGHierarchy* H, int l, int color, hipStream_t s)
 {
   const MGLevel& L = H->L[l];
-  hipLaunchKernelGGL(k_mg_sweep, dim3(grid_of((L.nx + 1) / 2 * L.ny)), dim3(kBlock), 0, s, L.nx, L.ny,
-                     L.fx, L.fy, H->gamma, L.b, L.x, color, H->per);
+  hipLaunchKernelGGL(k_mg_sweep, dim3(grid_of((L.nx + 1) / 2 * L.ny)), dim3(kBlock), 0, s, dev_level(L),
+                     H->gamma, color, H->per);
 }
 
 void vcycle(MGHierarchy* H, int l, hipStream_t s)
 {
+  if (l >= H->lc) {  // the rest of the cycle in one workgroup
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kCoarseThreads), H->lc_lds, s, H->d_levels, l, H->nl,
+                       H->gamma, H->relax, H->nrelax, H->per);
+    return;
+  }
   const MGLevel& L = H->L[l];
   hipLaunchKernelGGL(k_mg_fill0, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, L.nx * L.ny, L.x);
   // post-smoothing order: black then red (the adjoint of the pre-smoother,
@@ -204,13 +312,13 @@ void vcycle(MGHierarchy* H, int l, hipStream_t s)
     sweep(H, l, 1, s);
   }
   const MGLevel& C = H->L[l + 1];
-  hipLaunchKernelGGL(k_mg_resid, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, L.nx, L.ny, L.fx, L.fy,
-                     H->gamma, L.b, L.x, L.r, H->per);
-  hipLaunchKernelGGL(k_mg_restrict, dim3(grid_of(C.nx * C.ny)), dim3(kBlock), 0, s, L.nx, L.ny, L.r, C.nx,
-                     C.ny, C.b, H->per);
+  hipLaunchKernelGGL(k_mg_resid, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, dev_level(L), H->gamma,
+                     H->per);
+  hipLaunchKernelGGL(k_mg_restrict, dim3(grid_of(C.nx * C.ny)), dim3(kBlock), 0, s, dev_level(L),
+                     dev_level(C), H->per);
   vcycle(H, l + 1, s);
-  hipLaunchKernelGGL(k_mg_prolong, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, L.nx, L.ny, L.x, C.nx,
-                     C.ny, C.x, H->per);
+  hipLaunchKernelGGL(k_mg_prolong, dim3(grid_of(L.nx * L.ny)), dim3(kBlock), 0, s, dev_level(L),
+                     dev_level(C), H->per);
   for (int k = 0; k < H->nrelax; k++) {
     sweep(H, l, p0, s);
     sweep(H, l, 1 - p0, s);
@@ -256,6 +364,31 @@ MGHierarchy* build(DiffusionProblem* P)
     ny = cy;
   }
   H->nl = l;
+  // the coarse cycle takes the trailing levels whose arrays fit in one
+  // workgroup's LDS (none when even the coarsest does not, e.g. a grid that
+  // cannot be coarsened)
+  int dev = 0;
+  CEDA_HIP_CHECK(hipGetDevice(&dev));
+  int lds_max = 0;
+  CEDA_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  H->lc = l;
+  // (the kernel's static descriptor array comes on top of the dynamic LDS)
+  const size_t lds_dyn_max = (size_t)lds_max - sizeof(MGLevelD) * kMGMaxL;
+  while (H->lc > 0 && H->L[H->lc - 1].nx * H->L[H->lc - 1].ny <= kMGCoarseCells &&
+         mg_coarse_lds(H->L, H->lc - 1, l) <= lds_dyn_max)
+    H->lc--;
+  H->lc_lds = (H->lc < l) ? mg_coarse_lds(H->L, H->lc, l) : 0;
+  if (H->lc_lds > 65536 &&
+      hipFuncSetAttribute((const void*)k_mg_coarse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)H->lc_lds) != hipSuccess) {
+    (void)hipGetLastError();
+    while (H->lc < l && mg_coarse_lds(H->L, H->lc, l) > 65536 - sizeof(MGLevelD) * kMGMaxL) H->lc++;
+    H->lc_lds = (H->lc < l) ? mg_coarse_lds(H->L, H->lc, l) : 0;
+  }
+  std::vector<MGLevelD> hl(l);
+  for (int k = 0; k < l; k++) hl[k] = dev_level(H->L[k]);
+  CEDA_HIP_CHECK(ceda::dev_malloc(&H->d_levels, sizeof(MGLevelD) * l));
+  CEDA_HIP_CHECK(ceda::copy_sync(H->d_levels, hl.data(), sizeof(MGLevelD) * l, hipMemcpyHostToDevice));
   CEDA_HIP_CHECK(ceda::dev_malloc(&H->gamma, sizeof(double)));
   CEDA_HIP_CHECK(ceda::host_malloc(&H->h_gamma, sizeof(double), hipHostMallocDefault));
   return H;
@@ -273,6 +406,7 @@ void mg_destroy(MGHierarchy* H)
       if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
   }
   if (H->gamma) CEDA_HIP_CHECK(ceda::dev_free(H->gamma));
+  if (H->d_levels) CEDA_HIP_CHECK(ceda::dev_free(H->d_levels));
   if (H->h_gamma) CEDA_HIP_CHECK(ceda::host_free(H->h_gamma));
   if (H->graph) CEDA_HIP_CHECK(ceda::graph_exec_destroy(H->graph));
   delete H;
