@@ -273,41 +273,62 @@ def implicit_workload(pk, args):
     gamma_a = 1767732205903.0 / 4055673282236.0
     ratio = gamma_a * dt / dx ** 2
     log(f"implicit: {side}^2, dt {dt}, gamma*dt/dx^2 = {ratio:.1f}, {n_steps} steps per method")
-    # training
-    t0 = time.perf_counter()
-    tn = args.train_n
-    R, E, iters, rc = pk.training_set(ctx, args.nsamp, tn, ratio, reltol=1e-12, seed=1234)
-    hip_sync()
-    t_data = time.perf_counter() - t0
-    assert rc == 0
+    methods = args.imethods.split(",")
+    if not args.no_untrained:
+        methods.append("identity")
     dil = tuple(int(x) for x in args.dil.split(","))
-    net = pk.LearnedPrec(ctx, dil)
-    net.init_identity(8e-4, 0)
-    net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
-    t0 = time.perf_counter()
-    hist = net.train(args.nsamp, tn, ratio, R, E, args.train_steps, args.train_batch, args.train_lr, 0)
-    hip_sync()
-    t_train = time.perf_counter() - t0
-    log(f"training set {args.nsamp} pairs of {tn}^2 (Jacobi-PCG <= {iters} its) {t_data:.1f} s; "
-        f"{args.train_steps} Adam steps {t_train:.1f} s; loss {hist[:20].mean():.3f} -> {hist[-50:].mean():.3f}")
-    R.destroy()
-    E.destroy()
+    sdil = tuple(int(x) for x in args.sym_dil.split(","))
+    training = {}
+
+    def train(symmetric):
+        """Pairs from batched Jacobi-PCG solves at relative residual 1e-12,
+        then Adam on the GPU; the network starts at the identity (Jacobi)."""
+        tn = args.sym_train_n if symmetric else args.train_n
+        steps = args.sym_train_steps if symmetric else args.train_steps
+        lr = args.sym_train_lr if symmetric else args.train_lr
+        t0 = time.perf_counter()
+        R, E, iters, rc = pk.training_set(ctx, args.nsamp, tn, ratio, reltol=1e-12, seed=1234)
+        hip_sync()
+        t_data = time.perf_counter() - t0
+        assert rc == 0
+        net = pk.LearnedPrec(ctx, sdil if symmetric else dil, symmetric=symmetric)
+        net.init_identity(8e-4, 0)
+        net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
+        t0 = time.perf_counter()
+        hist = net.train(args.nsamp, tn, ratio, R, E, steps, args.train_batch, lr, 0)
+        hip_sync()
+        t_train = time.perf_counter() - t0
+        tag = "symmetric" if symmetric else "relu"
+        log(f"{tag} network: training set {args.nsamp} pairs of {tn}^2 (Jacobi-PCG <= {iters} its) "
+            f"{t_data:.1f} s; {steps} Adam steps {t_train:.1f} s; loss {hist[:20].mean():.3f} -> "
+            f"{hist[-50:].mean():.3f}")
+        training[tag] = {"pairs": args.nsamp, "grid": tn, "data_s": t_data, "train_s": t_train,
+                         "steps": steps, "batch": args.train_batch, "lr": lr,
+                         "dilations": list(sdil if symmetric else dil),
+                         "loss_first20": float(hist[:20].mean()), "loss_last50": float(hist[-50:].mean())}
+        R.destroy()
+        E.destroy()
+        return net
+
+    net = train(False) if any(m in ("learned", "learned-fcg") for m in methods) else None
+    snet = train(True) if any(m.startswith("learned-sym") for m in methods) else None
     ident = pk.LearnedPrec(ctx, dil)
     ident.init_identity(0.0, 0)
 
-    def run(kind, mrestart=args.mrestart):
+    def run(kind):
         """kind: 'jacobi' (Jacobi-PCG), 'mg' (PCG + one multigrid V-cycle),
         'learned' (FGMRES + network), 'learned-fcg' (flexible PCG + network),
-        'identity' (FGMRES + untrained network)."""
+        'learned-sym' (FGMRES + symmetric network), 'learned-sym-pcg' (PCG +
+        symmetric network), 'identity' (FGMRES + untrained network)."""
         G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=side, ny=side, kx=1.0, ky=1.0)
         if kind not in ("jacobi", "mg"):
-            (ident if kind == "identity" else net).attach(G)
-        ls = {"jacobi": "cg", "mg": "cg", "learned-fcg": "fcg"}.get(kind, "fgmres")
+            {"identity": ident, "learned-sym": snet, "learned-sym-pcg": snet}.get(kind, net).attach(G)
+        ls = {"jacobi": "cg", "mg": "cg", "learned-fcg": "fcg", "learned-sym-pcg": "cg"}.get(kind, "fgmres")
         prec = {"jacobi": True, "mg": "mg"}.get(kind, "learned")
         I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=20000 if ls == "cg" else args.imaxl,
                           precond=prec)
         if ls == "fgmres":
-            pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
+            pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, args.sym_mrestart if kind == "learned-sym" else args.mrestart)
         if args.jtimes:
             I.analytic_jtimes()
             if args.fused_pcg:
@@ -339,17 +360,15 @@ def implicit_workload(pk, args):
         log(f"{kind}: {out}")
         return out, y
 
-    methods = args.imethods.split(",")
-    if not args.no_untrained:
-        methods.append("identity")
     res, ys = {}, {}
     for kind in methods:
         res[kind], ys[kind] = run(kind)
     names = {"jacobi": "jacobi_pcg", "mg": "multigrid_pcg", "learned": "learned_fgmres",
-             "learned-fcg": "learned_flexible_pcg", "identity": "untrained_fgmres"}
+             "learned-fcg": "learned_flexible_pcg", "identity": "untrained_fgmres",
+             "learned-sym": "learned_symmetric_fgmres", "learned-sym-pcg": "learned_symmetric_pcg"}
     out = {"metric": "time-to-solution at fixed L2 err, learned-preconditioned FGMRES vs Jacobi-PCG",
            "config": f"mms-dirichlet {side}x{side} kx=ky=1 FP64, DIRK3 fixed dt=1e-3, LS reltol 1e-8",
-           "steps_timed": n_steps, "gamma_dt_over_dx2": ratio, "network_dilations": list(dil),
+           "steps_timed": n_steps, "gamma_dt_over_dx2": ratio,
            "jtimes": "analytic (stencil)" if args.jtimes else "difference quotient (as the reference)"}
     for kind in methods:
         out[names[kind]] = res[kind]
@@ -362,14 +381,15 @@ def implicit_workload(pk, args):
             out[f"speedup_{names[kind]}_vs_jacobi_pcg"] = res["jacobi"]["s_per_step"] / res[kind]["s_per_step"]
             out[f"rel_l2_diff_{names[kind]}_vs_jacobi_pcg"] = float(
                 np.sqrt(np.mean((ys[kind] - yj) ** 2)) / np.sqrt(np.mean(yj ** 2)))
-    if "learned" in res and "identity" in res:
-        out["krylov_iters_ratio_learned_vs_untrained"] = res["learned"]["lin_iters"] / res["identity"]["lin_iters"]
-    out["training"] = {"pairs": args.nsamp, "grid": tn, "data_s": t_data, "train_s": t_train,
-                       "steps": args.train_steps, "batch": args.train_batch, "lr": args.train_lr,
-                       "loss_first20": float(hist[:20].mean()), "loss_last50": float(hist[-50:].mean())}
+    for kind in ("learned", "learned-sym"):
+        if kind in res and "identity" in res:
+            out[f"krylov_iters_ratio_{names[kind]}_vs_untrained"] = res[kind]["lin_iters"] / res["identity"]["lin_iters"]
+    out["training"] = training
     out["fgmres_restart"] = args.mrestart
-    net.release()
-    ident.release()
+    out["fgmres_restart_symmetric"] = args.sym_mrestart
+    for nt in (net, snet, ident):
+        if nt is not None:
+            nt.release()
     print(json.dumps(out))
 
 
@@ -582,6 +602,11 @@ def main():
     ap.add_argument("--dil", default="1,1,1,1", help="dilations of the network's 3x3 layers")
     ap.add_argument("--fused-pcg", action="store_true",
                     help="implicit/c5: PCG in three fused kernels per iteration (with the analytic J v)")
+    ap.add_argument("--sym-dil", default="1,3,9", help="dilations of the symmetric network's layers")
+    ap.add_argument("--sym-train-n", type=int, default=128, help="training grid side, symmetric network")
+    ap.add_argument("--sym-train-steps", type=int, default=3000)
+    ap.add_argument("--sym-train-lr", type=float, default=3e-3)
+    ap.add_argument("--sym-mrestart", type=int, default=20, help="FGMRES restart with the symmetric network")
     ap.add_argument("--imethods", default="jacobi,mg,learned,learned-fcg",
                     help="implicit: methods to run (jacobi first)")
     ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")

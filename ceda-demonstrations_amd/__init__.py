@@ -134,6 +134,8 @@ _SIGS = {
     "ARKodeSetJacTimes": (i32, [vp, vp, vp]),
     "ARKodeGetNumLSFallbacks": (i32, [vp, C.POINTER(C.c_long)]),
     "LearnedPrec_Create": (vp, [i32, C.POINTER(i32), vp]),
+    "LearnedPrec_CreateSymmetric": (vp, [i32, C.POINTER(i32), vp]),
+    "LearnedPrec_IsSymmetric": (i32, [vp]),
     "LearnedPrec_Destroy": (None, [vp]),
     "LearnedPrec_NumParams": (i32, [vp]),
     "LearnedPrec_SetParams": (i32, [vp, C.POINTER(C.c_float)]),
@@ -582,13 +584,17 @@ class Integrator:
 
 class LearnedPrec:
     """The learned preconditioner network (learned_prec.h): 3x3 conv layers
-    with the given dilations, 16 hidden channels, BF16 on the matrix cores."""
+    with the given dilations, 16 hidden channels, BF16 on the matrix cores.
+    symmetric=True: the linear network K^T K (K the layers, applied and then
+    its adjoint), symmetric positive semi-definite."""
 
-    def __init__(self, ctx, dil=(1, 1, 1, 1)):
+    def __init__(self, ctx, dil=(1, 1, 1, 1), symmetric=False):
         d = (i32 * len(dil))(*dil)
         self.ctx = ctx
         self.dil = tuple(dil)
-        self.h = lib().LearnedPrec_Create(len(dil), d, ctx.h)
+        self.symmetric = bool(symmetric)
+        create = lib().LearnedPrec_CreateSymmetric if symmetric else lib().LearnedPrec_Create
+        self.h = create(len(dil), d, ctx.h)
         if not self.h:
             raise ValueError("bad network shape")
         ctx.register(self)

@@ -745,19 +745,23 @@ int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double
 int pcg(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
 {
   const bool pre = A->psolve && A->LS->pretype != SUN_PREC_NONE;
-  // the fused kernels know J (the stencil), Jacobi and the multigrid cycle
-  // of the problem that is also the callbacks' user data
+  // the fused kernels know J (the stencil) and Jacobi of the problem that
+  // is also the callbacks' user data
   if (A->fast_pcg && A->jtimes == JTimes && A->LS->type == LS_PCG && A->ud == (void*)A->fast_pcg) {
-    const int pmode = !pre ? 0 : (A->psolve == PSolve ? 1 : (A->psolve == MGPSolve ? 2 : -1));
-    if (pmode >= 0)
-      return pcg_fused(A, A->LS, A->fast_pcg, pmode, A->psolve, A->ud, t, gamma, b, x);
+    // Jacobi is fused into the update kernel; any other preconditioner is
+    // called through its callback
+    const int pmode = !pre ? 0 : (A->psolve == PSolve ? 1 : 2);
+    return pcg_fused(A, A->LS, A->fast_pcg, pmode, A->psolve, A->ud, t, gamma, b, x);
   }
   return pcg_with(A, A->LS, pre ? A->psolve : nullptr, A->ud, t, gamma, b, x, A->LS->type == LS_FPCG);
 }
 
 // restarted flexible GMRES, right preconditioning, classical Gram-Schmidt
-// with one re-orthogonalisation pass (two fused dot-product reductions per
-// iteration regardless of the basis size)
+// as SUNDIALS' SUNClassicalGS: one pass (a fused multi-dot that also gives
+// |w|, then one linear combination), repeated only when it cancelled more
+// than a factor kReorthFactor of |w|
+constexpr double kReorthFactor = 1000.0;
+
 int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
 {
   SUNLinearSolver S = A->LS;
@@ -788,8 +792,11 @@ int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
       total++;
       A->nli++;
       N_Vector w = S->V[j + 1];
+      double hn = 0.0, wnorm = 0.0;
       for (int pass = 0; pass < 2; pass++) {
-        N_VDotProdMulti(j + 1, w, S->V.data(), h2.data());
+        // S->V[j + 1] is w: the first pass's multi-dot also gives w.w
+        N_VDotProdMulti(pass == 0 ? j + 2 : j + 1, w, S->V.data(), h2.data());
+        if (pass == 0) wnorm = sqrt(h2[j + 1]);
         coef[0] = 1.0;
         X[0] = w;
         for (int i = 0; i <= j; i++) {
@@ -798,8 +805,9 @@ int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
           hv[i] = (pass == 0) ? h2[i] : hv[i] + h2[i];
         }
         N_VLinearCombination(j + 2, coef.data(), X.data(), w);
+        hn = sqrt(N_VDotProd(w, w));
+        if (!(kReorthFactor * hn < wnorm)) break;
       }
-      const double hn = sqrt(N_VDotProd(w, w));
       hv[j + 1] = hn;
       for (int i = 0; i < j; i++) {
         const double a = cs[i] * hv[i] + sn[i] * hv[i + 1];

@@ -44,6 +44,7 @@ namespace {
 constexpr int kC = 16;        // hidden channels
 constexpr int kMaxL = 8;      // layers
 constexpr int kWavesPB = kBlock / kWave;
+constexpr int kConvU = 2;     // 16-pixel blocks per wavefront step of the convolution
 constexpr int kPW = 9 * 256 + 16;  // weight-gradient partial of one wavefront
 constexpr int kWGradWaves = 1024;
 
@@ -90,58 +91,125 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
     } else bv[0] = a.bias[0];
   }
   const int nx = a.nx, ny = a.ny, d = a.dil;
+  // XCD-aware order: workgroups go round-robin to the 8 XCDs, so XCD k
+  // (blockIdx % 8) takes the k-th of 8 contiguous bands of image rows and
+  // its workgroups sweep that band row by row, one segment of
+  // kWavesPB * kConvU 16-pixel blocks at a time (kConvU consecutive blocks
+  // per wavefront).  The rows y - d .. y + d a tile reads are then read by
+  // the same XCD at about the same time and stay in its L2.
   const int bpr = (nx + 15) >> 4;
-  const int64_t nblk = (int64_t)a.nb * ny * bpr;
-  const int64_t nwaves = (int64_t)gridDim.x * kWavesPB;
-  for (int64_t blk = (int64_t)blockIdx.x * kWavesPB + (threadIdx.x >> 6); blk < nblk; blk += nwaves) {
-    const int bx = (int)(blk % bpr);
-    const int64_t rowi = blk / bpr;
-    const int y = (int)(rowi % ny);
-    const int64_t b = rowi / ny;
-    const int x = bx * 16 + n;
-    fx4 acc = {0.f, 0.f, 0.f, 0.f};
-    if (CIN == kC) {
-      const unsigned short* in = static_cast<const unsigned short*>(a.in);
+  const int spr = (bpr + kWavesPB * kConvU - 1) / (kWavesPB * kConvU);
+  const int rows = a.nb * ny;
+  const int band = (rows + 7) >> 3;
+  const int r0 = (int)(blockIdx.x & 7) * band;
+  const int nrow = (r0 < rows) ? min(rows - r0, band) : 0;
+  const int nseg = nrow * spr;
+  const int wave = threadIdx.x >> 6;
+  for (int c = (int)(blockIdx.x >> 3); c < nseg; c += (int)(gridDim.x >> 3)) {
+    const int q = c / spr;
+    const int bx0 = (c - q * spr) * (kWavesPB * kConvU) + wave * kConvU;
+    if (bx0 >= bpr) continue;  // wavefront-uniform: past the row end
+    const int rowi = r0 + q;
+    const int bimg = rowi / ny, y = rowi - bimg * ny;
+    const int64_t img = (int64_t)bimg * ny;  // first row of the image
+    fx4 acc[kConvU];
+    // interior segment (wavefront-uniform): every tap of every pixel is
+    // inside the image, so the taps are constant offsets from one address
+    const bool interior = bx0 * 16 - d >= 0 && (bx0 + kConvU) * 16 + d <= nx && y - d >= 0 && y + d < ny;
+    // Elsewhere every load is issued unconditionally from a clamped
+    // (in-bounds) address and the zero padding applied by a select
+    // afterwards, so the loads of all kConvU blocks are in flight together
+    // instead of one branch and one wait per tap.
+    if (CIN == kC && interior) {
+      const unsigned short* p =
+          static_cast<const unsigned short*>(a.in) + ((img + y) * nx + bx0 * 16 + n) * kC + 4 * g;
+      bfx4 Bt[kConvU][9];
 #pragma unroll
-      for (int t = 0; t < 9; t++) {
-        const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
-        bfx4 B = {0, 0, 0, 0};
-        if (x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny)
-          B = *reinterpret_cast<const bfx4*>(in + (((b * ny + ys) * nx + xs) * kC + 4 * g));
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[t], B, acc, 0, 0, 0);
+      for (int u = 0; u < kConvU; u++)
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+          Bt[u][t] = *reinterpret_cast<const bfx4*>(
+              p + ((int64_t)(t / 3 - 1) * d * nx + (t % 3 - 1) * d + u * 16) * kC);
+#pragma unroll
+      for (int u = 0; u < kConvU; u++) {
+        acc[u] = fx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 9; t++) acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[t], Bt[u][t], acc[u], 0, 0, 0);
+      }
+    } else if (CIN == kC) {
+      const unsigned short* in = static_cast<const unsigned short*>(a.in);
+      bfx4 Bt[kConvU][9];
+#pragma unroll
+      for (int u = 0; u < kConvU; u++) {
+        const int x = (bx0 + u) * 16 + n;
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
+          const int xc = min(max(xs, 0), nx - 1), yc = min(max(ys, 0), ny - 1);
+          Bt[u][t] = *reinterpret_cast<const bfx4*>(in + (((img + yc) * nx + xc) * kC + 4 * g));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kConvU; u++) {
+        const int x = (bx0 + u) * 16 + n;
+        acc[u] = fx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+          const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
+          const bool ok = x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny;
+          const bfx4 zero = {0, 0, 0, 0};
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[t], ok ? Bt[u][t] : zero, acc[u], 0, 0, 0);
+        }
       }
     } else {
       const double* in = static_cast<const double*>(a.in);
-      const double is = a.inv_scale ? a.inv_scale[b] : 1.0;
-      bfx4 B;
+      const double is = a.inv_scale ? a.inv_scale[bimg] : 1.0;
+      double v[kConvU][4];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int t = 4 * g + j;
-        unsigned short v = 0;
-        if (t < 9 && x < nx) {
+      for (int u = 0; u < kConvU; u++) {
+        const int x = (bx0 + u) * 16 + n;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int t = min(4 * g + j, 8);
           const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
-          if (xs >= 0 && xs < nx && ys >= 0 && ys < ny) v = d2bf(in[(b * ny + ys) * nx + xs] * is);
+          const int xc = min(max(xs, 0), nx - 1), yc = min(max(ys, 0), ny - 1);
+          v[u][j] = in[(img + yc) * nx + xc];
         }
-        B[j] = (short)v;
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[0], B, acc, 0, 0, 0);
-    }
-    if (x >= nx) continue;
-    const int64_t pix = (b * ny + y) * nx + x;
-    if (COUT == kC) {
-      bfx4 o;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        float v = acc[i] + bv[i];
-        if (RELU) v = fmaxf(v, 0.0f);
-        if (a.mask && !(bf2f(a.mask[pix * kC + 4 * g + i]) > 0.0f)) v = 0.0f;
-        o[i] = (short)f2bf(v);
+      for (int u = 0; u < kConvU; u++) {
+        const int x = (bx0 + u) * 16 + n;
+        bfx4 B;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int t = 4 * g + j;
+          const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
+          const bool ok = t < 9 && x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny;
+          B[j] = (short)(ok ? d2bf(v[u][j] * is) : (unsigned short)0);
+        }
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[0], B, fx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       }
-      *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + pix * kC + 4 * g) = o;
-    } else if (g == 0) {
-      const double v = (double)(acc[0] + bv[0]) * a.out_mul * (a.scale ? a.scale[b] : 1.0);
-      double* out = static_cast<double*>(a.out);
-      out[pix] = a.addto ? out[pix] + v : v;
+    }
+#pragma unroll
+    for (int u = 0; u < kConvU; u++) {
+      const int x = (bx0 + u) * 16 + n;
+      if (x >= nx) continue;
+      const int64_t pix = (img + y) * nx + x;
+      if (COUT == kC) {
+        bfx4 o;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          float v = acc[u][i] + bv[i];
+          if (RELU) v = fmaxf(v, 0.0f);
+          if (a.mask && !(bf2f(a.mask[pix * kC + 4 * g + i]) > 0.0f)) v = 0.0f;
+          o[i] = (short)f2bf(v);
+        }
+        *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + pix * kC + 4 * g) = o;
+      } else if (g == 0) {
+        const double v = (double)(acc[u][0] + bv[0]) * a.out_mul * (a.scale ? a.scale[bimg] : 1.0);
+        double* out = static_cast<double*>(a.out);
+        out[pix] = a.addto ? out[pix] + v : v;
+      }
     }
   }
 }
@@ -236,9 +304,10 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const WGradArgs a)
 // b[cout]: 64 consecutive partial offsets per block (coalesced reads), each
 // summed over the wavefronts in 4 interleaved groups, the groups combined in
 // a fixed order (deterministic), then scattered to the parameter it belongs to
+// (added to it when accum is set; no bias gradient when gb is null)
 __global__ __launch_bounds__(kBlock) void k_wreduce(const float* __restrict__ partial, int nwaves,
                                                     int cin, int cout, float* __restrict__ gw,
-                                                    float* __restrict__ gb)
+                                                    float* __restrict__ gb, int accum)
 {
   const int q = threadIdx.x / 64, off = blockIdx.x * 64 + (threadIdx.x & 63);
   float s = 0.0f;
@@ -251,15 +320,17 @@ __global__ __launch_bounds__(kBlock) void k_wreduce(const float* __restrict__ pa
   const float v = ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
   if (off >= 9 * 256) {
     const int b = off - 9 * 256;
-    if (b < cout) gb[b] = v;
+    if (gb && b < cout) gb[b] = accum ? gb[b] + v : v;
     return;
   }
   const int t = off / 256, m = (off % 256) / 16, n = off % 16;
+  int k = -1;
   if (cin == kC) {
-    if (m < cout) gw[(m * cin + n) * 9 + t] = v;  // tile t: m = co, n = ci
+    if (m < cout) k = (m * cin + n) * 9 + t;  // tile t: m = co, n = ci
   } else if (t == 0 && n < 9 && m < cout) {
-    gw[m * 9 + n] = v;  // single tile: m = co, n = tap
+    k = m * 9 + n;  // single tile: m = co, n = tap
   }
+  if (k >= 0) gw[k] = accum ? gw[k] + v : v;
 }
 
 // ------------------------------------------------- packing of the A operands
@@ -302,13 +373,20 @@ __global__ __launch_bounds__(kBlock) void k_sumsq(int64_t npix, int P, const dou
   __syncthreads();
   if (threadIdx.x == 0) part[(int64_t)b * P + p] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
-// s = RMS, inv = 1/s (an all-zero image keeps s = 1)
-__global__ void k_rms_final(int nb, int P, int64_t npix, const double* part, double* s, double* inv)
+// s = RMS, inv = 1/s (an all-zero image keeps s = 1); one workgroup per
+// image sums its P partials in a fixed order
+__global__ __launch_bounds__(kBlock) void k_rms_final(int nb, int P, int64_t npix, const double* part,
+                                                      double* s, double* inv)
 {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+  const int b = blockIdx.x;
   double t = 0.0;
-  for (int p = 0; p < P; p++) t += part[(int64_t)b * P + p];
+  for (int p = threadIdx.x; p < P; p += kBlock) t += part[(int64_t)b * P + p];
+  __shared__ double sh[kWavesPB];
+  for (int off = kWave / 2; off > 0; off >>= 1) t += __shfl_xor(t, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  t = ((sh[0] + sh[1]) + sh[2]) + sh[3];
   double v = sqrt(t / (double)npix);
   if (!(v > 0.0)) v = 1.0;
   s[b] = v;
@@ -552,6 +630,7 @@ unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 
 // ============================================================== host side
 struct LearnedPrec {
   int nl = 0;
+  bool sym = false;  // M = K^T K with K the layers (linear, no biases)
   int dil[kMaxL], cin[kMaxL], cout[kMaxL], woff[kMaxL], boff[kMaxL];
   int np = 0;
   double out_scale = 1.0;
@@ -564,7 +643,8 @@ struct LearnedPrec {
   // workspace
   int64_t cap_pix = 0;
   int cap_nb = 0;
-  unsigned short* act[kMaxL] = {};  // act[l] = input of layer l (l >= 1), NHWC16
+  unsigned short* act[kMaxL] = {};  // act[l] = input of layer l (l >= 1), NHWC16; sym: act[nl] = K x
+  unsigned short* uact[kMaxL] = {}; // sym: uact[l] = input of the adjoint of layer l - 1
   unsigned short *gA = nullptr, *gB = nullptr;  // gradient ping-pong, NHWC16
   double *u = nullptr, *dd = nullptr, *ad = nullptr, *gy = nullptr;
   double *sc = nullptr, *isc = nullptr, *part = nullptr, *lossb = nullptr, *coef = nullptr;
@@ -579,25 +659,32 @@ void lp_pack(LearnedPrec* L)
   hipStream_t s = L->ex->stream;
   for (int l = 0; l < L->nl; l++)
     hipLaunchKernelGGL(k_pack, dim3(9), dim3(256), 0, s, L->d_p + L->woff[l], L->cin[l], L->cout[l],
-                       L->d_afw + (int64_t)l * 9 * 256, l > 0 ? L->d_abw + (int64_t)l * 9 * 256 : nullptr);
+                       L->d_afw + (int64_t)l * 9 * 256,
+                       (l > 0 || L->sym) ? L->d_abw + (int64_t)l * 9 * 256 : nullptr);
   CEDA_HIP_CHECK(hipGetLastError());
 }
+
+constexpr int kMaxParts = 512;  // workgroups per image of the per-image reductions
+int rms_parts(int64_t npix) { return (int)std::min<int64_t>(kMaxParts, (npix + 8191) / 8192); }
 
 void lp_workspace(LearnedPrec* L, int nb, int64_t npix)
 {
   const int64_t tot = npix * nb;
-  const int P = (int)std::min<int64_t>(64, (npix + 4095) / 4096);
+  const int P = rms_parts(npix);
   if (tot > L->cap_pix) {
     CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
-    for (int l = 0; l < kMaxL; l++)
-      if (L->act[l]) {
-        CEDA_HIP_CHECK(ceda::dev_free(L->act[l]));
-        L->act[l] = nullptr;
-      }
+    for (int l = 0; l < kMaxL; l++) {
+      if (L->act[l]) CEDA_HIP_CHECK(ceda::dev_free(L->act[l]));
+      if (L->uact[l]) CEDA_HIP_CHECK(ceda::dev_free(L->uact[l]));
+      L->act[l] = L->uact[l] = nullptr;
+    }
     void* ps[] = {L->gA, L->gB, L->u, L->dd, L->ad, L->gy};
     for (void* p : ps)
       if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
-    for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(ceda::dev_malloc(&L->act[l], sizeof(unsigned short) * kC * tot));
+    for (int l = 1; l < L->nl + (L->sym ? 1 : 0); l++)
+      CEDA_HIP_CHECK(ceda::dev_malloc(&L->act[l], sizeof(unsigned short) * kC * tot));
+    if (L->sym)
+      for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(ceda::dev_malloc(&L->uact[l], sizeof(unsigned short) * kC * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->gA, sizeof(unsigned short) * kC * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->gB, sizeof(unsigned short) * kC * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->u, sizeof(double) * tot));
@@ -612,7 +699,7 @@ void lp_workspace(LearnedPrec* L, int nb, int64_t npix)
     for (void* p : ps)
       if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
     const int cnb = std::max(nb, L->cap_nb);
-    const int pc = std::max(cnb * 64 * 2, L->part_cap);
+    const int pc = std::max(cnb * P * 2, L->part_cap);
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->sc, sizeof(double) * cnb));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->isc, sizeof(double) * cnb));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->part, sizeof(double) * pc));
@@ -624,7 +711,6 @@ void lp_workspace(LearnedPrec* L, int nb, int64_t npix)
   if (!L->wpart) CEDA_HIP_CHECK(ceda::dev_malloc(&L->wpart, sizeof(float) * kPW * kWGradWaves));
 }
 
-int rms_parts(int64_t npix) { return (int)std::min<int64_t>(64, (npix + 4095) / 4096); }
 
 // per-image scale s = RMS(in) and 1/s
 void lp_scale(LearnedPrec* L, int nb, int64_t npix, const double* in)
@@ -632,15 +718,58 @@ void lp_scale(LearnedPrec* L, int nb, int64_t npix, const double* in)
   hipStream_t s = L->ex->stream;
   const int P = rms_parts(npix);
   hipLaunchKernelGGL(k_sumsq, dim3(P, nb), dim3(kBlock), 0, s, npix, P, in, L->part);
-  hipLaunchKernelGGL(k_rms_final, dim3((nb + 63) / 64), dim3(64), 0, s, nb, P, npix, L->part, L->sc, L->isc);
+  hipLaunchKernelGGL(k_rms_final, dim3(nb), dim3(kBlock), 0, s, nb, P, npix, L->part, L->sc, L->isc);
 }
 
 template <int CIN, int COUT, bool RELU>
 void conv(LearnedPrec* L, const ConvArgs& a)
 {
-  const int64_t blocks = (int64_t)a.nb * a.ny * ((a.nx + 15) / 16);
-  const unsigned grid = (unsigned)std::min<int64_t>((blocks + kWavesPB - 1) / kWavesPB, 16384);
+  // one row segment (kWavesPB * kConvU blocks of 16 pixels) per workgroup
+  // and step; a multiple of 8 workgroups (the kernel's XCD bands)
+  const int64_t segs = (int64_t)a.nb * a.ny * (((a.nx + 15) / 16 + kWavesPB * kConvU - 1) / (kWavesPB * kConvU));
+  const unsigned grid = (unsigned)std::min<int64_t>((segs + 7) / 8 * 8, 16384);
   hipLaunchKernelGGL((k_conv<CIN, COUT, RELU>), dim3(grid), dim3(kBlock), 0, L->ex->stream, a);
+}
+
+// symmetric network: h_0 = in/s, h_{l+1} = W_l * h_l (l < nl), then the
+// adjoint layers u_nl = h_nl, u_l = W_l^T * u_{l+1} (the input-gradient
+// convolution: flipped taps, exchanged channels), out = [out +] scale *
+// out_scale * u_0.  No biases and no activation function, so the map is
+// linear, symmetric and positive semi-definite up to the BF16 roundings.
+void lp_forward_sym(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto,
+                    bool normalised)
+{
+  ConvArgs a{};
+  a.nb = nb;
+  a.nx = nx;
+  a.ny = ny;
+  const int nl = L->nl;
+  for (int l = 0; l < nl; l++) {
+    a.dil = L->dil[l];
+    a.apack = L->d_afw + (int64_t)l * 9 * 256;
+    a.in = (l == 0) ? (const void*)in : (const void*)L->act[l];
+    a.inv_scale = (l == 0) ? L->isc : nullptr;
+    a.out = L->act[l + 1];
+    if (l == 0) conv<1, kC, false>(L, a);
+    else conv<kC, kC, false>(L, a);
+  }
+  a.inv_scale = nullptr;
+  for (int l = nl - 1; l >= 0; l--) {
+    a.dil = L->dil[l];
+    a.apack = L->d_abw + (int64_t)l * 9 * 256;
+    a.in = (l == nl - 1) ? (const void*)L->act[nl] : (const void*)L->uact[l + 1];
+    if (l == 0) {
+      a.out = out;
+      a.scale = normalised ? nullptr : L->sc;
+      a.out_mul = L->out_scale;
+      a.addto = addto;
+      conv<kC, 1, false>(L, a);
+    } else {
+      a.out = L->uact[l];
+      conv<kC, kC, false>(L, a);
+    }
+  }
+  CEDA_HIP_CHECK(hipGetLastError());
 }
 
 // forward pass; the output (FP64) goes to out with out = [out +] scale*out_scale*N
@@ -648,6 +777,10 @@ void conv(LearnedPrec* L, const ConvArgs& a)
 void lp_forward(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto,
                 bool normalised)
 {
+  if (L->sym) {
+    lp_forward_sym(L, nb, nx, ny, in, out, addto, normalised);
+    return;
+  }
   ConvArgs a{};
   a.nb = nb;
   a.nx = nx;
@@ -676,16 +809,68 @@ void lp_forward(LearnedPrec* L, int nb, int nx, int ny, const double* in, double
   CEDA_HIP_CHECK(hipGetLastError());
 }
 
+// weight gradient of layer l from input x (FP64 images scaled by isc, or
+// NHWC16) and output gradient dy; written to the gradient, or added to it
 template <int CIN, int COUT>
-void wgrad(LearnedPrec* L, int l, int nb, int nx, int ny, const void* x, const void* dy)
+void wgrad(LearnedPrec* L, int l, int nb, int nx, int ny, const void* x, const double* isc, const void* dy,
+           int accum)
 {
-  WGradArgs w{nb, nx, ny, L->dil[l], x, L->isc, dy, L->wpart};
+  WGradArgs w{nb, nx, ny, L->dil[l], x, isc, dy, L->wpart};
   // about 8 pixel blocks per wavefront, at most kWGradWaves partials
   const int64_t nblk = (int64_t)nb * ny * ((nx + 15) / 16);
   const int nw = (int)std::min<int64_t>(kWGradWaves, std::max<int64_t>(kWavesPB, ((nblk + 7) / 8 + 3) / 4 * 4));
   hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(nw / kWavesPB), dim3(kBlock), 0, L->ex->stream, w);
   hipLaunchKernelGGL(k_wreduce, dim3((kPW + 63) / 64), dim3(kBlock), 0, L->ex->stream, L->wpart, nw,
-                     L->cin[l], L->cout[l], L->d_g + L->woff[l], L->d_g + L->boff[l]);
+                     L->cin[l], L->cout[l], L->d_g + L->woff[l], L->sym ? nullptr : L->d_g + L->boff[l],
+                     accum);
+}
+
+// backward pass of the symmetric network from the output gradient L->gy.
+// Each layer's weights are used twice, so its gradient has two parts: from
+// the adjoint layer u_l = W_l^T u_{l+1} with output gradient g,
+//   dW[co][ci][t] = sum_p u_{l+1}[p][co] g[p + off_t][ci]
+// (the weight-gradient GEMM with the roles of input and output gradient
+// exchanged), written first; then from the layer h_{l+1} = W_l h_l, added.
+// The gradient of an adjoint layer's input is the forward convolution of its
+// output gradient, that of a layer's input the adjoint convolution.
+void lp_backward_sym(LearnedPrec* L, int nb, int nx, int ny, const double* r)
+{
+  const int nl = L->nl;
+  ConvArgs a{};
+  a.nb = nb;
+  a.nx = nx;
+  a.ny = ny;
+  const void* g = L->gy;  // FP64 for the adjoint of layer 0
+  unsigned short* gout = L->gA;
+  for (int l = 0; l < nl; l++) {
+    const void* uin = (l == nl - 1) ? (const void*)L->act[nl] : (const void*)L->uact[l + 1];
+    if (l == 0) wgrad<1, kC>(L, l, nb, nx, ny, g, nullptr, uin, 0);
+    else wgrad<kC, kC>(L, l, nb, nx, ny, g, nullptr, uin, 0);
+    a.dil = L->dil[l];
+    a.apack = L->d_afw + (int64_t)l * 9 * 256;
+    a.in = g;
+    a.out = gout;
+    if (l == 0) conv<1, kC, false>(L, a);
+    else conv<kC, kC, false>(L, a);
+    g = gout;
+    gout = (gout == L->gA) ? L->gB : L->gA;
+  }
+  // g is now the gradient of h_nl = u_nl
+  for (int l = nl - 1; l >= 0; l--) {
+    if (l == 0) {
+      wgrad<1, kC>(L, l, nb, nx, ny, r, L->isc, g, 1);
+      break;
+    }
+    wgrad<kC, kC>(L, l, nb, nx, ny, L->act[l], nullptr, g, 1);
+    a.dil = L->dil[l];
+    a.apack = L->d_abw + (int64_t)l * 9 * 256;
+    a.in = g;
+    a.out = gout;
+    conv<kC, kC, false>(L, a);
+    g = gout;
+    gout = (gout == L->gA) ? L->gB : L->gA;
+  }
+  CEDA_HIP_CHECK(hipGetLastError());
 }
 
 // loss (device scalar *dloss) and parameter gradient (L->d_g) of a batch
@@ -707,6 +892,10 @@ void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const dou
   hipLaunchKernelGGL(k_mean, dim3(1), dim3(1), 0, s, nb, L->lossb, dloss);
   // dL/d(pre-activation of the last layer) = coef_b * A d
   hipLaunchKernelGGL(k_scale_img, dim3(gg), dim3(kBlock), 0, s, npix, nb, L->coef, L->ad, L->gy);
+  if (L->sym) {
+    lp_backward_sym(L, nb, nx, ny, r);
+    return;
+  }
   // backward through the layers
   const int last = L->nl - 1;
   const void* dy = L->gy;  // FP64 for the last layer
@@ -714,10 +903,10 @@ void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const dou
   for (int l = last; l >= 0; l--) {
     const bool first = (l == 0), isl = (l == last);
     const void* x = first ? (const void*)r : (const void*)L->act[l];
-    if (first && isl) wgrad<1, 1>(L, l, nb, nx, ny, x, dy);
-    else if (first) wgrad<1, kC>(L, l, nb, nx, ny, x, dy);
-    else if (isl) wgrad<kC, 1>(L, l, nb, nx, ny, x, dy);
-    else wgrad<kC, kC>(L, l, nb, nx, ny, x, dy);
+    if (first && isl) wgrad<1, 1>(L, l, nb, nx, ny, x, L->isc, dy, 0);
+    else if (first) wgrad<1, kC>(L, l, nb, nx, ny, x, L->isc, dy, 0);
+    else if (isl) wgrad<kC, 1>(L, l, nb, nx, ny, x, L->isc, dy, 0);
+    else wgrad<kC, kC>(L, l, nb, nx, ny, x, L->isc, dy, 0);
     if (first) break;
     ConvArgs a{};
     a.nb = nb;
@@ -746,23 +935,17 @@ void adam_step(LearnedPrec* L, double lr)
   lp_pack(L);
 }
 
-}  // namespace
-
-extern "C" {
-
-LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx)
+LearnedPrec* lp_create(int nlayer, const int* dil, SUNContext ctx, bool sym)
 {
-  if (nlayer < 2 || nlayer > kMaxL || !dil || !ctx) return nullptr;
-  for (int l = 0; l < nlayer; l++)
-    if (dil[l] < 1 || dil[l] > 64) return nullptr;
   LearnedPrec* L = new LearnedPrec();
   L->nl = nlayer;
+  L->sym = sym;
   L->ex = ceda::context_exec(ctx);
   int off = 0;
   for (int l = 0; l < nlayer; l++) {
     L->dil[l] = dil[l];
     L->cin[l] = (l == 0) ? 1 : kC;
-    L->cout[l] = (l == nlayer - 1) ? 1 : kC;
+    L->cout[l] = (l == nlayer - 1 && !sym) ? 1 : kC;
     L->woff[l] = off;
     off += L->cout[l] * L->cin[l] * 9;
     L->boff[l] = off;
@@ -774,6 +957,7 @@ LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx)
   CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_m, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_v, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::set_sync(L->d_p, 0, sizeof(float) * off));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_g, 0, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::set_sync(L->d_m, 0, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::set_sync(L->d_v, 0, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_afw, sizeof(unsigned short) * 9 * 256 * nlayer));
@@ -785,6 +969,28 @@ LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx)
   return L;
 }
 
+}  // namespace
+
+extern "C" {
+
+LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx)
+{
+  if (nlayer < 2 || nlayer > kMaxL || !dil || !ctx) return nullptr;
+  for (int l = 0; l < nlayer; l++)
+    if (dil[l] < 1 || dil[l] > 64) return nullptr;
+  return lp_create(nlayer, dil, ctx, false);
+}
+
+LearnedPrec* LearnedPrec_CreateSymmetric(int nlayer, const int* dil, SUNContext ctx)
+{
+  if (nlayer < 1 || nlayer > kMaxL - 1 || !dil || !ctx) return nullptr;
+  for (int l = 0; l < nlayer; l++)
+    if (dil[l] < 1 || dil[l] > 64) return nullptr;
+  return lp_create(nlayer, dil, ctx, true);
+}
+
+int LearnedPrec_IsSymmetric(const LearnedPrec* L) { return L ? (L->sym ? 1 : 0) : -1; }
+
 void LearnedPrec_Destroy(LearnedPrec* L)
 {
   if (!L) return;
@@ -793,8 +999,10 @@ void LearnedPrec_Destroy(LearnedPrec* L)
                 L->ad, L->gy, L->sc, L->isc, L->part, L->lossb, L->coef, L->wpart};
   for (void* p : ps)
     if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
-  for (int l = 0; l < kMaxL; l++)
+  for (int l = 0; l < kMaxL; l++) {
     if (L->act[l]) CEDA_HIP_CHECK(ceda::dev_free(L->act[l]));
+    if (L->uact[l]) CEDA_HIP_CHECK(ceda::dev_free(L->uact[l]));
+  }
   delete L;
 }
 
@@ -828,7 +1036,8 @@ int LearnedPrec_InitIdentity(LearnedPrec* L, double noise, unsigned long seed)
     for (int k = 0; k < nw; k++) p[L->woff[l] + k] = (float)(noise * U(rng));
     auto W = [&](int co, int ci, int t) -> float& { return p[L->woff[l] + (co * L->cin[l] + ci) * 9 + t]; };
     const bool first = (l == 0), last = (l == L->nl - 1);
-    if (first && last) W(0, 0, 4) += 1.0f;
+    if (L->sym) W(0, 0, 4) += 1.0f;  // K passes channel 0 through: K^T K = I
+    else if (first && last) W(0, 0, 4) += 1.0f;
     else if (first) {
       W(0, 0, 4) += 1.0f;   // relu(x)
       W(1, 0, 4) -= 1.0f;   // relu(-x)

@@ -107,8 +107,9 @@ int LSRKStepGetMaxNumStages(void* arkode_mem, int* stage_max);
 int LSRKStepSetDiffusionFastPath(void* arkode_mem, DiffusionProblem* prob, int use_graphs);
 /* [ext] PCG (SUNLinSol_PCG, main.cpp:305) in three fused kernels per
    iteration for a DiffusionProblem that is also the user data, when J v is
-   the analytic JTimes and the preconditioner is PSolve, MGPSolve or none
-   (otherwise the N_Vector PCG runs).  Same algorithm and stopping test as
+   the analytic JTimes (otherwise the N_Vector PCG runs); Jacobi (PSolve) is
+   fused into the update kernel, any other preconditioner is called through
+   its callback.  Same algorithm and stopping test as
    the N_Vector PCG, not bitwise equal to it (the dot products are summed in
    another order).  prob = NULL switches it off. */
 int ARKodeSetDiffusionFastPCG(void* arkode_mem, DiffusionProblem* prob);

@@ -19,6 +19,19 @@
  *   N(r) = s * out_scale * (W_L * x_L + b_L)      (FP64 output)
  * Parameters (FP32, NumParams of them): for each layer W[cout][cin][3][3]
  * (row-major taps, dy outer) then b[cout].
+ *
+ * Symmetric network (LearnedPrec_CreateSymmetric).  nlayer layers K_l (1..7):
+ * layer 0 maps 1 -> 16 channels, the others 16 -> 16, no activation function
+ * and no biases (their slots stay in the layout and at zero); the network is
+ * K followed by its adjoint:
+ *   h_0 = bf16(r / s);  h_{l+1} = bf16(W_l * h_l);  u_nl = h_nl;
+ *   u_l = bf16(W_l^T * u_{l+1});  N(r) = s * out_scale * (W_0^T * u_1)
+ * where W^T * is the adjoint convolution (taps flipped, channels exchanged).
+ * N = K^T K is linear, symmetric and positive semi-definite up to the BF16
+ * roundings, which keeps every eigenvalue of the preconditioned operator
+ * positive (a ReLU network can map some residuals to near-zero or opposing
+ * corrections, which stalls the Krylov method).  With dilations 1, 3, 9 its
+ * receptive field is 53 x 53 nodes.
  * ------------------------------------------------------------------------- */
 #ifndef LEARNED_PREC_H
 #define LEARNED_PREC_H
@@ -33,6 +46,10 @@ extern "C" {
 typedef struct LearnedPrec LearnedPrec;
 
 LearnedPrec* LearnedPrec_Create(int nlayer, const int* dil, SUNContext ctx);
+/* the symmetric network N = K^T K of nlayer linear layers (see above) */
+LearnedPrec* LearnedPrec_CreateSymmetric(int nlayer, const int* dil, SUNContext ctx);
+/* 1 for a symmetric network, 0 otherwise, -1 for NULL */
+int LearnedPrec_IsSymmetric(const LearnedPrec* lp);
 void LearnedPrec_Destroy(LearnedPrec* lp);
 int LearnedPrec_NumParams(const LearnedPrec* lp);
 /* host FP32 parameters in the layout above */
@@ -40,7 +57,8 @@ int LearnedPrec_SetParams(LearnedPrec* lp, const float* params);
 int LearnedPrec_GetParams(const LearnedPrec* lp, float* params);
 /* deterministic start: the identity map through the ReLUs (channel 0 carries
    relu(x), channel 1 relu(-x), the last layer takes their difference) plus
-   uniform noise of the given amplitude (seeded) on every weight */
+   uniform noise of the given amplitude (seeded) on every weight; for the
+   symmetric network every layer passes channel 0 through (K^T K = I) */
 int LearnedPrec_InitIdentity(LearnedPrec* lp, double noise, unsigned long seed);
 /* multiplier of the network output (1/diag of the operator it is trained for) */
 int LearnedPrec_SetOutputScale(LearnedPrec* lp, double out_scale);

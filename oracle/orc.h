@@ -135,6 +135,11 @@ int orc_lp_num_params(int nl, const int* dil);
 double orc_lp_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,
                        int ny, double ratio, const double* r, const double* e, int emu, double* grad,
                        double* u_out);
+/* the symmetric network N = K^T K (linear layers, no biases) */
+int orc_lp_sym_num_params(int nl, const int* dil);
+double orc_lp_sym_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,
+                           int ny, double ratio, const double* r, const double* e, int emu, double* grad,
+                           double* u_out);
 
 #ifdef __cplusplus
 }

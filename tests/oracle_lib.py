@@ -216,17 +216,25 @@ def lp_num_params(dil):
     return f(len(d), d.ctypes.data_as(C.POINTER(C.c_int)))
 
 
-def lp_lossgrad(dil, params, out_scale, nb, nx, ny, ratio, r, e, emu=1, want_grad=True):
+def lp_num_params_sym(dil):
+    d = np.ascontiguousarray(dil, dtype=np.int32)
+    f = lib().orc_lp_sym_num_params
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    return f(len(d), d.ctypes.data_as(C.POINTER(C.c_int)))
+
+
+def lp_lossgrad(dil, params, out_scale, nb, nx, ny, ratio, r, e, emu=1, want_grad=True, sym=False):
     """Loss, parameter gradient and normalised network output of the learned
     preconditioner restated on the CPU (emu=1: BF16 roundings of the GPU path
-    emulated; emu=0: exact FP64)."""
+    emulated; emu=0: exact FP64; sym: the symmetric network K^T K)."""
     d = np.ascontiguousarray(dil, dtype=np.int32)
     p = np.ascontiguousarray(params, dtype=np.float32)
     r = np.ascontiguousarray(r, dtype=np.float64)
     e = np.ascontiguousarray(e, dtype=np.float64)
     g = np.zeros(len(p)) if want_grad else None
     u = np.zeros(nb * nx * ny)
-    f = lib().orc_lp_lossgrad
+    f = lib().orc_lp_sym_lossgrad if sym else lib().orc_lp_lossgrad
     f.restype = C.c_double
     f.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_double, C.c_int, C.c_int,
                   C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,

@@ -204,3 +204,116 @@ def test_flexible_pcg(pk, ctx):
     assert fl["flag"] == 0 and fl["fallbacks"] == 0
     assert np.abs(fl["y"] - ref["y"]).max() <= 1e-6 * scale
     net.release()
+
+
+# ------------------------------------------ the symmetric network N = K^T K
+SDIL = (1, 3, 9)
+
+
+@pytest.mark.parametrize("dil,nb,nx,ny", [((1, 2), 1, 16, 16), (SDIL, 3, 37, 21), (SDIL, 2, 64, 9)])
+def test_sym_forward_matches_oracle(pk, ctx, dil, nb, nx, ny):
+    net = pk.LearnedPrec(ctx, dil, symmetric=True)
+    assert pk.lib().LearnedPrec_IsSymmetric(net.h) == 1
+    assert net.nparams == O.lp_num_params_sym(dil)
+    p = random_params(net.nparams, nb * nx, 0.2)
+    net.set_params(p)
+    net.set_output_scale(0.3)
+    rng = np.random.default_rng(17)
+    r = rng.standard_normal(nb * nx * ny)
+    R, Z = pk.Vector(ctx, n=r.size), pk.Vector(ctx, n=r.size)
+    R.set(r)
+    assert net.apply(nb, nx, ny, R, Z) == 0
+    z = Z.get()
+    s = np.sqrt((r.reshape(nb, -1) ** 2).mean(axis=1))
+    _, _, u = O.lp_lossgrad(dil, p, 0.3, nb, nx, ny, 1.0, r, r, emu=1, want_grad=False, sym=True)
+    ref = u.reshape(nb, -1) * s[:, None]
+    err = z.reshape(nb, -1) - ref
+    assert np.abs(err).max() <= 1e-2 * np.abs(ref).max()
+    assert np.sqrt((err ** 2).mean()) <= 2e-3 * np.sqrt((ref ** 2).mean())
+    net.release()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_sym_loss_and_gradient_match_oracle(pk, ctx, seed):
+    nb, nx, ny, ratio = 2, 33, 20, 40.0
+    net = pk.LearnedPrec(ctx, SDIL, symmetric=True)
+    if seed == 0:
+        net.init_identity(0.05, 3)
+        p = net.params()
+    else:
+        p = random_params(net.nparams, 12, 0.15)
+        net.set_params(p)
+    net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
+    rng = np.random.default_rng(seed + 5)
+    e = rng.standard_normal(nb * nx * ny)
+    r = apply_A(e, nb, nx, ny, ratio)
+    R, E = pk.Vector(ctx, n=r.size), pk.Vector(ctx, n=r.size)
+    R.set(r)
+    E.set(e)
+    loss, g = net.loss_grad(nb, nx, ny, ratio, R, E)
+    lref, gref, _ = O.lp_lossgrad(SDIL, p, 1.0 / (1.0 + 4.0 * ratio), nb, nx, ny, ratio, r, e, emu=1, sym=True)
+    assert abs(loss - lref) <= 1e-3 * abs(lref)
+    assert np.linalg.norm(g - gref) <= 1e-2 * np.linalg.norm(gref)
+    # the unused bias slots get no gradient
+    off = 0
+    for l in range(len(SDIL)):
+        off += 16 * (1 if l == 0 else 16) * 9
+        assert np.all(g[off:off + 16] == 0.0)
+        off += 16
+    net.release()
+
+
+def test_sym_network_is_symmetric_and_positive(pk, ctx):
+    nx, ny = 48, 40
+    net = pk.LearnedPrec(ctx, SDIL, symmetric=True)
+    net.set_params(random_params(net.nparams, 21, 0.2))
+    rng = np.random.default_rng(22)
+    xs = [rng.standard_normal(nx * ny) for _ in range(2)]
+    outs = []
+    for x in xs:
+        X, Z = pk.Vector(ctx, n=x.size), pk.Vector(ctx, n=x.size)
+        X.set(x)
+        assert net.apply(1, nx, ny, X, Z) == 0
+        outs.append(Z.get())
+    a, b = xs[0] @ outs[1], xs[1] @ outs[0]
+    # symmetric up to the BF16 roundings of the activations
+    assert abs(a - b) <= 2e-2 * np.linalg.norm(xs[0]) * np.linalg.norm(outs[1])
+    for x, z in zip(xs, outs):
+        assert x @ z > 0.0
+    net.release()
+
+
+def trained_sym_net(pk, ctx, ratio, n=64, nsamp=2048, steps=1500, batch=16, seed=0, lr=3e-3):
+    R, E, _, rc = pk.training_set(ctx, nsamp, n, ratio, reltol=1e-12, seed=1234)
+    assert rc == 0
+    net = pk.LearnedPrec(ctx, SDIL, symmetric=True)
+    net.init_identity(8e-4, seed)
+    net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
+    hist = net.train(nsamp, n, ratio, R, E, steps, batch, lr, seed)
+    return net, hist
+
+
+def test_sym_learned_krylov_on_the_implicit_path(pk, ctx):
+    """The trained symmetric network inside FGMRES and inside PCG reaches the
+    Jacobi-PCG solution with far fewer Krylov iterations; the untrained one
+    (K^T K = I, i.e. Jacobi) does not gain."""
+    side, dt = 128, 0.02
+    dx = 1.0 / (side + 1)
+    ratio = 1767732205903.0 / 4055673282236.0 * dt / dx ** 2
+    net, hist = trained_sym_net(pk, ctx, ratio)
+    assert np.all(np.isfinite(hist)) and hist[-50:].mean() < 0.5 * hist[:10].mean()
+    ident = pk.LearnedPrec(ctx, SDIL, symmetric=True)
+    ident.init_identity(0.0, 0)
+    ref = dirk_run(pk, ctx, "jacobi", 0.06, dt, side, jtimes=True)
+    fg = dirk_run(pk, ctx, "learned", 0.06, dt, side, net=net, jtimes=True)
+    cg = dirk_run(pk, ctx, "learned", 0.06, dt, side, net=net, ls="cg", jtimes=True)
+    idn = dirk_run(pk, ctx, "learned", 0.06, dt, side, net=ident, jtimes=True)
+    scale = np.abs(ref["y"]).max()
+    for o in (fg, cg, idn):
+        assert o["flag"] == 0 and o["fallbacks"] == 0
+        assert np.abs(o["y"] - ref["y"]).max() <= 1e-6 * scale
+    nj, nf, nc, ni = (o["stats"]["nli"] for o in (ref, fg, cg, idn))
+    assert nf < 0.35 * nj and nc < 0.35 * nj, (nj, nf, nc)
+    assert nf < 0.35 * ni, (nf, ni)
+    net.release()
+    ident.release()

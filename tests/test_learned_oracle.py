@@ -81,3 +81,110 @@ def test_oracle_bf16_emulation_is_a_small_perturbation():
     rel = np.abs(u1 - u0).max() / np.abs(u0).max()
     assert 0.0 < rel < 0.05
     assert np.linalg.norm(g1 - g0) / np.linalg.norm(g0) < 0.05
+
+
+# ----------------------------------------------------- the symmetric network
+def dense_conv(W, d, nx, ny):
+    """The dense matrix of one zero-padded 3x3 convolution (dilation d) from
+    [ny][nx][cin] to [ny][nx][cout], built entry by entry."""
+    cout, cin = W.shape[:2]
+    M = np.zeros((ny * nx * cout, ny * nx * cin))
+    for y in range(ny):
+        for x in range(nx):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    ys, xs = y + dy * d, x + dx * d
+                    if 0 <= ys < ny and 0 <= xs < nx:
+                        for co in range(cout):
+                            for ci in range(cin):
+                                M[(y * nx + x) * cout + co, (ys * nx + xs) * cin + ci] += W[co, ci, dy + 1, dx + 1]
+    return M
+
+
+def sym_problem(seed, dil, nb=2, nx=7, ny=6):
+    rng = np.random.default_rng(seed)
+    n = O.lp_num_params_sym(dil)
+    params = (0.3 * rng.standard_normal(n)).astype(np.float32)
+    r = rng.standard_normal(nb * nx * ny)
+    e = rng.standard_normal(nb * nx * ny) + 0.5 * r
+    return params, r, e
+
+
+def sym_matrix(dil, params, nx, ny):
+    """K^T K from the dense layer matrices and numpy's transpose (independent
+    of the oracle's adjoint convolution)."""
+    K = None
+    off = 0
+    for l, d in enumerate(dil):
+        cin = 1 if l == 0 else 16
+        W = params[off:off + 16 * cin * 9].reshape(16, cin, 3, 3).astype(np.float64)
+        off += 16 * cin * 9 + 16
+        M = dense_conv(W, d, nx, ny)
+        K = M if K is None else M @ K
+    return K.T @ K
+
+
+@pytest.mark.parametrize("dil", [(1,), (1, 2), (1, 2, 3)])
+def test_sym_oracle_forward_matches_dense_adjoint(dil):
+    nx, ny, nb = 7, 6, 2
+    params, r, e = sym_problem(4, dil, nb, nx, ny)
+    assert O.lp_num_params_sym(dil) == sum(16 * (1 if l == 0 else 16) * 9 + 16 for l in range(len(dil)))
+    _, _, u = O.lp_lossgrad(dil, params, 0.7, nb, nx, ny, 3.0, r, e, emu=0, want_grad=False, sym=True)
+    M = sym_matrix(dil, params, nx, ny)
+    for b in range(nb):
+        x = r[b * nx * ny:(b + 1) * nx * ny]
+        s = np.sqrt((x ** 2).mean())
+        ref = 0.7 * (M @ (x / s))
+        got = u[b * nx * ny:(b + 1) * nx * ny]
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_sym_oracle_is_symmetric_and_positive():
+    dil = (1, 2)
+    nx, ny = 9, 8
+    params, _, _ = sym_problem(5, dil, 1, nx, ny)
+    rng = np.random.default_rng(6)
+    outs = []
+    xs = [rng.standard_normal(nx * ny) for _ in range(2)]
+    for x in xs:
+        _, _, u = O.lp_lossgrad(dil, params, 1.0, 1, nx, ny, 1.0, x, x, emu=0, want_grad=False, sym=True)
+        outs.append(u * np.sqrt((x ** 2).mean()))  # undo the normalisation: N x
+    a, b = xs[0] @ outs[1], xs[1] @ outs[0]
+    assert abs(a - b) <= 1e-12 * max(abs(a), abs(b))
+    assert xs[0] @ outs[0] > 0 and xs[1] @ outs[1] > 0
+
+
+@pytest.mark.parametrize("dil", [(1,), (1, 2), (1, 3, 2)])
+def test_sym_oracle_gradient_matches_finite_differences(dil):
+    params, r, e = sym_problem(7, dil)
+    ratio = 5.0
+    loss, g, _ = O.lp_lossgrad(dil, params, 0.4, 2, 7, 6, ratio, r, e, emu=0, sym=True)
+    nw = [16 * (1 if l == 0 else 16) * 9 for l in range(len(dil))]
+    # the bias slots are not used: zero gradient
+    off = 0
+    for w in nw:
+        assert np.all(g[off + w:off + w + 16] == 0.0)
+        off += w + 16
+    rng = np.random.default_rng(8)
+    ks = rng.choice(len(params), size=min(len(params), 160), replace=False)
+    for k in ks:
+        h = 1e-3 * max(1.0, abs(float(params[k])))
+        pp, pm = params.copy(), params.copy()
+        pp[k] = np.float32(params[k] + h)
+        pm[k] = np.float32(params[k] - h)
+        hh = float(pp[k]) - float(pm[k])
+        lp_, _, _ = O.lp_lossgrad(dil, pp, 0.4, 2, 7, 6, ratio, r, e, emu=0, want_grad=False, sym=True)
+        lm_, _, _ = O.lp_lossgrad(dil, pm, 0.4, 2, 7, 6, ratio, r, e, emu=0, want_grad=False, sym=True)
+        fd = (lp_ - lm_) / hh
+        # smooth (polynomial) loss: central differences to O(h^2)
+        assert abs(fd - g[k]) <= 1e-5 * np.abs(g).max() + 1e-6 * abs(g[k]), (k, fd, g[k])
+
+
+def test_sym_oracle_bf16_emulation_is_a_small_perturbation():
+    dil = (1, 2)
+    params, r, e = sym_problem(9, dil)
+    l0, g0, u0 = O.lp_lossgrad(dil, params, 0.5, 2, 7, 6, 2.0, r, e, emu=0, sym=True)
+    l1, g1, u1 = O.lp_lossgrad(dil, params, 0.5, 2, 7, 6, 2.0, r, e, emu=1, sym=True)
+    rel = np.abs(u1 - u0).max() / np.abs(u0).max()
+    assert 0.0 < rel < 0.05
+    assert np.linalg.norm(g1 - g0) / np.linalg.norm(g0) < 0.05
