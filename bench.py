@@ -260,10 +260,10 @@ def implicit_workload(pk, args):
     """The implicit path: mms-dirichlet 2048^2, kx = ky = 1, ARKStep DIRK3
     (ARK3(2)4L[2]SA), fixed dt = 1e-3, linear solves to relative residual
     1e-8: Jacobi-PCG against FGMRES with the learned preconditioner (Jacobi-PCG
-    fallback at the iteration cap).  The network (4 layers of 3x3
-    convolutions, 16 channels, BF16) is trained on the GPU first, on pairs
-    from Jacobi-PCG solves on 64^2 grids with the same gamma*dt/dx^2, outside
-    the timed solves.  A prefix of `--isteps` steps is timed for each method;
+    fallback at the iteration cap).  The networks (4 layers of 3x3
+    convolutions, 16 channels, BF16: the symmetric K^T K with dilations
+    1,3,9,27 and the ReLU network) are trained on the GPU first, on pairs from
+    Jacobi-PCG solves with the same gamma*dt/dx^2, outside the timed solves.  A prefix of `--isteps` steps is timed for each method;
     errors are L2 against the exact solution and between the two runs."""
     import numpy as np
 
@@ -430,25 +430,34 @@ def c5_workload(pk, args):
     cells = sum(p["nx"] * p["ny"] for p in probs)
     log(f"c5: {len(probs)} problems, {cells} cells, sides {min(p['nx'] for p in probs)}..."
         f"{max(p['nx'] for p in probs)}, gamma*dt*k/dx^2 {min(ratios):.0f}..{max(ratios):.0f}, {steps} steps")
-    shared = None
+    methods = args.c5_methods.split(",")
+    sdil = tuple(int(x) for x in args.sym_dil.split(","))
+    # network shape per learned method: (dilations, symmetric)
+    shapes = {"learned": ((1, 1, 1, 1), False), "learned-sym": (sdil, True)}
+    shared = {}
     train = {}
-    if "learned" in args.c5_methods:
+    for kind in methods:
+        if kind not in shapes:
+            continue
+        dil, sym = shapes[kind]
         ctx = pk.Context()
         ratio = float(np.median(ratios))
+        tn = args.sym_train_n if sym else args.train_n
+        steps = args.sym_train_steps if sym else args.train_steps
+        lr = args.sym_train_lr if sym else args.train_lr
         t0 = time.perf_counter()
-        R, E, iters, rc = pk.training_set(ctx, args.nsamp, args.train_n, ratio, reltol=1e-12, seed=1234)
+        R, E, iters, rc = pk.training_set(ctx, args.nsamp, tn, ratio, reltol=1e-12, seed=1234)
         assert rc == 0
-        net = pk.LearnedPrec(ctx, (1, 1, 1, 1))
+        net = pk.LearnedPrec(ctx, dil, symmetric=sym)
         net.init_identity(8e-4, 0)
         net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
-        hist = net.train(args.nsamp, args.train_n, ratio, R, E, args.train_steps, args.train_batch,
-                         args.train_lr, 0)
+        hist = net.train(args.nsamp, tn, ratio, R, E, steps, args.train_batch, lr, 0)
         hip_sync()
-        shared = net.params()
-        train = dict(ratio=ratio, pairs=args.nsamp, grid=args.train_n, steps=args.train_steps,
-                     seconds=time.perf_counter() - t0, loss_first20=float(hist[:20].mean()),
-                     loss_last50=float(hist[-50:].mean()))
-        log(f"shared network trained at ratio {ratio:.0f}: {train}")
+        shared[kind] = net.params()
+        train[kind] = dict(ratio=ratio, pairs=args.nsamp, grid=tn, steps=steps, lr=lr, dilations=list(dil),
+                           symmetric=sym, seconds=time.perf_counter() - t0,
+                           loss_first20=float(hist[:20].mean()), loss_last50=float(hist[-50:].mean()))
+        log(f"shared {kind} network trained at ratio {ratio:.0f}: {train[kind]}")
         ctx.close()
 
     def solve_one(ctx, net, k, kind):
@@ -457,14 +466,14 @@ def c5_workload(pk, args):
         ls, prec, maxl = "cg", True, 20000
         if kind == "mg":
             prec = "mg"
-        elif kind == "learned":
+        elif kind in shapes:
             dx2 = 1.0 / (p["nx"] + 1) ** 2
             net.set_output_scale(1.0 / (1.0 + 2.0 * gamma_a * dt * (p["kx"] + p["ky"]) / dx2))
             net.attach(G)
             ls, prec, maxl = "fgmres", "learned", args.imaxl
         I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=maxl, precond=prec)
         if ls == "fgmres":
-            pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, args.mrestart)
+            pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, args.sym_mrestart if kind == "learned-sym" else args.mrestart)
         I.analytic_jtimes()
         if args.fused_pcg:
             I.fused_pcg()
@@ -492,9 +501,9 @@ def c5_workload(pk, args):
         def worker():
             ctx = pk.Context()
             net = None
-            if kind == "learned":
-                net = pk.LearnedPrec(ctx, (1, 1, 1, 1))
-                net.set_params(shared)
+            if kind in shapes:
+                net = pk.LearnedPrec(ctx, shapes[kind][0], symmetric=shapes[kind][1])
+                net.set_params(shared[kind])
             try:
                 while True:
                     with lock:
@@ -504,7 +513,7 @@ def c5_workload(pk, args):
                     recs[k] = solve_one(ctx, net, k, kind)
                     with lock:
                         done = sum(r is not None for r in recs)
-                    if done % 8 == 0:
+                    if done % 8 == 0 or workers == 1:
                         log(f"  {kind} x{workers}: {done}/{len(probs)} problems")
             except Exception as e:  # reported after the join
                 with lock:
@@ -533,7 +542,7 @@ def c5_workload(pk, args):
 
     res = {}
     errs = {}
-    for kind in args.c5_methods.split(","):
+    for kind in methods:
         res[kind], errs[kind] = run(kind, args.c5_workers)
     if "jacobi" in res:
         res["jacobi_one_worker"], _ = run("jacobi", 1)
@@ -602,12 +611,12 @@ def main():
     ap.add_argument("--dil", default="1,1,1,1", help="dilations of the network's 3x3 layers")
     ap.add_argument("--fused-pcg", action="store_true",
                     help="implicit/c5: PCG in three fused kernels per iteration (with the analytic J v)")
-    ap.add_argument("--sym-dil", default="1,3,9", help="dilations of the symmetric network's layers")
+    ap.add_argument("--sym-dil", default="1,3,9,27", help="dilations of the symmetric network's layers")
     ap.add_argument("--sym-train-n", type=int, default=128, help="training grid side, symmetric network")
-    ap.add_argument("--sym-train-steps", type=int, default=3000)
+    ap.add_argument("--sym-train-steps", type=int, default=1000)
     ap.add_argument("--sym-train-lr", type=float, default=3e-3)
     ap.add_argument("--sym-mrestart", type=int, default=20, help="FGMRES restart with the symmetric network")
-    ap.add_argument("--imethods", default="jacobi,mg,learned,learned-fcg",
+    ap.add_argument("--imethods", default="jacobi,mg,learned-sym,learned-sym-pcg,learned",
                     help="implicit: methods to run (jacobi first)")
     ap.add_argument("--imaxl", type=int, default=3000, help="FGMRES iteration cap before the PCG fallback")
     ap.add_argument("--nprob", type=int, default=256)

@@ -13,9 +13,12 @@
 // A operand (packed once per layer, loaded once per wavefront), the B operand
 // is 4 channels of pixel n read as one 8-byte load from the NHWC activation
 // (16 lanes cover 512 contiguous bytes), and each lane's D registers are 4
-// consecutive channels of one pixel: one 8-byte store.  Activations are NHWC
-// with 16 BF16 channels (32 bytes per pixel); the network input and output
-// are FP64 images.
+// consecutive channels of one pixel: one 8-byte store.  Activations are 16
+// BF16 channels per pixel in a blocked layout: rows padded to whole blocks of
+// 16 pixels, and within a block the 4 groups of 4 channels one after another
+// (16 pixels x 8 bytes each), so the 16 lanes of a channel group read or
+// write 128 contiguous bytes (with plain NHWC they would touch 4 cache lines
+// at a 32-byte stride).  The network input and output are FP64 images.
 //
 // Backward.  The input gradient of a layer is the same convolution with the
 // taps flipped and the channel roles exchanged (a separately packed A
@@ -58,6 +61,14 @@ __device__ inline unsigned short f2bf(float f)
   return (unsigned short)(u >> 16);
 }
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// offset (in BF16 elements) of channels 4g..4g+3 of pixel x of image row
+// `row` (image b, y: row = b*ny + y) in the blocked activation layout with
+// bpr blocks of 16 pixels per row
+__device__ inline int64_t act_off(int64_t row, int bpr, int x, int g)
+{
+  return ((row * bpr + (x >> 4)) * 4 + g) * 64 + (x & 15) * 4;
+}
 __device__ inline unsigned short d2bf(double x) { return f2bf((float)x); }
 
 
@@ -121,15 +132,14 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
     // afterwards, so the loads of all kConvU blocks are in flight together
     // instead of one branch and one wait per tap.
     if (CIN == kC && interior) {
-      const unsigned short* p =
-          static_cast<const unsigned short*>(a.in) + ((img + y) * nx + bx0 * 16 + n) * kC + 4 * g;
+      const unsigned short* in = static_cast<const unsigned short*>(a.in);
       bfx4 Bt[kConvU][9];
 #pragma unroll
       for (int u = 0; u < kConvU; u++)
 #pragma unroll
         for (int t = 0; t < 9; t++)
           Bt[u][t] = *reinterpret_cast<const bfx4*>(
-              p + ((int64_t)(t / 3 - 1) * d * nx + (t % 3 - 1) * d + u * 16) * kC);
+              in + act_off(img + y + (t / 3 - 1) * d, bpr, (bx0 + u) * 16 + n + (t % 3 - 1) * d, g));
 #pragma unroll
       for (int u = 0; u < kConvU; u++) {
         acc[u] = fx4{0.f, 0.f, 0.f, 0.f};
@@ -146,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
         for (int t = 0; t < 9; t++) {
           const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
           const int xc = min(max(xs, 0), nx - 1), yc = min(max(ys, 0), ny - 1);
-          Bt[u][t] = *reinterpret_cast<const bfx4*>(in + (((img + yc) * nx + xc) * kC + 4 * g));
+          Bt[u][t] = *reinterpret_cast<const bfx4*>(in + act_off(img + yc, bpr, xc, g));
         }
       }
 #pragma unroll
@@ -201,10 +211,10 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
         for (int i = 0; i < 4; i++) {
           float v = acc[u][i] + bv[i];
           if (RELU) v = fmaxf(v, 0.0f);
-          if (a.mask && !(bf2f(a.mask[pix * kC + 4 * g + i]) > 0.0f)) v = 0.0f;
+          if (a.mask && !(bf2f(a.mask[act_off(img + y, bpr, x, g) + i]) > 0.0f)) v = 0.0f;
           o[i] = (short)f2bf(v);
         }
-        *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + pix * kC + 4 * g) = o;
+        *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + act_off(img + y, bpr, x, g)) = o;
       } else if (g == 0) {
         const double v = (double)(acc[u][0] + bv[0]) * a.out_mul * (a.scale ? a.scale[bimg] : 1.0);
         double* out = static_cast<double*>(a.out);
@@ -250,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const WGradArgs a)
       unsigned short v = 0;
       if (px < nx) {
         const int64_t pix = (b * ny + y) * nx + px;
-        if (COUT == kC) v = static_cast<const unsigned short*>(a.dy)[pix * kC + n];
+        if (COUT == kC) v = static_cast<const unsigned short*>(a.dy)[act_off(b * ny + y, bpr, px, n >> 2) + (n & 3)];
         else if (n == 0) v = d2bf(static_cast<const double*>(a.dy)[pix]);
       }
       Aop[j] = (short)v;
@@ -267,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_wgrad(const WGradArgs a)
           const int xs = px + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
           unsigned short v = 0;
           if (px < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny)
-            v = xin[((b * ny + ys) * nx + xs) * kC + n];
+            v = xin[act_off(b * ny + ys, bpr, xs, n >> 2) + (n & 3)];
           Bop[j] = (short)v;
         }
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Aop, Bop, acc[t], 0, 0, 0);
@@ -641,7 +651,7 @@ struct LearnedPrec {
   unsigned short *d_afw = nullptr, *d_abw = nullptr;  // [layer][9*256]
   int adam_t = 0;
   // workspace
-  int64_t cap_pix = 0;
+  int64_t cap_pix = 0, cap_act = 0;
   int cap_nb = 0;
   unsigned short* act[kMaxL] = {};  // act[l] = input of layer l (l >= 1), NHWC16; sym: act[nl] = K x
   unsigned short* uact[kMaxL] = {}; // sym: uact[l] = input of the adjoint of layer l - 1
@@ -667,11 +677,16 @@ void lp_pack(LearnedPrec* L)
 constexpr int kMaxParts = 512;  // workgroups per image of the per-image reductions
 int rms_parts(int64_t npix) { return (int)std::min<int64_t>(kMaxParts, (npix + 8191) / 8192); }
 
-void lp_workspace(LearnedPrec* L, int nb, int64_t npix)
+// activation buffers hold nb*ny rows of whole 16-pixel blocks
+int64_t act_elems(int nb, int nx, int ny) { return (int64_t)nb * ny * ((nx + 15) / 16) * 16 * kC; }
+
+void lp_workspace(LearnedPrec* L, int nb, int nx, int ny)
 {
+  const int64_t npix = (int64_t)nx * ny;
   const int64_t tot = npix * nb;
+  const int64_t ae = act_elems(nb, nx, ny);
   const int P = rms_parts(npix);
-  if (tot > L->cap_pix) {
+  if (tot > L->cap_pix || ae > L->cap_act) {
     CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
     for (int l = 0; l < kMaxL; l++) {
       if (L->act[l]) CEDA_HIP_CHECK(ceda::dev_free(L->act[l]));
@@ -682,16 +697,17 @@ void lp_workspace(LearnedPrec* L, int nb, int64_t npix)
     for (void* p : ps)
       if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
     for (int l = 1; l < L->nl + (L->sym ? 1 : 0); l++)
-      CEDA_HIP_CHECK(ceda::dev_malloc(&L->act[l], sizeof(unsigned short) * kC * tot));
+      CEDA_HIP_CHECK(ceda::dev_malloc(&L->act[l], sizeof(unsigned short) * ae));
     if (L->sym)
-      for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(ceda::dev_malloc(&L->uact[l], sizeof(unsigned short) * kC * tot));
-    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gA, sizeof(unsigned short) * kC * tot));
-    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gB, sizeof(unsigned short) * kC * tot));
+      for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(ceda::dev_malloc(&L->uact[l], sizeof(unsigned short) * ae));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gA, sizeof(unsigned short) * ae));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gB, sizeof(unsigned short) * ae));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->u, sizeof(double) * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->dd, sizeof(double) * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->ad, sizeof(double) * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->gy, sizeof(double) * tot));
     L->cap_pix = tot;
+    L->cap_act = ae;
   }
   if (nb > L->cap_nb || nb * P * 2 > L->part_cap) {
     CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
@@ -879,7 +895,7 @@ void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const dou
 {
   hipStream_t s = L->ex->stream;
   const int64_t npix = (int64_t)nx * ny, tot = npix * nb;
-  lp_workspace(L, nb, npix);
+  lp_workspace(L, nb, nx, ny);
   lp_scale(L, nb, npix, r);
   lp_forward(L, nb, nx, ny, r, L->u, 0, true);
   const unsigned gg = grid_for(tot);
@@ -1068,7 +1084,7 @@ int LearnedPrec_Apply(LearnedPrec* L, int nb, int nx, int ny, const double* in, 
 {
   if (!L || nb < 1 || nx < 1 || ny < 1 || !in || !out) return -1;
   const int64_t npix = (int64_t)nx * ny;
-  lp_workspace(L, nb, npix);
+  lp_workspace(L, nb, nx, ny);
   lp_scale(L, nb, npix, in);
   lp_forward(L, nb, nx, ny, in, out, addto, false);
   return 0;
