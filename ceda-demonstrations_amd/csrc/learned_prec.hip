@@ -49,9 +49,11 @@ constexpr int kMaxL = 8;      // layers
 constexpr int kWavesPB = kBlock / kWave;
 constexpr int kConvU = 2;     // 16-pixel blocks per wavefront step of the convolution
 constexpr int kPW = 9 * 256 + 16;  // weight-gradient partial of one wavefront
+constexpr int kPack = 5 * 512;      // packed A operands of one layer (BF16 elements)
 constexpr int kWGradWaves = 1024;
 
 typedef short bfx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float fx4 __attribute__((ext_vector_type(4)));
 
 __device__ inline unsigned short f2bf(float f)
@@ -62,12 +64,17 @@ __device__ inline unsigned short f2bf(float f)
 }
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-// offset (in BF16 elements) of channels 4g..4g+3 of pixel x of image row
-// `row` (image b, y: row = b*ny + y) in the blocked activation layout with
-// bpr blocks of 16 pixels per row
+// Blocked activation layout: image rows (row = b*ny + y) of bpr blocks of 16
+// pixels; in a block, channels 0-7 of the 16 pixels, then channels 8-15
+// (16 bytes per pixel and half).  chunk_off: offset (in BF16 elements) of
+// channels 8h..8h+7 of pixel x; act_off: of channels 4g..4g+3.
+__device__ inline int64_t chunk_off(int64_t row, int bpr, int x, int h)
+{
+  return ((row * bpr + (x >> 4)) * 2 + h) * 128 + (x & 15) * 8;
+}
 __device__ inline int64_t act_off(int64_t row, int bpr, int x, int g)
 {
-  return ((row * bpr + (x >> 4)) * 4 + g) * 64 + (x & 15) * 4;
+  return chunk_off(row, bpr, x, g >> 1) + 4 * (g & 1);
 }
 __device__ inline unsigned short d2bf(double x) { return f2bf((float)x); }
 
@@ -90,10 +97,17 @@ template <int CIN, int COUT, bool RELU>
 __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
 {
   const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, n = lane & 15;
-  constexpr int NT = (CIN == kC) ? 9 : 1;
-  bfx4 A[NT];
+  // CIN = 16: v_mfma_f32_16x16x32_bf16 over tap pairs, K = (tap 2p + g/2,
+  // channels 8(g%2)..+7) for lane group g; the 10th tap is zero.  CIN = 1:
+  // v_mfma_f32_16x16x16bf16_1k with the 9 taps on K.
+  bf16x8 A8[5];
+  bfx4 A1 = {0, 0, 0, 0};
+  if (CIN == kC) {
 #pragma unroll
-  for (int t = 0; t < NT; t++) A[t] = reinterpret_cast<const bfx4*>(a.apack)[t * kWave + lane];
+    for (int p = 0; p < 5; p++) A8[p] = __builtin_bit_cast(bf16x8, reinterpret_cast<const uint4*>(a.apack)[p * kWave + lane]);
+  } else {
+    A1 = reinterpret_cast<const bfx4*>(a.apack)[lane];
+  }
   float bv[4] = {0.f, 0.f, 0.f, 0.f};
   if (a.bias) {
     if (COUT == kC) {
@@ -131,32 +145,22 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
     // (in-bounds) address and the zero padding applied by a select
     // afterwards, so the loads of all kConvU blocks are in flight together
     // instead of one branch and one wait per tap.
-    if (CIN == kC && interior) {
+    if (CIN == kC) {
+      // lane group g reads tap 2p + g/2 (channels 8(g%2)..+7, one 16-byte
+      // load per pixel); interior segments need no clamp
       const unsigned short* in = static_cast<const unsigned short*>(a.in);
-      bfx4 Bt[kConvU][9];
-#pragma unroll
-      for (int u = 0; u < kConvU; u++)
-#pragma unroll
-        for (int t = 0; t < 9; t++)
-          Bt[u][t] = *reinterpret_cast<const bfx4*>(
-              in + act_off(img + y + (t / 3 - 1) * d, bpr, (bx0 + u) * 16 + n + (t % 3 - 1) * d, g));
-#pragma unroll
-      for (int u = 0; u < kConvU; u++) {
-        acc[u] = fx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int t = 0; t < 9; t++) acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[t], Bt[u][t], acc[u], 0, 0, 0);
-      }
-    } else if (CIN == kC) {
-      const unsigned short* in = static_cast<const unsigned short*>(a.in);
-      bfx4 Bt[kConvU][9];
+      const int h = g & 1;
+      uint4 Bt[kConvU][5];
 #pragma unroll
       for (int u = 0; u < kConvU; u++) {
         const int x = (bx0 + u) * 16 + n;
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
+        for (int p = 0; p < 5; p++) {
+          const int t = min(2 * p + (g >> 1), 8);
           const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
-          const int xc = min(max(xs, 0), nx - 1), yc = min(max(ys, 0), ny - 1);
-          Bt[u][t] = *reinterpret_cast<const bfx4*>(in + act_off(img + yc, bpr, xc, g));
+          const int xc = interior ? xs : min(max(xs, 0), nx - 1);
+          const int yc = interior ? ys : min(max(ys, 0), ny - 1);
+          Bt[u][p] = *reinterpret_cast<const uint4*>(in + chunk_off(img + yc, bpr, xc, h));
         }
       }
 #pragma unroll
@@ -164,11 +168,13 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
         const int x = (bx0 + u) * 16 + n;
         acc[u] = fx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t = 0; t < 9; t++) {
+        for (int p = 0; p < 5; p++) {
+          const int t = 2 * p + (g >> 1);
           const int xs = x + (t % 3 - 1) * d, ys = y + (t / 3 - 1) * d;
-          const bool ok = x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny;
-          const bfx4 zero = {0, 0, 0, 0};
-          acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[t], ok ? Bt[u][t] : zero, acc[u], 0, 0, 0);
+          const bool ok = t < 9 && x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny;
+          const uint4 zero = {0u, 0u, 0u, 0u};
+          acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A8[p], __builtin_bit_cast(bf16x8, ok ? Bt[u][p] : zero),
+                                                           acc[u], 0, 0, 0);
         }
       }
     } else {
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
           const bool ok = t < 9 && x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny;
           B[j] = (short)(ok ? d2bf(v[u][j] * is) : (unsigned short)0);
         }
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[0], B, fx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, B, fx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       }
     }
 #pragma unroll
@@ -344,26 +350,34 @@ __global__ __launch_bounds__(kBlock) void k_wreduce(const float* __restrict__ pa
 }
 
 // ------------------------------------------------- packing of the A operands
-// forward: A[t][lane]: m = co = lane%16, k = 4g+j -> ci (16 -> *) or tap (1 -> *)
-// backward (input gradient of layer l): m = ci, k = co (hidden) or tap (cout 1),
-// weights of the flipped tap 8 - t
+// The A operand of a layer (cin -> cout) and of its adjoint (cout -> cin,
+// taps flipped, channels exchanged), in the layouts k_conv reads, kPack
+// elements per layer:
+//  16 input channels (v_mfma_f32_16x16x32_bf16, taps in pairs): element
+//    p*512 + lane*8 + j holds A[m = lane%16][tap 2p + g/2, channel 8(g%2)+j],
+//    g = lane/16 (tap 9: zero);
+//  1 input channel (v_mfma_f32_16x16x16bf16_1k): element lane*4 + j holds
+//    A[m = lane%16][tap 4g+j] (taps 9..15: zero).
 __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned short* __restrict__ afw,
                        unsigned short* __restrict__ abw)
 {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // t*256 + lane*4 + j
-  if (idx >= 9 * 256) return;
-  const int t = idx / 256, lane = (idx / 4) % 64, j = idx % 4;
-  const int g = lane / 16, n = lane % 16, k = 4 * g + j;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kPack) return;
   auto W = [&](int co, int ci, int tap) -> float {
-    return (co < cout && ci < cin && tap < 9) ? w[(co * cin + ci) * 9 + tap] : 0.0f;
+    return (co < cout && ci < cin && tap >= 0 && tap < 9) ? w[(co * cin + ci) * 9 + tap] : 0.0f;
   };
-  // forward
-  if (cin == kC) afw[idx] = f2bf(W(n, k, t));
-  else if (t == 0) afw[idx] = f2bf(W(n, 0, k));
-  // backward
+  // K = 32 layout indices
+  const int p = idx / 512, lane = (idx / 8) % 64, j = idx % 8;
+  const int g = lane / 16, m = lane % 16, tap = 2 * p + g / 2, ch = 8 * (g % 2) + j;
+  // K = 16 layout indices (first 256 elements)
+  const int lane1 = idx / 4, k1 = 4 * (lane1 / 16) + idx % 4, m1 = lane1 % 16;
+  // forward: A[m = co][k = (tap, ci)]
+  if (cin == kC) afw[idx] = f2bf(tap < 9 ? W(m, ch, tap) : 0.0f);
+  else afw[idx] = (idx < 256) ? f2bf(W(m1, 0, k1)) : (unsigned short)0;
+  // adjoint: A[m = ci][k = (tap, co)], weights of the flipped tap
   if (abw) {
-    if (cout == kC) abw[idx] = f2bf(W(k, n, 8 - t));
-    else if (t == 0) abw[idx] = f2bf(k < 9 ? W(0, n, 8 - k) : 0.0f);
+    if (cout == kC) abw[idx] = f2bf(tap < 9 ? W(ch, m, 8 - tap) : 0.0f);
+    else abw[idx] = (idx < 256) ? f2bf(W(0, m1, 8 - k1)) : (unsigned short)0;
   }
 }
 
@@ -668,9 +682,9 @@ void lp_pack(LearnedPrec* L)
 {
   hipStream_t s = L->ex->stream;
   for (int l = 0; l < L->nl; l++)
-    hipLaunchKernelGGL(k_pack, dim3(9), dim3(256), 0, s, L->d_p + L->woff[l], L->cin[l], L->cout[l],
-                       L->d_afw + (int64_t)l * 9 * 256,
-                       (l > 0 || L->sym) ? L->d_abw + (int64_t)l * 9 * 256 : nullptr);
+    hipLaunchKernelGGL(k_pack, dim3((kPack + 255) / 256), dim3(256), 0, s, L->d_p + L->woff[l], L->cin[l], L->cout[l],
+                       L->d_afw + (int64_t)l * kPack,
+                       (l > 0 || L->sym) ? L->d_abw + (int64_t)l * kPack : nullptr);
   CEDA_HIP_CHECK(hipGetLastError());
 }
 
@@ -762,7 +776,7 @@ void lp_forward_sym(LearnedPrec* L, int nb, int nx, int ny, const double* in, do
   const int nl = L->nl;
   for (int l = 0; l < nl; l++) {
     a.dil = L->dil[l];
-    a.apack = L->d_afw + (int64_t)l * 9 * 256;
+    a.apack = L->d_afw + (int64_t)l * kPack;
     a.in = (l == 0) ? (const void*)in : (const void*)L->act[l];
     a.inv_scale = (l == 0) ? L->isc : nullptr;
     a.out = L->act[l + 1];
@@ -772,7 +786,7 @@ void lp_forward_sym(LearnedPrec* L, int nb, int nx, int ny, const double* in, do
   a.inv_scale = nullptr;
   for (int l = nl - 1; l >= 0; l--) {
     a.dil = L->dil[l];
-    a.apack = L->d_abw + (int64_t)l * 9 * 256;
+    a.apack = L->d_abw + (int64_t)l * kPack;
     a.in = (l == nl - 1) ? (const void*)L->act[nl] : (const void*)L->uact[l + 1];
     if (l == 0) {
       a.out = out;
@@ -803,7 +817,7 @@ void lp_forward(LearnedPrec* L, int nb, int nx, int ny, const double* in, double
   a.ny = ny;
   for (int l = 0; l < L->nl; l++) {
     a.dil = L->dil[l];
-    a.apack = L->d_afw + (int64_t)l * 9 * 256;
+    a.apack = L->d_afw + (int64_t)l * kPack;
     a.bias = L->d_p + L->boff[l];
     a.mask = nullptr;
     const bool first = (l == 0), last = (l == L->nl - 1);
@@ -863,7 +877,7 @@ void lp_backward_sym(LearnedPrec* L, int nb, int nx, int ny, const double* r)
     if (l == 0) wgrad<1, kC>(L, l, nb, nx, ny, g, nullptr, uin, 0);
     else wgrad<kC, kC>(L, l, nb, nx, ny, g, nullptr, uin, 0);
     a.dil = L->dil[l];
-    a.apack = L->d_afw + (int64_t)l * 9 * 256;
+    a.apack = L->d_afw + (int64_t)l * kPack;
     a.in = g;
     a.out = gout;
     if (l == 0) conv<1, kC, false>(L, a);
@@ -879,7 +893,7 @@ void lp_backward_sym(LearnedPrec* L, int nb, int nx, int ny, const double* r)
     }
     wgrad<kC, kC>(L, l, nb, nx, ny, L->act[l], nullptr, g, 1);
     a.dil = L->dil[l];
-    a.apack = L->d_abw + (int64_t)l * 9 * 256;
+    a.apack = L->d_abw + (int64_t)l * kPack;
     a.in = g;
     a.out = gout;
     conv<kC, kC, false>(L, a);
@@ -931,7 +945,7 @@ void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const dou
     a.dil = L->dil[l];
     a.in = dy;
     a.out = gout;
-    a.apack = L->d_abw + (int64_t)l * 9 * 256;
+    a.apack = L->d_abw + (int64_t)l * kPack;
     a.mask = L->act[l];
     if (isl) conv<1, kC, false>(L, a);
     else conv<kC, kC, false>(L, a);
@@ -976,10 +990,10 @@ LearnedPrec* lp_create(int nlayer, const int* dil, SUNContext ctx, bool sym)
   CEDA_HIP_CHECK(ceda::set_sync(L->d_g, 0, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::set_sync(L->d_m, 0, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::set_sync(L->d_v, 0, sizeof(float) * off));
-  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_afw, sizeof(unsigned short) * 9 * 256 * nlayer));
-  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_abw, sizeof(unsigned short) * 9 * 256 * nlayer));
-  CEDA_HIP_CHECK(ceda::set_sync(L->d_afw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
-  CEDA_HIP_CHECK(ceda::set_sync(L->d_abw, 0, sizeof(unsigned short) * 9 * 256 * nlayer));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_afw, sizeof(unsigned short) * kPack * nlayer));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_abw, sizeof(unsigned short) * kPack * nlayer));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_afw, 0, sizeof(unsigned short) * kPack * nlayer));
+  CEDA_HIP_CHECK(ceda::set_sync(L->d_abw, 0, sizeof(unsigned short) * kPack * nlayer));
   lp_pack(L);
   CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
   return L;
