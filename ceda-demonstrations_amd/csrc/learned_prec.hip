@@ -757,7 +757,7 @@ void conv(LearnedPrec* L, const ConvArgs& a)
   // one row segment (kWavesPB * kConvU blocks of 16 pixels) per workgroup
   // and step; a multiple of 8 workgroups (the kernel's XCD bands)
   const int64_t segs = (int64_t)a.nb * a.ny * (((a.nx + 15) / 16 + kWavesPB * kConvU - 1) / (kWavesPB * kConvU));
-  const unsigned grid = (unsigned)std::min<int64_t>((segs + 7) / 8 * 8, 16384);
+  const unsigned grid = (unsigned)std::min<int64_t>((segs + 7) / 8 * 8, 4096);
   hipLaunchKernelGGL((k_conv<CIN, COUT, RELU>), dim3(grid), dim3(kBlock), 0, L->ex->stream, a);
 }
 
