@@ -431,6 +431,15 @@ def c5_workload(pk, args):
     log(f"c5: {len(probs)} problems, {cells} cells, sides {min(p['nx'] for p in probs)}..."
         f"{max(p['nx'] for p in probs)}, gamma*dt*k/dx^2 {min(ratios):.0f}..{max(ratios):.0f}, {steps} steps")
     methods = args.c5_methods.split(",")
+    if any(m.startswith("learned") for m in methods):
+        # Runs that trained a network in this process before the solves
+        # stalled on some later problems (fused Jacobi-PCG and learned FGMRES
+        # alike) while the same problems solve normally without the training
+        # step and on the CPU oracle; until the cause is found the learned
+        # methods are not run on this workload.
+        raise SystemExit("bench.py --workload c5: the learned methods are disabled on this workload "
+                         "(a run that trains a network before the solves has stalled; cause under "
+                         "investigation); use --c5-methods jacobi,mg")
     sdil = tuple(int(x) for x in args.sym_dil.split(","))
     # network shape per learned method: (dilations, symmetric)
     shapes = {"learned": ((1, 1, 1, 1), False), "learned-sym": (sdil, True)}
@@ -523,11 +532,14 @@ def c5_workload(pk, args):
 
         hip_sync()
         t0 = time.perf_counter()
-        th = [threading.Thread(target=worker) for _ in range(workers)]
-        for x in th:
-            x.start()
-        for x in th:
-            x.join()
+        if workers == 1:
+            worker()  # one worker: the calling thread
+        else:
+            th = [threading.Thread(target=worker) for _ in range(workers)]
+            for x in th:
+                x.start()
+            for x in th:
+                x.join()
         hip_sync()
         el = time.perf_counter() - t0
         assert not errors, errors
@@ -544,7 +556,7 @@ def c5_workload(pk, args):
     errs = {}
     for kind in methods:
         res[kind], errs[kind] = run(kind, args.c5_workers)
-    if "jacobi" in res:
+    if "jacobi" in res and args.c5_workers != 1:
         res["jacobi_one_worker"], _ = run("jacobi", 1)
     out = {"metric": "time-to-solution at fixed L2 err, batched implicit: shared learned preconditioner "
                      "(FGMRES) and multigrid (PCG) vs Jacobi-PCG",
@@ -623,7 +635,7 @@ def main():
     ap.add_argument("--c5-n", type=int, default=64, help="c5: number of problems")
     ap.add_argument("--c5-steps", type=int, default=3, help="c5: DIRK3 steps per problem")
     ap.add_argument("--c5-workers", type=int, default=8, help="c5: concurrent host threads")
-    ap.add_argument("--c5-methods", default="jacobi,mg,learned")
+    ap.add_argument("--c5-methods", default="jacobi,mg")
     ap.add_argument("--tout", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
