@@ -624,7 +624,7 @@ def main():
     ap.add_argument("--fused-pcg", action="store_true",
                     help="implicit/c5: PCG in three fused kernels per iteration (with the analytic J v)")
     ap.add_argument("--sym-dil", default="1,3,9,27", help="dilations of the symmetric network's layers")
-    ap.add_argument("--sym-train-n", type=int, default=128, help="training grid side, symmetric network")
+    ap.add_argument("--sym-train-n", type=int, default=192, help="training grid side, symmetric network")
     ap.add_argument("--sym-train-steps", type=int, default=1000)
     ap.add_argument("--sym-train-lr", type=float, default=3e-3)
     ap.add_argument("--sym-mrestart", type=int, default=20, help="FGMRES restart with the symmetric network")
