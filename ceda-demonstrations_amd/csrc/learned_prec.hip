@@ -3,22 +3,23 @@
 // backward pass, the Adam trainer, the training-set generator (batched
 // Jacobi-PCG) and the ARKLsPrec callbacks.
 //
-// Convolution as implicit GEMM on v_mfma_f32_16x16x16bf16_1k.  For a block
-// of 16 consecutive pixels of one row, D[co][px] = sum_k A[co][k] B[k][px]
-// with k running over (tap, input channel): for a 16 -> 16 layer one MFMA per
-// tap (K = the 16 input channels of the shifted pixel), for the 1 -> 16 input
-// layer one MFMA whose K index is the tap (9 of 16 used).  Operand layout of
-// the instruction (lane l, g = l/16, n = l%16): A holds A[n][4g..4g+3], B
-// holds B[4g..4g+3][n], D register i holds D[4g+i][n].  So the weights are the
-// A operand (packed once per layer, loaded once per wavefront), the B operand
-// is 4 channels of pixel n read as one 8-byte load from the NHWC activation
-// (16 lanes cover 512 contiguous bytes), and each lane's D registers are 4
-// consecutive channels of one pixel: one 8-byte store.  Activations are 16
-// BF16 channels per pixel in a blocked layout: rows padded to whole blocks of
-// 16 pixels, and within a block the 4 groups of 4 channels one after another
-// (16 pixels x 8 bytes each), so the 16 lanes of a channel group read or
-// write 128 contiguous bytes (with plain NHWC they would touch 4 cache lines
-// at a 32-byte stride).  The network input and output are FP64 images.
+// Convolution as implicit GEMM.  For a block of 16 consecutive pixels of one
+// row, D[co][px] = sum_k A[co][k] B[k][px] with k running over (tap, input
+// channel).
+//  - 16 input channels: v_mfma_f32_16x16x32_bf16, one MFMA per pair of taps
+//    (K = 2 taps x 16 channels; 5 MFMAs, the tenth tap zero).  Operand
+//    layout (lane l, g = l/16, n = l%16): A holds A[n][8g..8g+7], B holds
+//    B[8g..8g+7][n], D register i holds D[4g+i][n].  Lane group g reads tap
+//    2p + g/2, channels 8(g%2)..+7 of pixel n: one 16-byte load.
+//  - 1 input channel: v_mfma_f32_16x16x16bf16_1k with the 9 taps on K (A
+//    holds A[n][4g..4g+3], B holds B[4g..4g+3][n]).
+// The weights are the A operand (packed once per layer, loaded once per
+// wavefront), and each lane's D registers are 4 consecutive channels of one
+// pixel: one 8-byte store.  Activations are 16 BF16 channels per pixel in a
+// blocked layout: rows padded to whole blocks of 16 pixels, and within a
+// block channels 0-7 of the 16 pixels, then channels 8-15, so a tap-pair
+// operand is one contiguous 16-byte load per lane.  The network input and
+// output are FP64 images.
 //
 // Backward.  The input gradient of a layer is the same convolution with the
 // taps flipped and the channel roles exchanged (a separately packed A
@@ -572,6 +573,15 @@ __global__ __launch_bounds__(kBlock) void k_synth(int nsamp, int n, int pass, co
       for (int k = 0; k < n; k++) s += S[(int64_t)a * n + k] * base[(int64_t)k * n + bcol];
     out[q] = s;
   }
+}
+
+// r[i] = a[i] for the odd samples i (the operator-applied fields), in place
+__global__ __launch_bounds__(kBlock) void k_take_odd(int64_t npix, int nsamp, const double* __restrict__ a,
+                                                     double* __restrict__ r)
+{
+  const int64_t tot = npix * nsamp;
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock)
+    if ((q / npix) & 1) r[q] = a[q];
 }
 
 // batched Jacobi-PCG, one workgroup per pair (constant diagonal dg)
@@ -1143,9 +1153,7 @@ int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, u
   hipLaunchKernelGGL(k_synth, dim3(gg), dim3(kBlock), 0, s, nsamp, n, 0, dS, p, tmp);
   hipLaunchKernelGGL(k_synth, dim3(gg), dim3(kBlock), 0, s, nsamp, n, 1, dS, tmp, r);
   hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nsamp, n, n, ratio, r, tmp);
-  for (int i = 1; i < nsamp; i += 2)
-    CEDA_HIP_CHECK(hipMemcpyAsync(r + (int64_t)i * n * n, tmp + (int64_t)i * n * n, sizeof(double) * n * n,
-                                  hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_take_odd, dim3(gg), dim3(kBlock), 0, s, (int64_t)n * n, nsamp, tmp, r);
   // Jacobi-PCG on every pair
   const double dg = 1.0 + 4.0 * ratio;
   hipLaunchKernelGGL(k_cg_init, dim3(nsamp), dim3(kBlock), 0, s, n, dg, r, e, tmp, p, st);
