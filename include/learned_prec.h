@@ -1,19 +1,21 @@
 /* ---------------------------------------------------------------------------
  * [ext] Learned preconditioner for the implicit path: a small convolutional
- * network (3x3 convolutions, 16 hidden channels, ReLU between layers, BF16
- * weights and activations, FP32 accumulation on MFMA) that maps a residual r
- * to a correction z ~ (I - gamma J)^{-1} r inside FGMRES.
+ * network (3x3 convolutions, 16 hidden channels, BF16 weights and
+ * activations, FP32 accumulation on MFMA) that maps a residual r to a
+ * correction z ~ (I - gamma J)^{-1} r inside FGMRES.  Two forms: the ReLU
+ * network (LearnedPrec_Create) and the symmetric network K^T K
+ * (LearnedPrec_CreateSymmetric), a fixed SPD operator that also serves PCG.
  *
  * There is no reference counterpart.  It plugs into the preconditioner slot
  * of the reference's implicit path (ARKodeSetPreconditioner, main.cpp:
  * 307-313; the Jacobi pair of preconditioner_jacobi.cpp:9-62 has the same
- * signatures) and is used with a flexible Krylov method (SUNLinSol_SPFGMR),
- * because the network is not a fixed linear operator.  The FP64 FGMRES
- * residual decides convergence; when FGMRES does not converge within its
- * iteration cap the system is re-solved by Jacobi-PCG (ARKodeSetLSFallbackPCG).
+ * signatures).  The ReLU network is not a fixed linear operator and needs a
+ * flexible Krylov method (SUNLinSol_SPFGMR).  The FP64 Krylov residual
+ * decides convergence; when FGMRES does not converge within its iteration
+ * cap the system is re-solved by Jacobi-PCG (ARKodeSetLSFallbackPCG).
  *
- * Network.  nlayer layers (2..8): layer 0 maps 1 -> 16 channels, the hidden
- * layers 16 -> 16, the last 16 -> 1; layer l has dilation dil[l].  Zero
+ * ReLU network.  nlayer layers (2..8): layer 0 maps 1 -> 16 channels, the
+ * hidden layers 16 -> 16, the last 16 -> 1; layer l has dilation dil[l].  Zero
  * padding at the grid edge (the Dirichlet ghosts).  Applied to an image r:
  *   s = RMS(r);  x_0 = bf16(r / s);  x_{l+1} = bf16(relu(W_l * x_l + b_l));
  *   N(r) = s * out_scale * (W_L * x_L + b_L)      (FP64 output)
@@ -30,8 +32,9 @@
  * N = K^T K is linear, symmetric and positive semi-definite up to the BF16
  * roundings, which keeps every eigenvalue of the preconditioned operator
  * positive (a ReLU network can map some residuals to near-zero or opposing
- * corrections, which stalls the Krylov method).  With dilations 1, 3, 9 its
- * receptive field is 53 x 53 nodes.
+ * corrections, which stalls the Krylov method).  With 4 layers of dilations
+ * 1, 3, 9, 27 its receptive field is 161 x 161 nodes (each layer is applied
+ * twice: 8 convolutions per application).
  * ------------------------------------------------------------------------- */
 #ifndef LEARNED_PREC_H
 #define LEARNED_PREC_H
