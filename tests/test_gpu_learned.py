@@ -123,7 +123,7 @@ def test_training_lowers_the_loss(pk, ctx):
     net.release()
 
 
-def dirk_run(pk, ctx, prec, tout, dt, side, net=None, maxl=400, mrestart=30, ls=None, jtimes=False):
+def dirk_run(pk, ctx, prec, tout, dt, side, net=None, maxl=400, mrestart=30, ls=None, jtimes=False, fused=False):
     G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=side, ny=side, kx=1.0, ky=1.0)
     if net is not None:
         net.attach(G)
@@ -133,6 +133,8 @@ def dirk_run(pk, ctx, prec, tout, dt, side, net=None, maxl=400, mrestart=30, ls=
         pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, mrestart)
     if jtimes:
         I.analytic_jtimes()
+        if fused:
+            I.fused_pcg()
     I.ARKodeSetOrder(3)
     I.ARKodeSetFixedStep(dt)
     I.ARKodeSetLSRelativeTolerance(1e-8)
@@ -317,3 +319,23 @@ def test_sym_learned_krylov_on_the_implicit_path(pk, ctx):
     assert nf < 0.35 * ni, (nf, ni)
     net.release()
     ident.release()
+
+
+def test_sym_network_in_fused_pcg(pk, ctx):
+    """The fused PCG calls the network through the preconditioner callback:
+    same Newton iterations and solution as the N_Vector PCG with the same
+    network (Krylov iterations within a few, the dot products being summed in
+    another order)."""
+    side, dt = 128, 0.02
+    dx = 1.0 / (side + 1)
+    ratio = 1767732205903.0 / 4055673282236.0 * dt / dx ** 2
+    net, _ = trained_sym_net(pk, ctx, ratio, steps=600)
+    ref = dirk_run(pk, ctx, "learned", 0.06, dt, side, net=net, ls="cg", jtimes=True, maxl=2000)
+    fus = dirk_run(pk, ctx, "learned", 0.06, dt, side, net=net, ls="cg", jtimes=True, maxl=2000, fused=True)
+    for o in (ref, fus):
+        assert o["flag"] == 0 and abs(o["t"] - 0.06) < 1e-12
+    assert fus["stats"]["nni"] == ref["stats"]["nni"]
+    assert abs(fus["stats"]["nli"] - ref["stats"]["nli"]) <= max(3, 0.05 * ref["stats"]["nli"])
+    scale = np.abs(ref["y"]).max()
+    assert np.abs(fus["y"] - ref["y"]).max() <= 1e-7 * scale
+    net.release()
