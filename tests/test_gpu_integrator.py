@@ -1,7 +1,7 @@
 """Integrator parity on the GPU: the HIP library (N_Vector path, fused stage
 kernels, fused kernels replayed as HIP graphs) against the CPU oracle on the
-same problem, method, tolerances and controller.  The bar is the final-state
-agreement of the north-star (1e-10 relative, FP64) and identical step
+same problem, method, tolerances and controller.  The bar is final-state
+agreement to 1e-10 relative in FP64 and identical step
 statistics; the MMS cases also report the L2 error against the exact solution
 for both."""
 import numpy as np
