@@ -15,8 +15,9 @@
 // "Total PFMG iterations" (main.cpp:489-492).  Added flags:
 //   --problem ceda|mms   manufactured Dirichlet problem of README.md:11-32
 //   --deefreq N --stagemax N --eigsafety X   LSRKStep settings
-//   --fastpath 0|1|2     0: N_Vector API path, 1: fused stage kernels,
-//                        2: fused kernels replayed as HIP graphs
+//   --fastpath 0|1|2|3   0: N_Vector API path, 1: fused stage kernels,
+//                        2: fused kernels replayed as HIP graphs, 3: as 2
+//                        with autotuned launch shapes (multi-stage passes)
 //   --ls cg|gmres|fgmres --ls_reltol X --write_final FILE
 #include <hip/hip_runtime.h>
 
@@ -185,6 +186,10 @@ int main(int argc, char** argv)
     LSRKStepSetDomEigFrequency(arkode_mem, deefreq);
     LSRKStepSetMaxNumStages(arkode_mem, stagemax);
     LSRKStepSetDomEigSafetyFactor(arkode_mem, eigsafety);
+    if (fastpath > 2 && DiffusionProblem_Autotune(udata, 3) != 0) {
+      fprintf(stderr, "autotune failed\n");
+      return 1;
+    }
     if (fastpath > 0) LSRKStepSetDiffusionFastPath(arkode_mem, udata, fastpath > 1);
   }
   SUNAdaptController C = nullptr;
