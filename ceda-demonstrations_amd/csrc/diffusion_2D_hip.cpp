@@ -19,6 +19,10 @@
 //                        2: fused kernels replayed as HIP graphs, 3: as 2
 //                        with autotuned launch shapes (multi-stage passes)
 //   --ls cg|gmres|fgmres --ls_reltol X --write_final FILE
+//   --learned 1          implicit path: the learned symmetric network in the
+//                        preconditioner slot, trained on the GPU before the
+//                        run (--learned_pairs N, --learned_grid n: training
+//                        set size), Jacobi-PCG fallback for the GMRES solvers
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -30,6 +34,7 @@
 
 #include "arkode_hip.h"
 #include "diffusion_hip.h"
+#include "learned_prec.h"
 #include "nvector_hipnative.h"
 
 static double wall()
@@ -50,6 +55,7 @@ int main(int argc, char** argv)
   int order = 2, controller = 0, maxsteps = 0, error = 0, linear = 1, precond = 1;
   int liniters = 20, msbp = 0, internaleig = 0, output = 1, nout = 20, fastpath = 0;
   int onestep = 0, pfmg = 0, pfmg_relax = 2, pfmg_nrelax = 2;
+  int learned = 0, learned_pairs = 20000, learned_grid = 192;
   long deefreq = 25;
   int stagemax = 200;
   const char* write_final = nullptr;
@@ -100,6 +106,9 @@ int main(int argc, char** argv)
     else if (TAKE("--eigsafety")) eigsafety = atof(v);
     else if (TAKE("--ls_reltol")) ls_reltol = atof(v);
     else if (TAKE("--fastpath")) fastpath = atoi(v);
+    else if (TAKE("--learned")) learned = atoi(v);
+    else if (TAKE("--learned_pairs")) learned_pairs = atoi(v);
+    else if (TAKE("--learned_grid")) learned_grid = atoi(v);
     else if (TAKE("--write_final")) write_final = v;
     else {
       fprintf(stderr, "ERROR: Unknown inputs: %s\n", k);
@@ -111,6 +120,11 @@ int main(int argc, char** argv)
   const bool sts = (integrator == "rkc" || integrator == "rkl");
   if (!impl && !sts) {
     fprintf(stderr, "ERROR: illegal integrator\n");
+    return 1;
+  }
+  if (learned && (!impl || !precond || pfmg || learned_pairs < 1 || learned_grid < 16 || learned_grid > 1024)) {
+    fprintf(stderr, "ERROR: --learned needs --integrator dirk with a preconditioner (not --pfmg_*), "
+                    "--learned_pairs >= 1 and --learned_grid 16..1024\n");
     return 1;
   }
 
@@ -158,6 +172,35 @@ int main(int argc, char** argv)
     else LS = SUNLinSol_SPFGMR(u, prectype, liniters, ctx);
   }
 
+  // learned preconditioner: the symmetric network K^T K (4 layers, dilations
+  // 1, 3, 9, 27), trained for this run's gamma*h*k/dx^2 (h = --fixedstep, or
+  // tf/nout), on pairs from Jacobi-PCG solves
+  LearnedPrec* net = nullptr;
+  if (learned) {
+    const double gam = (order == 3) ? 1767732205903.0 / 4055673282236.0 : 1.0;  // DIRK diagonal
+    const double h = (hfixed > 0.0) ? hfixed : prm.tf / nout;
+    const double ratio = gam * h * 0.5 * (prm.kx / (dx * dx) + prm.ky / (dy * dy));
+    const int dil[4] = {1, 3, 9, 27};
+    const double tt = wall();
+    net = LearnedPrec_CreateSymmetric(4, dil, ctx);
+    double *r = nullptr, *e = nullptr;
+    const size_t bytes = sizeof(double) * (size_t)learned_pairs * learned_grid * learned_grid;
+    int its = 0;
+    bool ok = net && hipMalloc(&r, bytes) == hipSuccess && hipMalloc(&e, bytes) == hipSuccess;
+    ok = ok && LearnedPrec_MakeTrainingSet(learned_pairs, learned_grid, ratio, 1e-12, 1234, r, e, &its, ctx) == 0;
+    ok = ok && LearnedPrec_InitIdentity(net, 8e-4, 0) == 0;
+    ok = ok && LearnedPrec_SetOutputScale(net, 1.0 / (1.0 + 4.0 * ratio)) == 0;
+    ok = ok && LearnedPrec_Train(net, learned_pairs, learned_grid, ratio, r, e, 1000, 16, 3e-3, 0, nullptr) == 0;
+    if (r) (void)hipFree(r);
+    if (e) (void)hipFree(e);
+    if (!ok || DiffusionProblem_SetLearnedPrec(udata, net)) {
+      fprintf(stderr, "ERROR: learned preconditioner setup\n");
+      return 1;
+    }
+    printf("  learned preconditioner: symmetric network trained in %.2f s (gamma*h*k/dx^2 = %g, %d pairs of "
+           "%d^2)\n", wall() - tt, ratio, learned_pairs, learned_grid);
+  }
+
   void* arkode_mem = impl ? ARKStepCreate(nullptr, diffusion, 0.0, u, ctx)
                           : LSRKStepCreateSTS(diffusion, 0.0, u, ctx);
   if (!arkode_mem) return 1;
@@ -168,7 +211,10 @@ int main(int argc, char** argv)
     ARKodeSetLinearSolver(arkode_mem, LS, nullptr);
     if (precond) {
       if (pfmg) ARKodeSetPreconditioner(arkode_mem, MGPSetup, MGPSolve);
-      else ARKodeSetPreconditioner(arkode_mem, PSetup, PSolve);
+      else if (net) {
+        ARKodeSetPreconditioner(arkode_mem, LearnedPSetup, LearnedPSolve);
+        if (ls != "cg") ARKodeSetLSFallbackPCG(arkode_mem, 20000, PSolve, udata);
+      } else ARKodeSetPreconditioner(arkode_mem, PSetup, PSolve);
       ARKodeSetLSetupFrequency(arkode_mem, msbp);
     }
     ARKodeSetEpsLin(arkode_mem, epslin);
@@ -318,6 +364,7 @@ int main(int argc, char** argv)
   N_VDestroy(u);
   if (uref) N_VDestroy(uref);
   if (uerr) N_VDestroy(uerr);
+  if (net) LearnedPrec_Destroy(net);
   DiffusionProblem_Destroy(udata);
   SUNContext_Free(&ctx);
   return 0;

@@ -149,3 +149,34 @@ def test_pfmg_flags_rejected_where_unsupported(tmp_path):
     run(["--integrator", "dirk", "--nx", "15", "--ny", "16", "--pfmg_relax", "3"], tmp_path, expect=1)
     run(["--problem", "mms", "--integrator", "dirk", "--nx", "16", "--ny", "16", "--pfmg_relax", "1"],
         tmp_path, expect=1)
+
+
+def ls_iters(stdout):
+    for line in stdout.split("\n"):
+        txt = line.split()
+        if txt[:2] == ["LS", "iters"]:
+            return int(txt[3])
+    raise AssertionError("no LS iters line")
+
+
+def test_learned_flag_puts_the_network_in_the_preconditioner_slot(tmp_path):
+    """--learned 1 trains the symmetric network and solves with it: the same
+    final state as Jacobi-PCG, with far fewer Krylov iterations."""
+    base = ["--problem", "mms", "--nx", "96", "--ny", "96", "--integrator", "dirk", "--order", "3",
+            "--fixedstep", "0.02", "--tf", "0.06", "--nout", "3", "--ls_reltol", "1e-8", "--liniters", "400"]
+    jac = run(base + ["--ls", "cg", "--write_final", "jacobi.bin"], tmp_path)
+    lrn = run(base + ["--ls", "fgmres", "--learned", "1", "--learned_pairs", "2048", "--learned_grid", "64",
+                      "--write_final", "learned.bin"], tmp_path)
+    assert "learned preconditioner: symmetric network trained" in lrn
+    assert "learned preconditioner" not in jac
+    assert ls_iters(lrn) < 0.5 * ls_iters(jac), (ls_iters(lrn), ls_iters(jac))
+    yj = np.fromfile(tmp_path / "jacobi.bin", dtype=np.float64)
+    yl = np.fromfile(tmp_path / "learned.bin", dtype=np.float64)
+    assert yj.size == yl.size == 96 * 96
+    assert np.abs(yl - yj).max() <= 1e-6 * np.abs(yj).max()
+
+
+def test_learned_flag_rejected_where_unsupported(tmp_path):
+    run(["--integrator", "rkl", "--learned", "1"], tmp_path, expect=1)
+    run(["--integrator", "dirk", "--learned", "1", "--pfmg_relax", "2"], tmp_path, expect=1)
+    run(["--integrator", "dirk", "--learned", "1", "--noprec"], tmp_path, expect=1)
