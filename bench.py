@@ -602,6 +602,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--fastpath", type=int, default=2, help="0 API path, 1 fused, 2 fused+graphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-vs-plain implicit comparison after the headline")
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
                     help="only launch the hot-path kernels REPS times each (profiling workload)")
     ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
@@ -701,7 +703,35 @@ def main():
                    "error_test_fails_timed": s1["netf"] - s0["netf"]},
         "roofline": rl, "cpu_baseline": cpu,
     }
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain()
     print(json.dumps(out))
+
+
+def learned_vs_plain():
+    """After the headline measurement: the implicit configuration's
+    time-to-solution comparison (mms-dirichlet 2048^2, DIRK3, dt = 1e-3, 2
+    timed steps; fused Jacobi-PCG against FGMRES with the symmetric learned
+    network, trained on the GPU first), run as a child process with its own
+    time limit so that nothing there can hold up the headline line."""
+    import subprocess
+
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "implicit", "--isteps", "2", "--jtimes",
+           "--fused-pcg", "--imethods", "jacobi,learned-sym", "--no-untrained"]
+    log("learned vs plain: " + " ".join(cmd[1:]))
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        return {"error": "time limit (300 s) reached"}
+    if r.returncode != 0:
+        return {"error": f"exit status {r.returncode}", "stderr_tail": r.stderr[-400:]}
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    keep = ("config", "steps_timed", "gamma_dt_over_dx2", "jacobi_pcg", "learned_symmetric_fgmres",
+            "speedup_learned_symmetric_fgmres_vs_jacobi_pcg", "rel_l2_diff_learned_symmetric_fgmres_vs_jacobi_pcg",
+            "training", "fgmres_restart_symmetric")
+    res = {k: d[k] for k in keep if k in d}
+    log(f"learned vs plain: speedup {res.get('speedup_learned_symmetric_fgmres_vs_jacobi_pcg')}")
+    return res
 
 
 if __name__ == "__main__":
