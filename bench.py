@@ -604,6 +604,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain implicit comparison after the headline")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-vs-sequential comparison after the headline")
     ap.add_argument("--kernels-only", type=int, default=0, metavar="REPS",
                     help="only launch the hot-path kernels REPS times each (profiling workload)")
     ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
@@ -705,32 +707,51 @@ def main():
     }
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain()
+    if not args.no_batch:
+        out["batched_vs_sequential"] = batched_vs_sequential()
     print(json.dumps(out))
 
 
-def learned_vs_plain():
-    """After the headline measurement: the implicit configuration's
-    time-to-solution comparison (mms-dirichlet 2048^2, DIRK3, dt = 1e-3, 2
-    timed steps; fused Jacobi-PCG against FGMRES with the symmetric learned
-    network, trained on the GPU first), run as a child process with its own
-    time limit so that nothing there can hold up the headline line."""
+def child_json(extra, keep, limit=300):
+    """One JSON-printing run of this script as a child process with its own
+    time limit (after the headline measurement, so nothing there can hold
+    up the headline line); returns the kept keys, or the error."""
     import subprocess
 
-    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "implicit", "--isteps", "2", "--jtimes",
-           "--fused-pcg", "--imethods", "jacobi,learned-sym", "--no-untrained"]
-    log("learned vs plain: " + " ".join(cmd[1:]))
+    cmd = [sys.executable, os.path.abspath(__file__)] + extra
+    log("child run: " + " ".join(cmd[1:]))
     try:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=limit)
     except subprocess.TimeoutExpired:
-        return {"error": "time limit (300 s) reached"}
+        return {"error": f"time limit ({limit} s) reached"}
     if r.returncode != 0:
         return {"error": f"exit status {r.returncode}", "stderr_tail": r.stderr[-400:]}
     d = json.loads(r.stdout.strip().splitlines()[-1])
-    keep = ("config", "steps_timed", "gamma_dt_over_dx2", "jacobi_pcg", "learned_symmetric_fgmres",
-            "speedup_learned_symmetric_fgmres_vs_jacobi_pcg", "rel_l2_diff_learned_symmetric_fgmres_vs_jacobi_pcg",
-            "training", "fgmres_restart_symmetric")
-    res = {k: d[k] for k in keep if k in d}
+    return {k: d[k] for k in keep if k in d}
+
+
+def learned_vs_plain():
+    """The implicit configuration's time-to-solution comparison (mms-dirichlet
+    2048^2, DIRK3, dt = 1e-3, 2 timed steps): fused Jacobi-PCG against FGMRES
+    with the symmetric learned network, trained on the GPU first."""
+    res = child_json(["--workload", "implicit", "--isteps", "2", "--jtimes", "--fused-pcg", "--imethods",
+                      "jacobi,learned-sym", "--no-untrained"],
+                     ("config", "steps_timed", "gamma_dt_over_dx2", "jacobi_pcg", "learned_symmetric_fgmres",
+                      "speedup_learned_symmetric_fgmres_vs_jacobi_pcg",
+                      "rel_l2_diff_learned_symmetric_fgmres_vs_jacobi_pcg", "training", "fgmres_restart_symmetric"))
     log(f"learned vs plain: speedup {res.get('speedup_learned_symmetric_fgmres_vs_jacobi_pcg')}")
+    return res
+
+
+def batched_vs_sequential():
+    """The 256-problem batched STS configuration: the batch engine against the
+    same problems solved one after another on the single-problem fast path."""
+    res = child_json(["--workload", "batch"],
+                     ("metric", "n_problems", "tout", "method", "rtol", "batched_s", "sequential_s", "speedup",
+                      "batched_solves_per_s", "sequential_solves_per_s", "rhs_evals_batched",
+                      "rhs_evals_sequential", "abs_l2_error_vs_exact_max",
+                      "max_rel_state_diff_batched_vs_sequential"))
+    log(f"batched vs sequential: speedup {res.get('speedup')}")
     return res
 
 
