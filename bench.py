@@ -726,7 +726,10 @@ def child_json(extra, keep, limit=300):
         return {"error": f"time limit ({limit} s) reached"}
     if r.returncode != 0:
         return {"error": f"exit status {r.returncode}", "stderr_tail": r.stderr[-400:]}
-    d = json.loads(r.stdout.strip().splitlines()[-1])
+    try:
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+    except (IndexError, ValueError) as e:
+        return {"error": f"no JSON line from the child ({e!r})", "stdout_tail": r.stdout[-400:]}
     return {k: d[k] for k in keep if k in d}
 
 
