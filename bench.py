@@ -393,6 +393,33 @@ def implicit_workload(pk, args):
     print(json.dumps(out))
 
 
+def learned_apply_workload(pk, args):
+    """The symmetric learned network (dilations --sym-dil, random weights:
+    the cost does not depend on them) applied --apply-reps times to one
+    2048^2 FP64 image: the network's kernels alone, for timing and counters."""
+    import numpy as np
+
+    ctx = pk.Context()
+    n = args.iside
+    rng = np.random.default_rng(0)
+    R, Z = pk.Vector(ctx, n=n * n), pk.Vector(ctx, n=n * n)
+    R.set(rng.standard_normal(n * n))
+    dil = tuple(int(x) for x in args.sym_dil.split(","))
+    net = pk.LearnedPrec(ctx, dil, symmetric=True)
+    net.set_params((0.1 * rng.standard_normal(net.nparams)).astype(np.float32))
+    net.apply(1, n, n, R, Z)
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.apply_reps):
+        net.apply(1, n, n, R, Z)
+    hip_sync()
+    ms = (time.perf_counter() - t0) / args.apply_reps * 1e3
+    out = {"metric": "symmetric learned network application time", "value": ms, "unit": "ms",
+           "config": {"image": f"{n}x{n} FP64", "dilations": list(dil), "reps": args.apply_reps}}
+    ctx.close()
+    print(json.dumps(out))
+
+
 def c5_problems(n=64, seed=20260102):
     """The batched implicit configuration: square grids with side uniform in
     [256, 1024], kx and ky log-uniform in [0.1, 10]."""
@@ -602,6 +629,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--fastpath", type=int, default=2, help="0 API path, 1 fused, 2 fused+graphs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--apply-reps", type=int, default=10, help="learned-apply: network applications")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-vs-plain implicit comparison after the headline")
     ap.add_argument("--no-batch", action="store_true",
@@ -611,7 +639,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="with --kernels-only: time every launch shape")
     ap.add_argument("--pin-pass", default=None, metavar="K,ROWS,NT",
                     help="with --kernels-only: launch the multi-stage pass at this shape")
-    ap.add_argument("--workload", default="sts", choices=["sts", "batch", "implicit", "c5"],
+    ap.add_argument("--workload", default="sts", choices=["sts", "batch", "implicit", "c5", "learned-apply"],
                     help="sts: the headline 4096^2 RKL2 line; batch: the 256-problem batched "
                          "comparison; implicit: learned-preconditioned FGMRES vs Jacobi-PCG")
     ap.add_argument("--iside", type=int, default=2048)
@@ -661,6 +689,9 @@ def main():
         return
     if args.workload == "c5":
         c5_workload(pk, args)
+        return
+    if args.workload == "learned-apply":
+        learned_apply_workload(pk, args)
         return
     ctx = pk.Context()
     G, I = make_integrator(pk, ctx, args.fastpath)
