@@ -180,3 +180,18 @@ def test_learned_flag_rejected_where_unsupported(tmp_path):
     run(["--integrator", "rkl", "--learned", "1"], tmp_path, expect=1)
     run(["--integrator", "dirk", "--learned", "1", "--pfmg_relax", "2"], tmp_path, expect=1)
     run(["--integrator", "dirk", "--learned", "1", "--noprec"], tmp_path, expect=1)
+
+
+def test_fastpath3_autotuned_passes_match_fastpath2_bitwise(tmp_path):
+    """--fastpath 3 adds the launch-shape autotuner (multi-stage passes where
+    they are faster); the passes are bitwise equal to the single-stage
+    kernels, so the run takes the same steps and ends in the same state."""
+    base = ["--problem", "mms", "--nx", "512", "--ny", "384", "--kx", "1", "--ky", "0.5", "--integrator", "rkl",
+            "--internaleig", "--rtol", "1e-6", "--tf", "0.02", "--nout", "2", "--maxsteps", "100000"]
+    a = run(base + ["--fastpath", "2", "--write_final", "f2.bin"], tmp_path)
+    b = run(base + ["--fastpath", "3", "--write_final", "f3.bin"], tmp_path)
+    sa, sb = harness_stats(a), harness_stats(b)
+    assert sa["Steps"] == sb["Steps"] and sa["Fails"] == sb["Fails"] and sa["FEvals"] == sb["FEvals"]
+    ya = np.fromfile(tmp_path / "f2.bin", dtype=np.float64)
+    yb = np.fromfile(tmp_path / "f3.bin", dtype=np.float64)
+    assert ya.size == 512 * 384 and np.array_equal(ya, yb)
