@@ -458,15 +458,7 @@ def c5_workload(pk, args):
     log(f"c5: {len(probs)} problems, {cells} cells, sides {min(p['nx'] for p in probs)}..."
         f"{max(p['nx'] for p in probs)}, gamma*dt*k/dx^2 {min(ratios):.0f}..{max(ratios):.0f}, {steps} steps")
     methods = args.c5_methods.split(",")
-    if any(m.startswith("learned") for m in methods):
-        # Runs that trained a network in this process before the solves
-        # stalled on some later problems (fused Jacobi-PCG and learned FGMRES
-        # alike) while the same problems solve normally without the training
-        # step and on the CPU oracle; until the cause is found the learned
-        # methods are not run on this workload.
-        raise SystemExit("bench.py --workload c5: the learned methods are disabled on this workload "
-                         "(a run that trains a network before the solves has stalled; cause under "
-                         "investigation); use --c5-methods jacobi,mg")
+    t_end = dt * steps  # every method integrates each problem to the same time
     sdil = tuple(int(x) for x in args.sym_dil.split(","))
     # network shape per learned method: (dilations, symmetric)
     shapes = {"learned": ((1, 1, 1, 1), False), "learned-sym": (sdil, True)}
@@ -479,7 +471,7 @@ def c5_workload(pk, args):
         ctx = pk.Context()
         ratio = float(np.median(ratios))
         tn = args.sym_train_n if sym else args.train_n
-        steps = args.sym_train_steps if sym else args.train_steps
+        train_steps = args.sym_train_steps if sym else args.train_steps
         lr = args.sym_train_lr if sym else args.train_lr
         t0 = time.perf_counter()
         R, E, iters, rc = pk.training_set(ctx, args.nsamp, tn, ratio, reltol=1e-12, seed=1234)
@@ -487,16 +479,16 @@ def c5_workload(pk, args):
         net = pk.LearnedPrec(ctx, dil, symmetric=sym)
         net.init_identity(8e-4, 0)
         net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
-        hist = net.train(args.nsamp, tn, ratio, R, E, steps, args.train_batch, lr, 0)
+        hist = net.train(args.nsamp, tn, ratio, R, E, train_steps, args.train_batch, lr, 0)
         hip_sync()
         shared[kind] = net.params()
-        train[kind] = dict(ratio=ratio, pairs=args.nsamp, grid=tn, steps=steps, lr=lr, dilations=list(dil),
+        train[kind] = dict(ratio=ratio, pairs=args.nsamp, grid=tn, steps=train_steps, lr=lr, dilations=list(dil),
                            symmetric=sym, seconds=time.perf_counter() - t0,
                            loss_first20=float(hist[:20].mean()), loss_last50=float(hist[-50:].mean()))
         log(f"shared {kind} network trained at ratio {ratio:.0f}: {train[kind]}")
         ctx.close()
 
-    def solve_one(ctx, net, k, kind):
+    def solve_one(ctx, net, k, kind, t_end):
         p = probs[k]
         G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=p["nx"], ny=p["ny"], kx=p["kx"], ky=p["ky"])
         ls, prec, maxl = "cg", True, 20000
@@ -517,8 +509,8 @@ def c5_workload(pk, args):
         I.ARKodeSetFixedStep(dt)
         I.ARKodeSetLSRelativeTolerance(1e-8)
         I.ARKodeSetMaxNumSteps(1000000)
-        flag, t = I.evolve(dt * steps)
-        assert flag == 0, (kind, k, flag)
+        flag, t = I.evolve(t_end)
+        assert flag == 0 and t == t_end, (kind, k, flag, t, t_end)
         st = I.stats()
         y = I.y.get()
         ex = G.exact(t).get()
@@ -546,7 +538,7 @@ def c5_workload(pk, args):
                         if not todo or errors:
                             break
                         k = todo.pop(0)
-                    recs[k] = solve_one(ctx, net, k, kind)
+                    recs[k] = solve_one(ctx, net, k, kind, t_end)
                     with lock:
                         done = sum(r is not None for r in recs)
                     if done % 8 == 0 or workers == 1:
@@ -622,7 +614,7 @@ def cpu_baseline():
                       f"estimate and power iteration: {st['nfe']} RHS evals in {dt:.1f} s"}
 
 
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
@@ -669,7 +661,11 @@ def main():
     ap.add_argument("--c5-workers", type=int, default=8, help="c5: concurrent host threads")
     ap.add_argument("--c5-methods", default="jacobi,mg")
     ap.add_argument("--tout", type=float, default=0.5)
-    args = ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def main():
+    args = parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (every box here has one MI355X); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)

@@ -1,9 +1,14 @@
-"""bench.py's command-line contract, checked without a GPU: more than one GPU
-is refused with a message, and the batched implicit workload refuses its
-disabled learned methods before touching the device."""
+"""bench.py's command-line contract and host logic, checked without a GPU:
+more than one GPU is refused with a message, and the batched implicit
+workload integrates every problem to steps*dt for every method, also after
+the shared network has been trained in the same run."""
+import importlib.util
 import os
 import subprocess
 import sys
+import types
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -13,6 +18,13 @@ def run(*args):
                           capture_output=True, text=True, timeout=120)
 
 
+def load_bench():
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def test_more_than_one_gpu_is_refused():
     r = run("--gpus", "2", "--steps", "1", "--warmup", "0")
     assert r.returncode == 2
@@ -20,7 +32,101 @@ def test_more_than_one_gpu_is_refused():
     assert r.stdout.strip() == ""
 
 
-def test_c5_learned_methods_are_refused():
-    r = run("--workload", "c5", "--c5-methods", "jacobi,learned-sym")
-    assert r.returncode != 0
-    assert "learned methods are disabled" in (r.stderr + r.stdout)
+class _Stub:
+    """Host-only stand-in for the package: records the end time each
+    integrator is asked to reach and the training steps requested."""
+
+    MMS_DIRICHLET = 1
+
+    def __init__(self):
+        self.evolve_targets = []
+        self.train_steps = []
+        stub = self
+
+        class Context:
+            def close(self):
+                pass
+
+        class Vec:
+            def __init__(self, n, value=0.0):
+                self.n, self.value = n, value
+
+            def get(self):
+                return np.full(self.n, self.value)
+
+        class Problem:
+            def __init__(self, ctx, kind, nx, ny, kx, ky):
+                self.n = nx * ny
+
+            def initial(self):
+                return Vec(self.n)
+
+            def exact(self, t):
+                return Vec(self.n, 1.0)
+
+            def destroy(self):
+                pass
+
+        class LearnedPrec:
+            def __init__(self, ctx, dil, symmetric):
+                pass
+
+            def init_identity(self, *a):
+                pass
+
+            def set_output_scale(self, s):
+                pass
+
+            def train(self, nsamp, tn, ratio, R, E, steps, batch, lr, seed):
+                stub.train_steps.append(steps)
+                return np.ones(max(steps, 50))
+
+            def params(self):
+                return np.zeros(4, np.float32)
+
+            def set_params(self, p):
+                pass
+
+            def attach(self, G):
+                pass
+
+        class Integrator:
+            def __init__(self, G, method, y0, ls, maxl, precond):
+                self.LS = None
+                self.y = Vec(G.n)
+
+            def __getattr__(self, name):
+                return lambda *a: 0
+
+            def evolve(self, tout):
+                stub.evolve_targets.append(tout)
+                return 0, tout
+
+            def stats(self):
+                return {"nli": 1}
+
+            def fallbacks(self):
+                return 0
+
+        self.Context, self.Problem, self.LearnedPrec, self.Integrator = Context, Problem, LearnedPrec, Integrator
+        self.training_set = lambda ctx, nsamp, tn, ratio, reltol, seed: (None, None, 0, 0)
+        self.lib = lambda: types.SimpleNamespace(SUNLinSol_SPFGMRSetMaxRestartDim=lambda *a: 0)
+
+
+def test_c5_problems_reach_steps_times_dt_after_training(capsys):
+    """The shared network's training loop once rebound the variable that held
+    the DIRK step count, so every problem was integrated to the training-step
+    count times dt (333x too far).  Each method must integrate to c5_steps*dt."""
+    bench = load_bench()
+    bench.hip_sync = lambda: None
+    stub = _Stub()
+    args = bench.parse_args(["--workload", "c5", "--c5-n", "3", "--c5-workers", "1", "--c5-steps", "3",
+                             "--c5-methods", "jacobi,learned-sym,mg", "--nsamp", "4",
+                             "--sym-train-steps", "1000"])
+    bench.c5_workload(stub, args)
+    assert stub.train_steps == [1000]
+    # jacobi, learned-sym, mg, then jacobi on one worker is skipped (workers == 1): 3 methods x 3 problems
+    assert len(stub.evolve_targets) == 9
+    assert all(t == 3 * 1e-3 for t in stub.evolve_targets), stub.evolve_targets
+    out = capsys.readouterr().out
+    assert "3 steps" in out and "1000 steps" not in out
