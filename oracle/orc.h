@@ -41,6 +41,7 @@ enum { ORC_CTRL_PID = 0, ORC_CTRL_PI = 1, ORC_CTRL_I = 2, ORC_CTRL_EXPGUS = 3,
 #define ORC_TOO_MUCH_WORK (-1)
 #define ORC_ERR_FAILURE (-3)
 #define ORC_CONV_FAILURE (-4)
+#define ORC_LSOLVE_FAIL (-7)
 #define ORC_RHSFUNC_FAIL (-8)
 #define ORC_MAX_STAGE_LIMIT_FAIL (-51)
 #define ORC_ILL_INPUT (-22)
@@ -103,6 +104,10 @@ void orc_set_jtimes_analytic(orc_integrator* I, int on);
 int orc_evolve(orc_integrator* I, double tout, double* yout, double* tret);
 int orc_get_stats(const orc_integrator* I, long* out, int nout);
 double orc_get_current_h(const orc_integrator* I);
+double orc_get_h0_used(const orc_integrator* I);
+double orc_get_last_dsm(const orc_integrator* I);
+int orc_test_controller(int ctype, int p_order, double C, double q, double h0, double ubound,
+                        int nattempts, double* h_out, double* dsm_out, int* acc_out);
 double orc_get_spectral_radius(const orc_integrator* I);
 void orc_print_stats(const orc_integrator* I, FILE* fp);
 int orc_test_linear_solve(orc_integrator* I, double t, double gamma, const double* b,

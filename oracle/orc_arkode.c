@@ -60,6 +60,7 @@ struct orc_integrator {
   double *sdata, *zcur, *fz, *rhs, *dz, *r, *pv, *zv, *Ap, *jwork, *jv;
   double epslin, ls_reltol, gammap, nlscoef;
   long nstlp;
+  double dsm_last;  /* error estimate of the last accepted step */
   /* statistics */
   long nst, attempts, nfe, netf, nstlim, ndomeig, ndee_iters, stage_max,
     last_stages, nfe_dee, nli, npsetup, npsolve, nni, ncfn, nlsfail, njtimes,
@@ -186,6 +187,19 @@ static void set_ctrl_params(orc_integrator* I)
   }
 }
 
+/* ARKODE's time-step adaptivity defaults: the PID controller, bias 1.5,
+   safety 0.96, growth 20 (10^4 on the first step), etamxf 0.3 after 2
+   error-test failures, etamin 0.1, etacf 0.25, fixed-step window [1, 1.5] */
+static void adapt_defaults(orc_integrator* I)
+{
+  I->ctype = ORC_CTRL_PID;
+  set_ctrl_params(I);
+  I->bias = 1.5;
+  I->safety = 0.96; I->growth = 20.0; I->etamx1 = 10000.0; I->etamxf = 0.3;
+  I->etamin = 0.1; I->etacf = 0.25; I->lbound = 1.0; I->ubound = 1.5;
+  I->small_nef = 2; I->maxnef = 7; I->maxncf = 10;
+}
+
 orc_integrator* orc_create(orc_problem* p, int method, double t0, const double* y0)
 {
   orc_integrator* I = (orc_integrator*)calloc(1, sizeof(orc_integrator));
@@ -199,12 +213,7 @@ orc_integrator* orc_create(orc_problem* p, int method, double t0, const double* 
   I->yn = vzalloc(I->n); I->ycur = vzalloc(I->n); I->fn = vzalloc(I->n);
   I->tv1 = vzalloc(I->n); I->tv2 = vzalloc(I->n); I->ewt = vzalloc(I->n);
   vcopy(I->yn, y0, I->n);
-  I->ctype = ORC_CTRL_PID;
-  set_ctrl_params(I);
-  I->bias = 1.5;
-  I->safety = 0.96; I->growth = 20.0; I->etamx1 = 10000.0; I->etamxf = 0.3;
-  I->etamin = 0.1; I->etacf = 0.25; I->lbound = 1.0; I->ubound = 1.5;
-  I->small_nef = 2; I->maxnef = 7; I->maxncf = 10;
+  adapt_defaults(I);
   I->mxstep = 500;
   /* LSRKStep defaults: safety 1.01, frequency 25, 200 stages */
   I->dom_eig_safety = 1.01; I->dom_eig_freq = 25; I->smax = 200;
@@ -351,6 +360,57 @@ static double failure_eta(orc_integrator* I, double dsm, int nef)
   eta = fmax(eta, I->etamin);
   if (nef >= I->small_nef) eta = fmin(eta, I->etamxf);
   return eta;
+}
+
+/* the step-size decisions of one attempt: an error-test failure shrinks h,
+   an accepted step records the error in the controller's history and sets
+   the next h (used by take_one_step and by the controller test hook) */
+static void on_error_test_fail(orc_integrator* I, double dsm, int nef)
+{
+  I->h *= failure_eta(I, dsm, nef);
+}
+
+static void on_accept(orc_integrator* I, double dsm)
+{
+  const double eta = accept_eta(I, dsm);
+  ctrl_update(I, I->h, dsm);
+  I->dsm_last = dsm;
+  I->h = I->h * eta;
+}
+
+/* test hook: the controller alone on the model error dsm = C h^q (q = p+1:
+   the asymptotic local error of an embedding of order p).  Default
+   adaptivity constants except the window [1, ubound] in which h is held;
+   nattempts attempts from h0; per attempt the h tried, its dsm and whether
+   it passed. */
+int orc_test_controller(int ctype, int p_order, double C, double q, double h0, double ubound,
+                        int nattempts, double* h_out, double* dsm_out, int* acc_out)
+{
+  orc_integrator I;
+  memset(&I, 0, sizeof(I));
+  adapt_defaults(&I);
+  I.ctype = ctype;
+  set_ctrl_params(&I);
+  I.p_order = p_order;
+  I.ubound = ubound;
+  I.h = h0;
+  int nef = 0;
+  for (int k = 0; k < nattempts; k++) {
+    const double dsm = C * pow(I.h, q);
+    h_out[k] = I.h;
+    dsm_out[k] = dsm;
+    acc_out[k] = dsm <= 1.0;
+    if (dsm > 1.0) {
+      nef++;
+      I.netf++;
+      on_error_test_fail(&I, dsm, nef);
+    } else {
+      nef = 0;
+      I.nst++;
+      on_accept(&I, dsm);
+    }
+  }
+  return 0;
 }
 
 /* ---------------- initial step (CVODE/ARKODE heuristic) ---------------- */
@@ -628,8 +688,18 @@ static void psolve(orc_integrator* I, const double* r, double* z)
   } else vcopy(z, r, I->n);
 }
 
+/* a non-finite scalar in a Krylov iteration: the solve cannot recover (no
+   residual test can pass once a NaN is in r), so it stops with a message
+   instead of iterating to the cap */
+static int krylov_nonfinite(const char* what, double v)
+{
+  if (isfinite(v)) return 0;
+  fprintf(stderr, "[oracle PCG] non-finite %s (%g): linear solve failed\n", what, v);
+  return 1;
+}
+
 /* solves (I - gamma J) x = b, zero initial guess; 0 converged, 1 residual
-   reduced, 2 failed */
+   reduced, 2 failed, -1 a non-finite value (unrecoverable) */
 static int pcg(orc_integrator* I, double t, double gamma, const double* b, double* x)
 {
   const int64_t n = I->n;
@@ -641,22 +711,27 @@ static int pcg(orc_integrator* I, double t, double gamma, const double* b, doubl
   const double eplifac = (I->epslin > 0.0) ? I->epslin : 0.05;
   const double delta = (I->ls_reltol > 0.0) ? I->ls_reltol * bnorm
                                             : eplifac * I->nlscoef * sqrt((double)n);
+  if (krylov_nonfinite("|b|", bnorm)) return -1;
   if (bnorm <= delta) return 0;
   psolve(I, I->r, I->zv);
   vcopy(I->pv, I->zv, n);
   double rz = orc_vdot(I->r, I->zv, n);
+  if (krylov_nonfinite("r.z", rz)) return -1;
   double rnorm = bnorm;
   for (int l = 1; l <= I->maxl; l++) {
     atimes(I, t, gamma, I->pv, I->Ap);
     const double pap = orc_vdot(I->pv, I->Ap, n);
     const double alpha = rz / pap;
+    if (krylov_nonfinite("alpha = r.z / p.Ap", alpha)) return -1;
     vlinsum(1.0, x, alpha, I->pv, x, n);
     vlinsum(1.0, I->r, -alpha, I->Ap, I->r, n);
     I->nli++;
     rnorm = rel ? sqrt(orc_vdot(I->r, I->r, n)) : vwl2(I->r, I->ewt, n);
+    if (krylov_nonfinite("|r|", rnorm)) return -1;
     if (rnorm <= delta) return 0;
     psolve(I, I->r, I->zv);
     const double rznew = orc_vdot(I->r, I->zv, n);
+    if (krylov_nonfinite("r.z", rznew)) return -1;
     const double beta = rznew / rz;
     rz = rznew;
     vlinsum(1.0, I->zv, beta, I->pv, I->pv, n);
@@ -687,6 +762,7 @@ static int newton(orc_integrator* I, double t, double gamma, int force_setup)
     const double* X3[3] = {I->sdata, I->zcur, I->fz};
     orc_vlinearcombination(3, c3, X3, I->rhs, n);
     const int ls = pcg(I, t, gamma, I->rhs, I->dz);
+    if (ls < 0) return ORC_LSOLVE_FAIL;
     if (ls == 2) {
       I->nlsfail++;
       return ORC_CONV_RECOVER;
@@ -786,7 +862,7 @@ static int take_one_step(orc_integrator* I, double tout)
       I->netf++;
       nef++;
       if (nef >= I->maxnef) return ORC_ERR_FAILURE;
-      I->h *= failure_eta(I, dsm, nef);
+      on_error_test_fail(I, dsm, nef);
       continue;
     }
     break;
@@ -798,11 +874,8 @@ static int take_one_step(orc_integrator* I, double tout)
   I->ycur = t;
   I->nst++;
   ewt_set(I, I->yn);
-  if (!I->fixedstep) {
-    const double eta = accept_eta(I, dsm);
-    ctrl_update(I, I->h, dsm);
-    I->h = I->h * eta;
-  } else I->h = I->hfixed;
+  if (!I->fixedstep) on_accept(I, dsm);
+  else I->h = I->hfixed;
   return ORC_SUCCESS;
 }
 
@@ -855,6 +928,8 @@ int orc_get_stats(const orc_integrator* I, long* out, int nout)
 }
 
 double orc_get_current_h(const orc_integrator* I) { return I->h; }
+double orc_get_h0_used(const orc_integrator* I) { return I->h0_used; }
+double orc_get_last_dsm(const orc_integrator* I) { return I->dsm_last; }
 double orc_get_spectral_radius(const orc_integrator* I) { return I->rho; }
 
 /* ARKodePrintAllStats(SUN_OUTPUTFORMAT_TABLE)-style lines; the tokens are the

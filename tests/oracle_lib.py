@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "build", "liborc.so")
 
 CEDA, MMS = 0, 1
 RKC, RKL, DIRK = 0, 1, 2
+LSOLVE_FAIL = -7  # ORC_LSOLVE_FAIL, the value of ARKODE's ARK_LSOLVE_FAIL
 ST = dict(nst=0, attempts=1, nfe=2, netf=3, nstlim=4, ndomeig=5, ndee_iters=6,
           stage_max=7, last_stages=8, nfe_dee=9, nli=10, npsetup=11, npsolve=12,
           nni=13, ncfn=14, nlsfail=15, njtimes=16, nfe_ls=17)

@@ -338,6 +338,32 @@ def test_pcg_with_analytic_jtimes_matches_dst_exact_tightly(gamma):
     np.testing.assert_allclose(x, g[f"g{gamma}_x"].ravel(), rtol=0, atol=1e-11 * np.abs(x).max())
 
 
+def test_pcg_stops_on_a_non_finite_value():
+    """A NaN can never pass the residual test, so PCG must report an
+    unrecoverable failure instead of running to its iteration cap (a stalled
+    solve otherwise looks like a hang).  Planted two ways: a NaN in the state
+    (|b| is NaN before the first iteration) and a NaN gamma without
+    preconditioner (p.Ap is NaN in the first iteration)."""
+    P = O.OracleProblem(O.MMS, nx=16, ny=16)
+    y0 = P.initial()
+    y0[37] = np.nan
+    I = O.OracleIntegrator(P, O.DIRK, y0)
+    I.set_order(3)
+    I.set_lin_solver(5000, 0.0, 0, 1)
+    I.set_fixed_step(1e-3)
+    flag, t, u = I.evolve(1e-3)
+    st = I.stats()
+    assert flag == O.LSOLVE_FAIL and st["nli"] == 0 and st["nst"] == 0
+    P = O.OracleProblem(O.MMS, nx=16, ny=16)
+    I = O.OracleIntegrator(P, O.DIRK, P.initial())
+    I.set_lin_solver(5000, 0.0, 0, 0)
+    ret, x, it = I.linear_solve(0.0, float("nan"), np.ones(P.n), 1e-10, 5000)
+    assert ret == -1 and it == 0
+    # the same solve with a finite gamma converges (the guard does not fire)
+    ret, x, it = I.linear_solve(0.0, 1e-4, np.ones(P.n), 1e-10, 5000)
+    assert ret == 0 and 0 < it < 5000
+
+
 def test_dirk3_fixed_step_third_order():
     g = gold("mms_dst.npz")
     nx, ny, kx, ky = g["24x20_params"]
