@@ -31,6 +31,10 @@ struct SUNLinearSolver_ {
   int mrestart = 30;
   N_Vector r = nullptr, p = nullptr, z = nullptr, Ap = nullptr;
   std::vector<N_Vector> V, Z;
+  // fused PCG: per-workgroup partial sums and the solve state (device), and
+  // the state's host copy (pinned); allocated on the first fused solve
+  double* d_ws = nullptr;
+  double* h_ws = nullptr;
 };
 
 namespace ceda_ark {
@@ -133,6 +137,9 @@ void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T, int kp);
 // 1 Jacobi (P->diag), 2 the preconditioner solve ps
 int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKLsPrecSolveFn ps, void* ud,
               double t, double gamma, N_Vector b, N_Vector x);
+// true (after a message on stderr) when a Krylov scalar is NaN or infinite:
+// the solve then fails unrecoverably instead of iterating to its cap
+bool krylov_nonfinite(const char* solver, const char* what, double v);
 // stages per pass of a multi-stage configuration (1 when disabled)
 inline int pass_stages(const StencilCfg& c) { return c.variant >= 2 ? c.variant : 1; }
 void attempt_begin(ArkMem* A, double tout, AttemptState& S);

@@ -32,6 +32,7 @@
 // s-stage step as one cached HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -684,6 +685,17 @@ double lnorm(ArkMem* A, N_Vector r)
   return N_VWL2Norm(r, A->ewt);
 }
 
+// A NaN or infinity in a Krylov scalar can never pass the residual test: the
+// solve would otherwise run to its iteration cap (and the integrator retry
+// it), which looks like a hang.  The solvers stop with this message and an
+// unrecoverable failure instead (the integrator returns ARK_LSOLVE_FAIL).
+bool krylov_nonfinite(const char* solver, const char* what, double v)
+{
+  if (std::isfinite(v)) return false;
+  fprintf(stderr, "[ARKLS ERROR] %s: non-finite %s (%g); linear solve failed\n", solver, what, v);
+  return true;
+}
+
 // (I - gamma J) x = b by PCG on solver S with preconditioner solve ps/ud
 // (ps = nullptr: none); 0 converged, 1 reduced, 2 failed, <0 fatal.  The
 // flexible variant takes the Polak-Ribiere step
@@ -702,16 +714,19 @@ int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double
     N_VScale(1.0, r, z);
     return 0;
   };
+  const char* name = flexible ? "flexible PCG" : "PCG";
   N_VConst(0.0, x);
   N_VScale(1.0, b, S->r);
   const double bnorm = lnorm(A, S->r);
   const double eplifac = (A->epslin > 0.0) ? A->epslin : 0.05;
   const double delta = (A->ls_reltol > 0.0) ? A->ls_reltol * bnorm
                                             : eplifac * A->nlscoef * sqrt((double)A->n);
+  if (krylov_nonfinite(name, "|b|", bnorm)) return -1;
   if (bnorm <= delta) return 0;
   if (prec(delta, S->r, S->z)) return -1;
   N_VScale(1.0, S->z, S->p);
   double rz = N_VDotProd(S->r, S->z);
+  if (krylov_nonfinite(name, "r.z", rz)) return -1;
   double rnorm = bnorm;
   int err = 0;
   for (int l = 1; l <= S->maxl; l++) {
@@ -719,10 +734,12 @@ int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double
     if (err) return -1;
     const double pap = N_VDotProd(S->p, S->Ap);
     const double alpha = rz / pap;
+    if (krylov_nonfinite(name, "alpha = r.z / p.Ap", alpha)) return -1;
     N_VLinearSum(1.0, x, alpha, S->p, x);
     N_VLinearSum(1.0, S->r, -alpha, S->Ap, S->r);
     A->nli++;
     rnorm = lnorm(A, S->r);
+    if (krylov_nonfinite(name, "|r|", rnorm)) return -1;
     if (rnorm <= delta) return 0;
     if (prec(delta, S->r, S->z)) return -1;
     double beta, rznew;
@@ -736,6 +753,7 @@ int pcg_with(ArkMem* A, SUNLinearSolver S, ARKLsPrecSolveFn ps, void* ud, double
       rznew = N_VDotProd(S->r, S->z);
       beta = rznew / rz;
     }
+    if (krylov_nonfinite(name, "beta", beta)) return -1;
     rz = rznew;
     N_VLinearSum(1.0, S->z, beta, S->p, S->p);
   }
@@ -774,6 +792,7 @@ int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
   const double eplifac = (A->epslin > 0.0) ? A->epslin : 0.05;
   const double delta = (A->ls_reltol > 0.0) ? A->ls_reltol * bnorm
                                             : eplifac * A->nlscoef * sqrt((double)A->n);
+  if (krylov_nonfinite("FGMRES", "|b|", bnorm)) return -1;
   if (bnorm <= delta) return 0;
   const double delta2 = (A->ls_reltol > 0.0) ? delta : delta / N_VMaxNorm(A->ewt);
   std::vector<double> H((m + 1) * m, 0.0), g(m + 1, 0.0), cs(m, 0.0), sn(m, 0.0), y(m, 0.0);
@@ -810,6 +829,8 @@ int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
         hn = sqrt(N_VDotProd(w, w));
         if (!(kReorthFactor * hn < wnorm)) break;
       }
+      // a NaN in w (from the preconditioner or J v) reaches |w| first
+      if (krylov_nonfinite("FGMRES", "|w|", hn)) return -1;
       hv[j + 1] = hn;
       for (int i = 0; i < j; i++) {
         const double a = cs[i] * hv[i] + sn[i] * hv[i + 1];
@@ -851,6 +872,7 @@ int fgmres(ArkMem* A, double t, double gamma, N_Vector b, N_Vector x)
     if (err) return -1;
     N_VLinearSum(1.0, b, -1.0, S->r, S->r);
     beta = sqrt(N_VDotProd(S->r, S->r));
+    if (krylov_nonfinite("FGMRES", "|r| at restart", beta)) return -1;
     resid = beta;
     if (resid <= delta2) return 0;
   }
@@ -1527,6 +1549,8 @@ int ARKodeGetNumLSFallbacks(void* m, long* nfallbacks)
 int ARKodeSetLSRelativeTolerance(void* m, sunrealtype reltol)
 {
   if (!m) return ARK_MEM_NULL;
+  // 0 restores the weighted-norm test; a NaN would never pass a residual test
+  if (!(reltol >= 0.0 && reltol < 1.0)) return ARK_ILL_INPUT;
   AM(m)->ls_reltol = reltol;
   return 0;
 }
@@ -1601,6 +1625,8 @@ int SUNLinSolFree(SUNLinearSolver S)
     if (v) N_VDestroy(v);
   for (N_Vector v : S->V) N_VDestroy(v);
   for (N_Vector v : S->Z) N_VDestroy(v);
+  if (S->d_ws) CEDA_HIP_CHECK(ceda::dev_free(S->d_ws));
+  if (S->h_ws) CEDA_HIP_CHECK(ceda::host_free(S->h_ws));
   delete S;
   return 0;
 }

@@ -5,15 +5,34 @@
 // (SUNLinSol_PCG, main.cpp:305; Hestenes-Stiefel with z = M r), with each
 // iteration in three kernels instead of about nine vector operations:
 //   k_pcg_ap:     Ap = p - gamma * J p, and the partial sums of p.Ap
-//   k_pcg_update: x += alpha p, r -= alpha Ap, z = Dinv r (Jacobi), and the
-//                 partial sums of the residual norm and of r.z
-//   k_pcg_dir:    p = z + beta p
-// so an iteration streams about 104 bytes per cell (176 unfused) and waits
-// for the host twice.  Sums are formed per workgroup in a fixed order and
-// combined by one workgroup, so a solve is deterministic; they are not
-// bitwise equal to the separate N_Vector reductions (a different summation
-// order), which is why this path is an option of the integrator
-// (ARKodeSetDiffusionFastPCG) and the N_Vector path stays the reference.
+//   k_pcg_update: alpha = r.z / p.Ap; x += alpha p, r -= alpha Ap,
+//                 z = Dinv r (Jacobi), and the partial sums of the residual
+//                 norm and of r.z
+//   k_pcg_dir:    beta = r.z(new) / r.z(old); the convergence test; p = z + beta p
+// so an iteration streams about 104 bytes per cell (176 unfused).
+//
+// The scalars never go to the host.  Every workgroup of k_pcg_update and
+// k_pcg_dir first sums the previous kernel's per-workgroup partials itself,
+// all in one fixed order (so every workgroup gets bitwise the same alpha or
+// beta, and the same value a separate one-workgroup combine would give).
+// k_pcg_dir also applies the residual test and records the outcome in a
+// device status word (running, converged at iteration l, non-finite); every
+// kernel of a later iteration reads the word first and returns at once when
+// the solve has ended.  The host therefore enqueues iterations in batches
+// and reads the status once per batch: the iteration count, x and the
+// statistics are those of a test after every iteration, and a NaN or
+// infinity stops the solve with an error instead of running to the cap.
+// Sums are deterministic; they are not bitwise equal to the separate
+// N_Vector reductions (a different summation order), which is why this path
+// is an option of the integrator (ARKodeSetDiffusionFastPCG) and the
+// N_Vector PCG stays the reference.
+//
+// With a preconditioner given as a callback (multigrid), z = M r is computed
+// on the host's call between iterations, so that path reads the residual
+// norm and r.z on the host (two waits per iteration, alpha on the device).
+#include <algorithm>
+#include <cmath>
+
 #include "arkode_internal.hpp"
 #include "diffusion_internal.hpp"
 #include "stencil_ops.hpp"
@@ -24,7 +43,17 @@ namespace ceda_ark {
 
 namespace {
 
-constexpr int kPcgBlocks = 1024;  // workgroups of the reduction kernels
+constexpr int kPcgBlocks = 1024;   // workgroups of the reduction kernels
+constexpr int kPcgBatchMax = 32;   // iterations enqueued between host checks
+// workspace layout (doubles): partials of p.Ap, of (|r|^2, r.z), of r.z
+// (callback preconditioner), then the solve state
+constexpr int kOffA = 0, kOffB = kPcgBlocks, kOffC = 3 * kPcgBlocks, kOffState = 4 * kPcgBlocks;
+constexpr int kWsDoubles = kOffState + 8;
+
+// solve state in device memory (doubles): r.z of the current iteration in
+// slot l & 1, the status word, the iteration it was set at, the last |r|^2
+enum { kRz0 = 0, kRz1 = 1, kStatus = 2, kIter = 3, kRr = 4, kDelta = 5 };
+enum { kRunning = 0, kConverged = 1, kNonFinite = 2 };
 
 // one cell of J v with the periodic wrap or zero Dirichlet ghosts
 __device__ inline double jv_cell(const double* __restrict__ v, const double* __restrict__ fx,
@@ -69,33 +98,91 @@ __device__ inline void block_sums(double (&acc)[NACC], double* __restrict__ part
   }
 }
 
+// the sum of accumulator a over nblocks per-workgroup partials, returned to
+// every thread of the workgroup: thread t adds partials t, t + 256, ... in
+// order, then the wavefront butterfly, then the four wavefront sums in order
+// (the combine order of k_pcg_final, so the value is bitwise the same)
+__device__ inline double sum_partials(const double* __restrict__ partials, int nblocks, int nacc, int a)
+{
+  __shared__ double s[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) v += partials[(int64_t)b * nacc + a];
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  if (lane == 0) s[wave] = v;
+  __syncthreads();
+  double t = s[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / kWave; w++) t += s[w];
+  __syncthreads();  // s is reused by the next call
+  return t;
+}
+
+// a grid-stride walk over the cells that keeps (i, j) without a division
+// per cell: one division per thread for the start and the stride
+struct CellWalk {
+  int64_t k, i, j, n, nx, step, di, dj;
+  __device__ CellWalk(int64_t n_, int64_t nx_) : n(n_), nx(nx_)
+  {
+    k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    j = k / nx;
+    i = k - j * nx;
+    step = (int64_t)gridDim.x * kBlock;
+    dj = step / nx;
+    di = step - dj * nx;
+  }
+  __device__ bool valid() const { return k < n; }
+  __device__ void next()
+  {
+    k += step;
+    i += di;
+    j += dj;
+    if (i >= nx) {
+      i -= nx;
+      j++;
+    }
+  }
+};
+
+__device__ inline bool solve_ended(const double* __restrict__ st) { return st[kStatus] != kRunning; }
+
 __global__ __launch_bounds__(kBlock) void k_pcg_ap(int64_t nx, int64_t ny, const double* __restrict__ fx,
                                                    const double* __restrict__ fy, int per, double gamma,
                                                    const double* __restrict__ p, double* __restrict__ ap,
-                                                   double* __restrict__ partials)
+                                                   double* __restrict__ partials, const double* __restrict__ st)
 {
+  if (st && solve_ended(st)) return;
   double acc[1] = {0.0};
-  const int64_t n = nx * ny;
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
-    const int64_t j = k / nx, i = k - j * nx;
-    const double jp = jv_cell(p, fx, fy, nx, ny, i, j, per != 0);
-    const double a = p[k] + (-gamma) * jp;  // N_VLinearSum(1, p, -gamma, Jp) order
-    ap[k] = a;
-    acc[0] += p[k] * a;
+  for (CellWalk c(nx * ny, nx); c.valid(); c.next()) {
+    const double jp = jv_cell(p, fx, fy, nx, ny, c.i, c.j, per != 0);
+    const double pk = p[c.k];
+    const double a = pk + (-gamma) * jp;  // N_VLinearSum(1, p, -gamma, Jp) order
+    ap[c.k] = a;
+    acc[0] += pk * a;
   }
   block_sums<1>(acc, partials);
 }
 
+// alpha = rz / p.Ap with p.Ap from k_pcg_ap's partials (parts_a), then
+// x += alpha p, r -= alpha Ap and
 // mode 0: no preconditioner (z = r), 1: Jacobi (z = dinv r), 2: z computed
 // elsewhere (only x and r are updated; acc[1] = 0).  wl2: the residual norm
-// is the weighted one, sum (r w)^2, else sum r^2.
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, double alpha, const double* __restrict__ p,
+// is the weighted one, sum (r w)^2, else sum r^2.  rz comes from the state
+// (slot l & 1) when st is given, else from the argument.
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, int nparts, const double* __restrict__ parts_a,
+                                                       double rz_arg, const double* __restrict__ st, int l,
+                                                       const double* __restrict__ p,
                                                        const double* __restrict__ ap, double* __restrict__ x,
                                                        double* __restrict__ r, double* __restrict__ z,
                                                        const double* __restrict__ dinv,
                                                        const double* __restrict__ w, int mode,
                                                        double* __restrict__ partials)
 {
+  if (st && solve_ended(st)) return;
+  const double pap = sum_partials(parts_a, nparts, 1, 0);
+  const double rz = st ? st[l & 1] : rz_arg;
+  const double alpha = rz / pap;
   double acc[2] = {0.0, 0.0};
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
     x[k] = x[k] + alpha * p[k];
@@ -121,6 +208,42 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dot(int64_t n, const double* __r
   block_sums<1>(acc, partials);
 }
 
+// iteration l of a solve whose scalars stay on the device: |r|^2 and r.z
+// from k_pcg_update's partials, the residual test (status converged at l),
+// the non-finite test (status non-finite), else beta and p = z + beta p;
+// workgroup 0 records the state for the next iteration and the host
+__global__ __launch_bounds__(kBlock) void k_pcg_dir_dev(int64_t n, int nparts, const double* __restrict__ parts_b,
+                                                        double* __restrict__ st, int l,
+                                                        const double* __restrict__ z, double* __restrict__ p)
+{
+  if (solve_ended(st)) return;
+  const double rr = sum_partials(parts_b, nparts, 2, 0);
+  const double rznew = sum_partials(parts_b, nparts, 2, 1);
+  const double rnorm = sqrt(rr);
+  const double rz = st[l & 1];
+  const double beta = rznew / rz;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  int status = kRunning;
+  if (!isfinite(rnorm)) status = kNonFinite;
+  else if (rnorm <= st[kDelta]) status = kConverged;
+  else if (!isfinite(beta)) status = kNonFinite;
+  if (status != kRunning) {
+    if (lead) {
+      st[kRr] = rr;
+      st[kIter] = l;
+      st[kStatus] = status;
+    }
+    return;
+  }
+  if (lead) {
+    st[(l + 1) & 1] = rznew;
+    st[kRr] = rr;
+    st[kIter] = l;
+  }
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
+    p[k] = z[k] + beta * p[k];
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t n, double beta, const double* __restrict__ z,
                                                     double* __restrict__ p)
 {
@@ -128,25 +251,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t n, double beta, cons
     p[k] = z[k] + beta * p[k];
 }
 
+__global__ void k_pcg_init(double* __restrict__ st, double rz, double delta)
+{
+  if (threadIdx.x == 0) {
+    st[kRz0] = 0.0;
+    st[kRz1] = rz;  // iteration 1 reads slot 1
+    st[kStatus] = kRunning;
+    st[kIter] = 0;
+    st[kRr] = 0.0;
+    st[kDelta] = delta;
+  }
+}
+
 // sum of the per-workgroup partials (nacc accumulators), in a fixed order
 __global__ __launch_bounds__(kBlock) void k_pcg_final(int nblocks, int nacc, const double* __restrict__ partials,
                                                       double* __restrict__ out)
 {
-  __shared__ double s[kBlock / kWave];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
   for (int a = 0; a < nacc; a++) {
-    double v = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) v += partials[(int64_t)b * nacc + a];
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    if (lane == 0) s[wave] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = s[0];
-      for (int w = 1; w < kBlock / kWave; w++) t += s[w];
-      out[a] = t;
-    }
-    __syncthreads();
+    const double t = sum_partials(partials, nblocks, nacc, a);
+    if (threadIdx.x == 0) out[a] = t;
   }
 }
 
@@ -157,15 +280,23 @@ unsigned blocks_for(int64_t n)
   return (unsigned)(g < 1 ? 1 : g);
 }
 
-// the nacc sums of the last reduction kernel, on the host
-const double* finish(Exec* ex, unsigned g, int nacc)
+// the fused solver's workspace (partials and state), allocated on the
+// solver's first fused solve and kept with it
+void ensure_ws(SUNLinearSolver S)
 {
-  hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, ex->stream, (int)g, nacc, ex->d_partials,
-                     ex->d_result);
-  CEDA_HIP_CHECK(hipMemcpyAsync(ex->h_result, ex->d_result, sizeof(double) * nacc, hipMemcpyDeviceToHost,
-                                ex->stream));
-  CEDA_HIP_CHECK(hipStreamSynchronize(ex->stream));
-  return ex->h_result;
+  if (S->d_ws) return;
+  CEDA_HIP_CHECK(dev_malloc(&S->d_ws, sizeof(double) * kWsDoubles));
+  CEDA_HIP_CHECK(host_malloc(&S->h_ws, sizeof(double) * kWsDoubles, hipHostMallocDefault));
+}
+
+// the nacc sums of partials on the host
+const double* finish(hipStream_t s, SUNLinearSolver S, const double* partials, unsigned g, int nacc)
+{
+  double* out = S->d_ws + kOffState;
+  hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, s, (int)g, nacc, partials, out);
+  CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
+  CEDA_HIP_CHECK(hipStreamSynchronize(s));
+  return S->h_ws;
 }
 
 }  // namespace
@@ -173,13 +304,22 @@ const double* finish(Exec* ex, unsigned g, int nacc)
 int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKLsPrecSolveFn ps, void* ud,
               double t, double gamma, N_Vector b, N_Vector x)
 {
-  // the solver's vectors' stream and reduction workspace (the problem is the
-  // integrator's user data and lives on the same context)
-  Exec* ex = content(S->r)->exec;
-  hipStream_t s = ex->stream;
+  // the solver's vectors' stream (the problem is the integrator's user data
+  // and lives on the same context)
+  hipStream_t s = vstream(S->r);
   const int64_t n = P->n, nx = P->p.nx, ny = P->p.ny;
+  if (vlen(x) != n || vlen(b) != n || vlen(S->r) != n) {
+    fprintf(stderr, "[ARKLS ERROR] fused PCG: vector length %lld does not match the problem's %lld cells\n",
+            (long long)vlen(x), (long long)n);
+    return -1;
+  }
   const int per = (P->p.kind == DIFFUSION_CEDA_PERIODIC) ? 1 : 0;
   const unsigned g = blocks_for(n);
+  ensure_ws(S);
+  double* pa = S->d_ws + kOffA;
+  double* pb = S->d_ws + kOffB;
+  double* pc = S->d_ws + kOffC;
+  double* st = S->d_ws + kOffState;
   double* dx = dptr(x);
   double* dr = dptr(S->r);
   double* dz = dptr(S->z);
@@ -195,6 +335,7 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
   const double eplifac = (A->epslin > 0.0) ? A->epslin : 0.05;
   const double delta = (A->ls_reltol > 0.0) ? A->ls_reltol * bnorm
                                             : eplifac * A->nlscoef * sqrt((double)A->n);
+  if (krylov_nonfinite("fused PCG", "|b|", bnorm)) return -1;
   if (bnorm <= delta) return 0;
   // z = M r
   if (pmode == 2) {
@@ -206,30 +347,65 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
   } else N_VScale(1.0, S->r, S->z);
   N_VScale(1.0, S->z, S->p);
   double rz = N_VDotProd(S->r, S->z);
+  if (krylov_nonfinite("fused PCG", "r.z", rz)) return -1;
+
+  if (pmode != 2) {
+    // scalars on the device; the host reads the state once per batch
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, st, rz, delta);
+    int l = 1, batch = 4;
+    while (l <= S->maxl) {
+      const int last = std::min(S->maxl, l + batch - 1);
+      for (int it = l; it <= last; it++) {
+        hipLaunchKernelGGL(k_pcg_ap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dp,
+                           dap, pa, st);
+        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, dp, dap, dx,
+                           dr, dz, dinv, w, pmode, pb);
+        hipLaunchKernelGGL(k_pcg_dir_dev, dim3(g), dim3(kBlock), 0, s, n, (int)g, pb, st, it, dz, dp);
+      }
+      CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+      CEDA_HIP_CHECK(hipStreamSynchronize(s));
+      const int status = (int)S->h_ws[kStatus];
+      const long iters = (long)S->h_ws[kIter];  // iterations run in this solve
+      if (status != kRunning) {
+        A->nli += iters;
+        A->njtimes += iters;
+        if (pmode == 1) A->npsolve += iters - 1;  // z of the converged iterate is not used
+        if (status == kNonFinite) {
+          fprintf(stderr, "[ARKLS ERROR] fused PCG: non-finite residual or direction at iteration %ld "
+                          "(|r|^2 = %g); linear solve failed\n", iters, S->h_ws[kRr]);
+          return -1;
+        }
+        return 0;
+      }
+      l = last + 1;
+      batch = std::min(2 * batch, kPcgBatchMax);
+    }
+    A->nli += S->maxl;
+    A->njtimes += S->maxl;
+    if (pmode == 1) A->npsolve += S->maxl;
+    return (sqrt(S->h_ws[kRr]) < bnorm) ? 1 : 2;
+  }
+
+  // callback preconditioner: alpha on the device, the residual test and
+  // beta on the host
   double rnorm = bnorm;
   for (int l = 1; l <= S->maxl; l++) {
     hipLaunchKernelGGL(k_pcg_ap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dp, dap,
-                       ex->d_partials);
+                       pa, nullptr);
     A->njtimes++;
-    const double pap = finish(ex, g, 1)[0];
-    const double alpha = rz / pap;
-    hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, alpha, dp, dap, dx, dr, dz, dinv, w,
-                       pmode, ex->d_partials);
-    const double* sums = finish(ex, g, 2);
+    hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, rz, nullptr, l, dp, dap, dx,
+                       dr, dz, dinv, w, pmode, pb);
+    const double* sums = finish(s, S, pb, g, 2);
     A->nli++;
     rnorm = sqrt(sums[0]);
+    if (krylov_nonfinite("fused PCG", "|r|", rnorm)) return -1;
     if (rnorm <= delta) return 0;
-    double rznew;
-    if (pmode == 2) {
-      A->npsolve++;
-      if (ps(t, A->zcur, A->fz, S->r, S->z, gamma, delta, 1, ud)) return -1;
-      hipLaunchKernelGGL(k_pcg_dot, dim3(g), dim3(kBlock), 0, s, n, dr, dz, ex->d_partials);
-      rznew = finish(ex, g, 1)[0];
-    } else {
-      if (pmode == 1) A->npsolve++;
-      rznew = sums[1];
-    }
+    A->npsolve++;
+    if (ps(t, A->zcur, A->fz, S->r, S->z, gamma, delta, 1, ud)) return -1;
+    hipLaunchKernelGGL(k_pcg_dot, dim3(g), dim3(kBlock), 0, s, n, dr, dz, pc);
+    const double rznew = finish(s, S, pc, g, 1)[0];
     const double beta = rznew / rz;
+    if (krylov_nonfinite("fused PCG", "beta", beta)) return -1;
     rz = rznew;
     hipLaunchKernelGGL(k_pcg_dir, dim3(g), dim3(kBlock), 0, s, n, beta, dz, dp);
   }

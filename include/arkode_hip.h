@@ -153,7 +153,8 @@ int ARKodeSetEpsLin(void* arkode_mem, sunrealtype eplifac);
 int ARKodeSetLinear(void* arkode_mem, int timedepend);
 int ARKodeSetNonlinear(void* arkode_mem);
 /* [ext] linear solves stop at ||r||_2 <= reltol ||b||_2 instead of the
-   ARKODE weighted-norm test (used by the fixed-step implicit benchmark) */
+   ARKODE weighted-norm test (used by the fixed-step implicit benchmark);
+   0 restores the weighted test; ARK_ILL_INPUT unless 0 <= reltol < 1 */
 int ARKodeSetLSRelativeTolerance(void* arkode_mem, sunrealtype reltol);
 /* [ext] fallback of a (F)GMRES linear solve: when it stops at its iteration
    cap short of the tolerance, the same system is solved again by PCG with at
