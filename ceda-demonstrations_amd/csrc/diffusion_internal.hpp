@@ -42,6 +42,7 @@ struct DiffusionProblem {
   // stages per pass K, 0 = single-stage kernels only; rows; nt)
   StencilCfg cfg[3];
   LearnedPrec* learned = nullptr;  // network of LearnedPSolve (not owned)
+  double learned_out_scale = 1.0;  // its output scale for this problem (LearnedPSetup)
   MGHierarchy* mg = nullptr;       // multigrid hierarchy of MGPSetup/MGPSolve (owned)
   int mg_relax = 2, mg_nrelax = 2;  // V-cycle smoother: PFMG relax type 2|3, sweeps
   bool tuned = false;

@@ -60,6 +60,9 @@ typedef float fx4 __attribute__((ext_vector_type(4)));
 __device__ inline unsigned short f2bf(float f)
 {
   unsigned u = __float_as_uint(f);
+  // a NaN keeps its sign and stays a (quiet) NaN: rounding its payload
+  // could carry into the exponent and give an infinity
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (unsigned short)((u >> 16) | 0x0040u);
   u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
   return (unsigned short)(u >> 16);
 }
@@ -1114,6 +1117,17 @@ int LearnedPrec_Apply(LearnedPrec* L, int nb, int nx, int ny, const double* in, 
   return 0;
 }
 
+// the network applied with a given output scale (the network's own one is
+// left as it is)
+static int lp_apply_scaled(LearnedPrec* L, int nx, int ny, const double* in, double* out, double out_scale)
+{
+  const double keep = L->out_scale;
+  L->out_scale = out_scale;
+  const int r = LearnedPrec_Apply(L, 1, nx, ny, in, out, 0);
+  L->out_scale = keep;
+  return r;
+}
+
 int LearnedPrec_LossGrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const double* r,
                          const double* e, double* loss, float* grad)
 {
@@ -1215,7 +1229,15 @@ int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const doub
 int DiffusionProblem_SetLearnedPrec(DiffusionProblem* P, LearnedPrec* lp)
 {
   if (!P) return -1;
+  // the network's kernels run on its context's stream; the preconditioner's
+  // vectors are on the problem's: one stream keeps them ordered
+  if (lp && lp->ex != P->exec) {
+    fprintf(stderr, "DiffusionProblem_SetLearnedPrec: the network and the problem belong to different "
+                    "SUNContexts (streams); create the network on the problem's context\n");
+    return -1;
+  }
   P->learned = lp;
+  P->learned_out_scale = lp ? lp->out_scale : 1.0;
   return 0;
 }
 
@@ -1227,7 +1249,8 @@ int LearnedPSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok, sun
   const int r = PSetup(t, u, f, jok, jcurPtr, gamma, P);
   if (r) return r;
   const double d = 1.0 + gamma * (2.0 * P->p.kx / (P->dx * P->dx) + 2.0 * P->p.ky / (P->dy * P->dy));
-  P->learned->out_scale = 1.0 / d;
+  // kept with the problem: problems sharing one network keep their own scale
+  P->learned_out_scale = 1.0 / d;
   return 0;
 }
 
@@ -1236,8 +1259,8 @@ int LearnedPSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z,
 {
   DiffusionProblem* P = static_cast<DiffusionProblem*>(user_data);
   if (!P || !P->learned || N_VGetLength(r) != P->n || N_VGetLength(z) != P->n) return -1;
-  return LearnedPrec_Apply(P->learned, 1, (int)P->p.nx, (int)P->p.ny, N_VGetDeviceArrayPointer(r),
-                           N_VGetDeviceArrayPointer(z), 0);
+  return lp_apply_scaled(P->learned, (int)P->p.nx, (int)P->p.ny, N_VGetDeviceArrayPointer(r),
+                         N_VGetDeviceArrayPointer(z), P->learned_out_scale);
 }
 
 }  // extern "C"

@@ -100,15 +100,19 @@ int LearnedPrec_Train(LearnedPrec* lp, int nsamp, int n, double ratio, const dou
                       double* loss_hist);
 
 /* attach a trained network to a problem (NULL detaches); the problem does
-   not own it */
+   not own it.  The network must have been created on the problem's
+   SUNContext (its kernels run on that context's stream, in order with the
+   problem's vectors); -1 otherwise.  Several problems may share one network:
+   each keeps its own output scale. */
 int DiffusionProblem_SetLearnedPrec(DiffusionProblem* prob, LearnedPrec* lp);
 /* preconditioner callbacks for ARKodeSetPreconditioner with the same
    user_data as PSetup/PSolve (the DiffusionProblem, which also carries the
    RHS, as the reference's UserData does, main.cpp:266, 307-313).  Setup
    refreshes the Jacobi diagonal (the fallback's preconditioner, so the pair
-   PSetup/PSolve stays valid) and sets the network output scale to
+   PSetup/PSolve stays valid) and sets the problem's network output scale to
    1/(1 + gamma*(2kx/dx^2 + 2ky/dy^2)), the inverse diagonal of I - gamma*J
-   for constant coefficients; solve: z = N(r). */
+   for constant coefficients (the network's own scale is not changed);
+   solve: z = N(r) with that scale. */
 int LearnedPSetup(sunrealtype t, N_Vector u, N_Vector f, sunbooleantype jok,
                   sunbooleantype* jcurPtr, sunrealtype gamma, void* user_data);
 int LearnedPSolve(sunrealtype t, N_Vector u, N_Vector f, N_Vector r, N_Vector z,

@@ -42,6 +42,7 @@ static double bfr(double x, int emu)
   float f = (float)x;
   uint32_t u;
   memcpy(&u, &f, 4);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return x; /* NaN stays NaN */
   u += 0x7FFFu + ((u >> 16) & 1u);
   u &= 0xFFFF0000u;
   memcpy(&f, &u, 4);
