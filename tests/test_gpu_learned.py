@@ -212,7 +212,14 @@ def test_flexible_pcg(pk, ctx):
 SDIL = (1, 3, 9)
 
 
-@pytest.mark.parametrize("dil,nb,nx,ny", [((1, 2), 1, 16, 16), (SDIL, 3, 37, 21), (SDIL, 2, 64, 9)])
+# the benchmark's network: dilation 27 reaches past a 16-pixel block, into
+# the clamped-address / zero-select padding path on every side
+BENCH_DIL = (1, 3, 9, 27)
+
+
+@pytest.mark.parametrize("dil,nb,nx,ny", [((1, 2), 1, 16, 16), (SDIL, 3, 37, 21), (SDIL, 2, 64, 9),
+                                          (BENCH_DIL, 1, 64, 64), (BENCH_DIL, 2, 70, 83),
+                                          (BENCH_DIL, 1, 128, 96)])
 def test_sym_forward_matches_oracle(pk, ctx, dil, nb, nx, ny):
     net = pk.LearnedPrec(ctx, dil, symmetric=True)
     assert pk.lib().LearnedPrec_IsSymmetric(net.h) == 1
@@ -235,10 +242,11 @@ def test_sym_forward_matches_oracle(pk, ctx, dil, nb, nx, ny):
     net.release()
 
 
-@pytest.mark.parametrize("seed", [0, 1])
-def test_sym_loss_and_gradient_match_oracle(pk, ctx, seed):
-    nb, nx, ny, ratio = 2, 33, 20, 40.0
-    net = pk.LearnedPrec(ctx, SDIL, symmetric=True)
+@pytest.mark.parametrize("seed,dil,nx,ny", [(0, SDIL, 33, 20), (1, SDIL, 33, 20), (0, BENCH_DIL, 70, 64),
+                                             (1, BENCH_DIL, 64, 75)])
+def test_sym_loss_and_gradient_match_oracle(pk, ctx, seed, dil, nx, ny):
+    nb, ratio = 2, 40.0
+    net = pk.LearnedPrec(ctx, dil, symmetric=True)
     if seed == 0:
         net.init_identity(0.05, 3)
         p = net.params()
@@ -253,12 +261,12 @@ def test_sym_loss_and_gradient_match_oracle(pk, ctx, seed):
     R.set(r)
     E.set(e)
     loss, g = net.loss_grad(nb, nx, ny, ratio, R, E)
-    lref, gref, _ = O.lp_lossgrad(SDIL, p, 1.0 / (1.0 + 4.0 * ratio), nb, nx, ny, ratio, r, e, emu=1, sym=True)
+    lref, gref, _ = O.lp_lossgrad(dil, p, 1.0 / (1.0 + 4.0 * ratio), nb, nx, ny, ratio, r, e, emu=1, sym=True)
     assert abs(loss - lref) <= 1e-3 * abs(lref)
     assert np.linalg.norm(g - gref) <= 1e-2 * np.linalg.norm(gref)
     # the unused bias slots get no gradient
     off = 0
-    for l in range(len(SDIL)):
+    for l in range(len(dil)):
         off += 16 * (1 if l == 0 else 16) * 9
         assert np.all(g[off:off + 16] == 0.0)
         off += 16
