@@ -195,3 +195,116 @@ def test_fastpath3_autotuned_passes_match_fastpath2_bitwise(tmp_path):
     ya = np.fromfile(tmp_path / "f2.bin", dtype=np.float64)
     yb = np.fromfile(tmp_path / "f3.bin", dtype=np.float64)
     assert ya.size == 512 * 384 and np.array_equal(ya, yb)
+
+
+# ------------------------------------------ the reference harness's sweep
+# runtests-diffusion2d.py:75-92: ky = 0, kx in {0.1, 1, 10}, the common flags
+# below, grids 32..256 (one rank per grid here), adaptive rtol 1e-2..1e-6.
+#
+# Parity bar.  With these flags some configurations are sensitive to the
+# last bit of any reduction: the oracle itself takes a different step
+# sequence when rtol moves by one ulp (rkc at 256^2: 56, 62 or 72 steps;
+# dirk2 at 32^2 with kx = 10: 161, 164 or 163 steps, and an --error metric
+# between 3.4e-5 and 3.7e-4; tests/test_oracle.py shows the latter).  The
+# sources are the stopping tests of the power iteration (relative change
+# below 0.01) and of the Krylov solves (a loose tolerance, 0.05 * 0.1 of the
+# error norm), which a last-bit change of a dot product can move by one
+# iteration, and the I controller passes the change on to every later step.
+# The GPU's reductions sum in another order than the serial oracle's, so
+# each case first runs the oracle at rtol and at rtol moved by 1 and 2 ulps:
+#  - where those five runs agree (ulp-stable), the GPU run must take the
+#    same steps (Steps, Fails, RHS evaluations as the harness parses them);
+#    the STS runs must reach the same state (1e-10 relative) and their
+#    --error metric must agree to 1e-8 (that metric's reference run, at
+#    rtol 1e-12, is itself ulp-sensitive).  The DIRK runs stop every linear
+#    solve at a loose tolerance (0.005 of the error norm), so two runs with
+#    the same steps still differ by up to that tolerance per solve: their
+#    states are held to the bound below, their metrics to the same order;
+#  - elsewhere the GPU's counts must lie in the oracle's own envelope
+#    (Steps and RHS evaluations within 10 % of its range, Fails at most 2
+#    above it), and the two final states must agree within the sum of the two
+#    runs' measured errors (each is within its error of the same solution).
+ORACLE_EXE = os.path.join(os.path.dirname(O.LIB_PATH), "diffusion_2D_oracle")
+HARNESS_COMMON = ["--inhomogeneous", "--atol", "1.e-11", "--controller", "2", "--error", "--nonlinear",
+                  "--msbp", "1", "--maxsteps", "100000", "--internaleig"]
+HARNESS_SOLVERS = {"rkc": ["--integrator", "rkc"], "rkl": ["--integrator", "rkl"],
+                   "dirk2-Jacobi": ["--integrator", "dirk", "--order", "2"],
+                   "dirk3-Jacobi": ["--integrator", "dirk", "--order", "3"]}
+ULPS = (0, 1, -1, 2, -2)
+COUNTS = ("Steps", "Fails", "FEvals")
+
+
+def ulp_moved(x, k):
+    for _ in range(abs(k)):
+        x = float(np.nextafter(x, np.inf if k > 0 else -np.inf))
+    return x
+
+
+def harness_case(tmp_path, solver, kx, grid, rtol):
+    base = HARNESS_SOLVERS[solver] + ["--nx", str(grid), "--ny", str(grid), "--kx", str(kx), "--ky", "0.0"] + \
+        HARNESS_COMMON + ["--write_final", "final.bin"]
+    procs = []
+    for k in ULPS:
+        d = tmp_path / f"oracle{k:+d}"
+        d.mkdir()
+        procs.append((d, subprocess.Popen([ORACLE_EXE] + base + ["--rtol", repr(ulp_moved(rtol, k))], cwd=d,
+                                          stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+    dg = tmp_path / "gpu"
+    dg.mkdir()
+    og = run(base + ["--rtol", repr(rtol)], dg)
+    ref = []
+    for d, p in procs:
+        out, err = p.communicate(timeout=600)
+        assert p.returncode == 0, out[-2000:] + err[-2000:]
+        ref.append((harness_stats(out), np.fromfile(d / "final.bin", dtype=np.float64)))
+    return harness_stats(og), np.fromfile(dg / "final.bin", dtype=np.float64), ref
+
+
+def check_harness_case(sg, yg, ref, grid, iterative=False):
+    so, yo = ref[0]
+    assert yg.size == yo.size == grid * grid
+    stable = all(all(r[key] == so[key] for key in COUNTS) for r, _ in ref)
+    # each final state is within its run's measured error of the same solution
+    within_errors = 1.1 * (sg["Accuracy"] + so["Accuracy"]) * np.abs(yo).max()
+    if stable or all(sg[key] == so[key] for key in COUNTS):
+        for key in COUNTS:
+            assert sg[key] == so[key], (key, sg, so, "oracle ulp-stable" if stable else "")
+        if iterative:
+            assert np.abs(yg - yo).max() <= within_errors
+            assert 0.5 * so["Accuracy"] <= sg["Accuracy"] <= 2.0 * so["Accuracy"], (sg["Accuracy"], so["Accuracy"])
+        else:
+            assert np.abs(yg - yo).max() <= 1e-10 * np.abs(yo).max()
+            assert abs(sg["Accuracy"] - so["Accuracy"]) <= 1e-8, (sg["Accuracy"], so["Accuracy"])
+        return "step-for-step"
+    for key in ("Steps", "FEvals"):
+        lo, hi = min(r[key] for r, _ in ref), max(r[key] for r, _ in ref)
+        assert 0.9 * lo <= sg[key] <= 1.1 * hi, (key, sg[key], lo, hi)
+    assert sg["Fails"] <= max(r["Fails"] for r, _ in ref) + 2
+    assert np.abs(yg - yo).max() <= within_errors
+    assert sg["Accuracy"] <= 2.0 * max(r["Accuracy"] for r, _ in ref)
+    return "within the oracle's ulp envelope"
+
+
+@pytest.mark.parametrize("solver", list(HARNESS_SOLVERS))
+@pytest.mark.parametrize("kx,grid", [(0.1, 32), (1.0, 32), (10.0, 32), (0.1, 64), (1.0, 64), (10.0, 64),
+                                     (1.0, 128)])
+def test_harness_sweep_matches_oracle(tmp_path, solver, kx, grid):
+    """The harness's adaptive runs at rtol 1e-4 (bar above)."""
+    sg, yg, ref = harness_case(tmp_path, solver, kx, grid, 1e-4)
+    print(solver, kx, grid, check_harness_case(sg, yg, ref, grid, solver.startswith("dirk")), flush=True)
+
+
+@pytest.mark.parametrize("solver", ["rkl", "dirk3-Jacobi"])
+@pytest.mark.parametrize("rtol", [1e-2, 1e-6])
+def test_harness_sweep_tolerance_ends(tmp_path, solver, rtol):
+    """The loosest and tightest tolerances of the sweep, at 32^2 and kx = 10
+    (the stiffest coefficient)."""
+    sg, yg, ref = harness_case(tmp_path, solver, 10.0, 32, rtol)
+    print(solver, rtol, check_harness_case(sg, yg, ref, 32, solver.startswith("dirk")), flush=True)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("solver", ["rkc", "rkl"])
+def test_harness_sweep_largest_grid(tmp_path, solver):
+    sg, yg, ref = harness_case(tmp_path, solver, 1.0, 256, 1e-4)
+    print(solver, 256, check_harness_case(sg, yg, ref, 256), flush=True)
