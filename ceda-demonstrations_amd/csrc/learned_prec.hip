@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -664,6 +665,57 @@ unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 
 
 }  // namespace
 
+// ============================================ fixed workspace (no allocation)
+// LearnedPrec_ApplyNoAlloc keeps the network application's workspace here
+// instead of in allocations: two activation buffers (ping-pong) of up to
+// 2^20 padded pixels (image rows padded to whole 16-pixel blocks, summed over
+// the images) with 16 BF16 channels each, and for up to 64 images the RMS
+// scale, its inverse and the per-image partial sums.  65 MiB per GPU the
+// library runs on, held until the process ends.
+namespace {
+constexpr int kNAMaxImages = 64;
+constexpr int64_t kNAMaxPixels = (int64_t)1 << 20;
+constexpr int64_t kNAActElems = kNAMaxPixels * 16;  // BF16 elements per activation buffer
+constexpr int kNAMaxParts = 512;                    // RMS partials per image (rms_parts)
+constexpr int64_t kNAScratchBytes = 2 * kNAActElems * (int64_t)sizeof(unsigned short) +
+                                    (int64_t)kNAMaxImages * (2 + kNAMaxParts) * (int64_t)sizeof(double);
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNAScratchBytes];
+
+namespace {
+constexpr int kNAMaxDevices = 64;
+std::mutex g_na_mutex;
+void* g_na_base[kNAMaxDevices] = {};
+
+// device address of the fixed workspace on the current device, resolved by
+// hipGetSymbolAddress once per device (host code never uses the array's
+// name, which is only a host-side stand-in for it)
+int na_scratch(void** base)
+{
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNAMaxDevices) return -1;
+  std::lock_guard<std::mutex> g(g_na_mutex);
+  if (!g_na_base[dev]) {
+    void* p = nullptr;
+    hipError_t e;
+    {
+      // may load the code object: kept out of other threads' graph captures
+      std::lock_guard<std::recursive_mutex> d(ceda::device_sync_mutex());
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    }
+    if (e != hipSuccess || !p) {
+      fprintf(stderr, "learned preconditioner: hipGetSymbolAddress of the fixed workspace failed on device %d: %s\n",
+              dev, hipGetErrorName(e));
+      return -1;
+    }
+    g_na_base[dev] = p;
+  }
+  *base = g_na_base[dev];
+  return 0;
+}
+}  // namespace
+
 // ============================================================== host side
 struct LearnedPrec {
   int nl = 0;
@@ -702,6 +754,7 @@ void lp_pack(LearnedPrec* L)
 }
 
 constexpr int kMaxParts = 512;  // workgroups per image of the per-image reductions
+static_assert(kNAMaxParts >= kMaxParts, "the fixed workspace holds every image's RMS partials");
 int rms_parts(int64_t npix) { return (int)std::min<int64_t>(kMaxParts, (npix + 8191) / 8192); }
 
 // activation buffers hold nb*ny rows of whole 16-pixel blocks
@@ -852,6 +905,85 @@ void lp_forward(LearnedPrec* L, int nb, int nx, int ny, const double* in, double
   CEDA_HIP_CHECK(hipGetLastError());
 }
 
+// the network applied with its workspace in given device memory: two
+// activation buffers used in turn (each layer reads one and writes the
+// other), the per-image scale and inverse, the RMS partials.  The same
+// kernels and operands as lp_scale + lp_forward, so the same result bit for
+// bit.
+void lp_apply_in(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto,
+                 unsigned short* buf0, unsigned short* buf1, double* sc, double* isc, double* part)
+{
+  hipStream_t s = L->ex->stream;
+  const int64_t npix = (int64_t)nx * ny;
+  const int P = rms_parts(npix);
+  hipLaunchKernelGGL(k_sumsq, dim3(P, nb), dim3(kBlock), 0, s, npix, P, in, part);
+  hipLaunchKernelGGL(k_rms_final, dim3(nb), dim3(kBlock), 0, s, nb, P, npix, part, sc, isc);
+  ConvArgs a{};
+  a.nb = nb;
+  a.nx = nx;
+  a.ny = ny;
+  const int nl = L->nl;
+  const void* src = in;
+  unsigned short* dst = buf0;
+  auto advance = [&]() {
+    src = dst;
+    dst = (dst == buf0) ? buf1 : buf0;
+  };
+  if (L->sym) {
+    for (int l = 0; l < nl; l++) {
+      a.dil = L->dil[l];
+      a.apack = L->d_afw + (int64_t)l * kPack;
+      a.in = src;
+      a.inv_scale = (l == 0) ? isc : nullptr;
+      a.out = dst;
+      if (l == 0) conv<1, kC, false>(L, a);
+      else conv<kC, kC, false>(L, a);
+      advance();
+    }
+    a.inv_scale = nullptr;
+    for (int l = nl - 1; l >= 0; l--) {
+      a.dil = L->dil[l];
+      a.apack = L->d_abw + (int64_t)l * kPack;
+      a.in = src;
+      if (l == 0) {
+        a.out = out;
+        a.scale = sc;
+        a.out_mul = L->out_scale;
+        a.addto = addto;
+        conv<kC, 1, false>(L, a);
+      } else {
+        a.out = dst;
+        conv<kC, kC, false>(L, a);
+        advance();
+      }
+    }
+  } else {
+    for (int l = 0; l < nl; l++) {
+      a.dil = L->dil[l];
+      a.apack = L->d_afw + (int64_t)l * kPack;
+      a.bias = L->d_p + L->boff[l];
+      a.mask = nullptr;
+      const bool first = (l == 0), last = (l == nl - 1);
+      a.in = src;
+      a.inv_scale = first ? isc : nullptr;
+      if (last) {
+        a.out = out;
+        a.scale = sc;
+        a.out_mul = L->out_scale;
+        a.addto = addto;
+        if (first) conv<1, 1, false>(L, a);
+        else conv<kC, 1, false>(L, a);
+      } else {
+        a.out = dst;
+        if (first) conv<1, kC, true>(L, a);
+        else conv<kC, kC, true>(L, a);
+        advance();
+      }
+    }
+  }
+  CEDA_HIP_CHECK(hipGetLastError());
+}
+
 // weight gradient of layer l from input x (FP64 images scaled by isc, or
 // NHWC16) and output gradient dy; written to the gradient, or added to it
 template <int CIN, int COUT>
@@ -995,6 +1127,8 @@ LearnedPrec* lp_create(int nlayer, const int* dil, SUNContext ctx, bool sym)
     off += L->cout[l];
   }
   L->np = off;
+  void* na = nullptr;  // the fixed workspace's address on this device, for LearnedPrec_ApplyNoAlloc
+  (void)na_scratch(&na);
   CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_p, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_g, sizeof(float) * off));
   CEDA_HIP_CHECK(ceda::dev_malloc(&L->d_m, sizeof(float) * off));
@@ -1114,6 +1248,26 @@ int LearnedPrec_Apply(LearnedPrec* L, int nb, int nx, int ny, const double* in, 
   lp_workspace(L, nb, nx, ny);
   lp_scale(L, nb, npix, in);
   lp_forward(L, nb, nx, ny, in, out, addto, false);
+  return 0;
+}
+
+int LearnedPrec_ApplyNoAlloc(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto)
+{
+  if (!L || nb < 1 || nx < 1 || ny < 1 || !in || !out) return -1;
+  if (nb > kNAMaxImages || act_elems(nb, nx, ny) > kNAActElems) {
+    fprintf(stderr, "LearnedPrec_ApplyNoAlloc: %d image(s) of %d x %d exceed the fixed workspace (at most %d "
+                    "images and %lld pixels in all, rows padded to multiples of 16)\n",
+            nb, nx, ny, kNAMaxImages, (long long)kNAMaxPixels);
+    return -2;
+  }
+  void* base = nullptr;
+  if (na_scratch(&base)) return -1;
+  unsigned short* buf0 = static_cast<unsigned short*>(base);
+  unsigned short* buf1 = buf0 + kNAActElems;
+  double* sc = reinterpret_cast<double*>(buf1 + kNAActElems);
+  double* isc = sc + kNAMaxImages;
+  double* part = isc + kNAMaxImages;
+  lp_apply_in(L, nb, nx, ny, in, out, addto, buf0, buf1, sc, isc, part);
   return 0;
 }
 

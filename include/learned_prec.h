@@ -72,6 +72,19 @@ double LearnedPrec_GetOutputScale(const LearnedPrec* lp);
 int LearnedPrec_Apply(LearnedPrec* lp, int nb, int nx, int ny, const double* in, double* out,
                       int addto);
 
+/* LearnedPrec_Apply without any allocation, for callers that cannot
+   allocate device memory (code under HIP graph capture, serving loops,
+   harnesses that forbid hipMalloc); it has no counterpart in the reference.
+   The workspace (two activation buffers, per-image scales and partial sums)
+   is a fixed device array of 65 MiB per GPU, held until the process ends:
+   at most 64 images and 2^20 pixels in all, rows padded to multiples of 16
+   (one 1024 x 1024 image); larger inputs return -2 with a message and launch
+   nothing.  Same kernels and operands as LearnedPrec_Apply, so the same
+   result bit for bit.  All calls on one GPU share the array: calls that
+   overlap on one GPU (other streams or host threads) race. */
+int LearnedPrec_ApplyNoAlloc(LearnedPrec* lp, int nb, int nx, int ny, const double* in, double* out,
+                             int addto);
+
 /* training objective on a batch of pairs (r_b, e_b) with (I - ratio*L) e = r
    on nb Dirichlet grids of nx*ny nodes (L the unit-spacing 5-point Laplacian,
    zero boundary), in units normalised by s_b = RMS(r_b):
