@@ -235,6 +235,144 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
   }
 }
 
+// ------------------------------------- 16-channel layers, LDS row chains
+// A layer of dilation d couples row y only with rows y - d and y + d, so the
+// rows of one residue class mod d form a chain that can be swept on its own
+// (for d = 1 the chain is the whole column).  A workgroup takes kChainR
+// consecutive output rows of one chain in one column strip of kChainW
+// pixels and keeps a ring of four input rows of the strip (plus d halo
+// pixels on each side) in LDS: three in use (y - d, y, y + d) and the next
+// one, loaded into registers while the current row is computed and stored
+// into the ring afterwards.  Each input row is read from memory about once
+// per strip (a (kChainW + 2d) / kChainW horizontal halo and two halo rows
+// per kChainR rows), against three times plus the taps' reuse through the
+// caches for k_conv.  Ring layout: [slot][half h][pixel] of 16-byte chunks
+// (channels 8h..8h+7 of one pixel), so the 16 pixels of a tap-pair operand
+// are 256 contiguous bytes (no bank conflicts).  The operands, the MFMA
+// sequence and the stores are those of k_conv: the same results bit for bit.
+constexpr int kChainW = kWavesPB * kConvU * 16;  // pixels per column strip
+constexpr int kChainR = 16;                      // output rows per workgroup
+constexpr int kChainMaxD = 64;                   // ring rows of at most kChainW + 128 pixels
+constexpr int kChainLoads = (2 * (kChainW + 2 * kChainMaxD) + kBlock - 1) / kBlock;  // chunks per thread
+
+// workgroups of one chain launch: per image, strips x min(d, ny) chains x
+// segments of kChainR rows (the longest chain has ceil(ny / d) rows)
+inline int64_t chain_tasks(int nb, int nx, int ny, int d)
+{
+  const int64_t strips = (nx + kChainW - 1) / kChainW;
+  const int64_t chains = std::min(d, ny);
+  const int64_t segs = ((ny + d - 1) / d + kChainR - 1) / kChainR;
+  return (int64_t)nb * strips * chains * segs;
+}
+
+template <int COUT, bool RELU>
+__global__ __launch_bounds__(kBlock) void k_conv_chain(const ConvArgs a)
+{
+  extern __shared__ uint4 ring[];
+  const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
+  const int nx = a.nx, ny = a.ny, d = a.dil;
+  const int span = kChainW + 2 * d;  // pixels of a ring row
+  const int bpr = (nx + 15) >> 4;
+  // task: strip fastest, then chain, then segment, then image
+  const int strips = (nx + kChainW - 1) / kChainW;
+  const int chains = min(d, ny);
+  const int segs = ((ny + d - 1) / d + kChainR - 1) / kChainR;
+  int64_t t = blockIdx.x;
+  const int strip = (int)(t % strips);
+  t /= strips;
+  const int c = (int)(t % chains);
+  t /= chains;
+  const int seg = (int)(t % segs);
+  const int bimg = (int)(t / segs);
+  const int64_t img = (int64_t)bimg * ny;
+  const int x0 = strip * kChainW;
+  const int y0 = c + seg * kChainR * d;
+  if (y0 >= ny) return;  // workgroup-uniform: an empty tail segment of a short chain
+
+  bf16x8 A8[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) A8[p] = __builtin_bit_cast(bf16x8, reinterpret_cast<const uint4*>(a.apack)[p * kWave + lane]);
+  float bv[4] = {0.f, 0.f, 0.f, 0.f};
+  if (a.bias) {
+    if (COUT == kC) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) bv[i] = a.bias[4 * g + i];
+    } else bv[0] = a.bias[0];
+  }
+  const unsigned short* in = static_cast<const unsigned short*>(a.in);
+  // input row yr (zero outside the image) of the strip into registers
+  auto fetch = [&](int yr, uint4 (&v)[kChainLoads]) {
+#pragma unroll
+    for (int k = 0; k < kChainLoads; k++) {
+      const int ci = threadIdx.x + k * kBlock;
+      const int h = ci >= span ? 1 : 0, px = ci - h * span;
+      const int x = x0 - d + px;
+      v[k] = uint4{0u, 0u, 0u, 0u};
+      if (ci < 2 * span && yr >= 0 && yr < ny && x >= 0 && x < nx)
+        v[k] = *reinterpret_cast<const uint4*>(in + chunk_off(img + yr, bpr, x, h));
+    }
+  };
+  auto store = [&](int slot, const uint4 (&v)[kChainLoads]) {
+#pragma unroll
+    for (int k = 0; k < kChainLoads; k++) {
+      const int ci = threadIdx.x + k * kBlock;
+      if (ci < 2 * span) ring[slot * 2 * span + ci] = v[k];
+    }
+  };
+  uint4 v[kChainLoads];
+  // chain index m of the segment is row y0 + m d, kept in ring slot m & 3
+  fetch(y0 - d, v);
+  store(3, v);
+  fetch(y0, v);
+  store(0, v);
+  fetch(y0 + d, v);
+  store(1, v);
+  __syncthreads();
+  const int h = g & 1;
+  for (int m = 0; m < kChainR; m++) {
+    const int y = y0 + m * d;
+    if (y >= ny) break;  // workgroup-uniform
+    const bool more = m + 1 < kChainR && y + d < ny;
+    if (more) fetch(y + 2 * d, v);
+#pragma unroll
+    for (int u = 0; u < kConvU; u++) {
+      const int bx = wave * kConvU + u;
+      const int x = x0 + bx * 16 + n;
+      fx4 acc = fx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int p = 0; p < 5; p++) {
+        const int tp = min(2 * p + (g >> 1), 8);
+        const int slot = (m + tp / 3 - 1) & 3;
+        const int px = bx * 16 + n + d + (tp % 3 - 1) * d;
+        const uint4 bt = ring[(slot * 2 + h) * span + px];
+        const bool ok = 2 * p + (g >> 1) < 9 && x < nx;
+        const uint4 zero = {0u, 0u, 0u, 0u};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A8[p], __builtin_bit_cast(bf16x8, ok ? bt : zero), acc, 0,
+                                                      0, 0);
+      }
+      if (x >= nx) continue;
+      const int64_t pix = (img + y) * nx + x;
+      if (COUT == kC) {
+        bfx4 o;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          float val = acc[i] + bv[i];
+          if (RELU) val = fmaxf(val, 0.0f);
+          if (a.mask && !(bf2f(a.mask[act_off(img + y, bpr, x, g) + i]) > 0.0f)) val = 0.0f;
+          o[i] = (short)f2bf(val);
+        }
+        *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + act_off(img + y, bpr, x, g)) = o;
+      } else if (g == 0) {
+        const double val = (double)(acc[0] + bv[0]) * a.out_mul * (a.scale ? a.scale[bimg] : 1.0);
+        double* out = static_cast<double*>(a.out);
+        out[pix] = a.addto ? out[pix] + val : val;
+      }
+    }
+    if (more) store((m + 2) & 3, v);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------- weight gradient
 struct WGradArgs {
   int nb, nx, ny, dil;
@@ -817,9 +955,19 @@ void lp_scale(LearnedPrec* L, int nb, int64_t npix, const double* in)
   hipLaunchKernelGGL(k_rms_final, dim3(nb), dim3(kBlock), 0, s, nb, P, npix, L->part, L->sc, L->isc);
 }
 
+// 1: the 16-input-channel layers use the LDS row-chain kernel (dilations up
+// to kChainMaxD), 0: every layer uses k_conv
+int g_conv_chain = 1;
+
 template <int CIN, int COUT, bool RELU>
 void conv(LearnedPrec* L, const ConvArgs& a)
 {
+  if (CIN == kC && g_conv_chain && a.dil >= 1 && a.dil <= kChainMaxD) {
+    const int64_t tasks = chain_tasks(a.nb, a.nx, a.ny, a.dil);
+    const size_t lds = sizeof(uint4) * 4 * 2 * (size_t)(kChainW + 2 * a.dil);
+    hipLaunchKernelGGL((k_conv_chain<COUT, RELU>), dim3((unsigned)tasks), dim3(kBlock), lds, L->ex->stream, a);
+    return;
+  }
   // one row segment (kWavesPB * kConvU blocks of 16 pixels) per workgroup
   // and step; a multiple of 8 workgroups (the kernel's XCD bands)
   const int64_t segs = (int64_t)a.nb * a.ny * (((a.nx + 15) / 16 + kWavesPB * kConvU - 1) / (kWavesPB * kConvU));
@@ -1240,6 +1388,13 @@ int LearnedPrec_SetOutputScale(LearnedPrec* L, double s)
 }
 
 double LearnedPrec_GetOutputScale(const LearnedPrec* L) { return L ? L->out_scale : 0.0; }
+
+int LearnedPrec_SetConvKernel(int kind)
+{
+  if (kind != 0 && kind != 1) return -1;
+  g_conv_chain = kind;
+  return 0;
+}
 
 int LearnedPrec_Apply(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto)
 {

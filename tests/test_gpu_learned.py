@@ -347,3 +347,48 @@ def test_sym_network_in_fused_pcg(pk, ctx):
     scale = np.abs(ref["y"]).max()
     assert np.abs(fus["y"] - ref["y"]).max() <= 1e-7 * scale
     net.release()
+
+
+# ------------------------ the two convolution kernels of the 16-channel layers
+def _set_conv(pk, kind):
+    assert pk.lib().LearnedPrec_SetConvKernel(kind) == 0
+
+
+@pytest.mark.parametrize("symmetric,dil", [(True, BENCH_DIL), (True, (1, 2, 6, 18)), (False, DIL)])
+@pytest.mark.parametrize("nb,nx,ny", [(1, 300, 211), (2, 70, 83), (3, 37, 21), (1, 128, 96), (1, 16, 130)])
+def test_row_chain_convolution_is_bitwise_the_direct_one(pk, ctx, symmetric, dil, nb, nx, ny):
+    """LearnedPrec_SetConvKernel(1) (input rows of one residue class mod the
+    dilation staged in LDS) and (0) (every tap read through the caches) must
+    give the same network output and training gradient bit for bit: the
+    same operands in the same MFMA sequence.  Shapes cover strips narrower
+    than one image, images narrower than a strip, fewer rows than the
+    dilation (21 < 27) and a batch of images."""
+    net = pk.LearnedPrec(ctx, dil, symmetric=symmetric)
+    p = random_params(net.nparams, 31 + nx, 0.2)
+    net.set_params(p)
+    net.set_output_scale(0.3)
+    rng = np.random.default_rng(nx * 7 + ny)
+    r = rng.standard_normal(nb * nx * ny)
+    e = rng.standard_normal(nb * nx * ny)
+    R, E, Z = (pk.Vector(ctx, n=r.size) for _ in range(3))
+    R.set(r)
+    E.set(e)
+    out = {}
+    try:
+        for kind in (0, 1):
+            _set_conv(pk, kind)
+            assert net.apply(nb, nx, ny, R, Z) == 0
+            loss, g = net.loss_grad(nb, nx, ny, 40.0, R, E)
+            out[kind] = (Z.get(), loss, g)
+    finally:
+        _set_conv(pk, 1)
+    assert np.all(np.isfinite(out[1][0])) and np.abs(out[1][0]).max() > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+    net.release()
+
+
+def test_conv_kernel_switch_rejects_other_values(pk):
+    assert pk.lib().LearnedPrec_SetConvKernel(2) == -1
+    assert pk.lib().LearnedPrec_SetConvKernel(-1) == -1
