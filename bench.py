@@ -21,8 +21,9 @@ Extra objects on the JSON line:
                 written) when the autotuner enables it, else the single fused
                 stage (40 bytes per cell); mean launch time from HIP events on
                 the integrator's stream
-  cpu_baseline  the CPU oracle (serial C restatement, 1 thread) timed on one
-                full adaptive step of the same workload
+  cpu_baseline  the CPU oracle (the C restatement, -O3 with OpenMP loops)
+                timed on the first adaptive steps of the same workload, on
+                all the process's cores and (reported beside it) on one
 """
 import argparse
 import ctypes as C
@@ -193,11 +194,13 @@ def c4_problems(n=256, seed=20260101):
     return out
 
 
-def batch_workload(pk, nprob, tout):
-    """Batched RKC (one persistent step group per step, one launch per stage
-    index) against the same problems solved one after another with the
+def batch_workload(pk, nprob, tout, batch_only=False, kpass=0):
+    """Batched RKC (one step group per step: every problem plans its attempt,
+    then one launch per phase over all problems still in their stage loop)
+    against the same problems solved one after another with the
     single-problem fused+graph path; absolute L2 errors against the exact
-    solution (u_exact(0.5) = 0, so a relative error is undefined there)."""
+    solution (u_exact(0.5) = 0, so a relative error is undefined there).
+    batch_only: the batched solve alone (a profiling workload)."""
     import numpy as np
 
     ctx = pk.Context()
@@ -205,14 +208,28 @@ def batch_workload(pk, nprob, tout):
     B = pk.Batch(ctx, plist, "rkc")
     B.SetTolerances(1e-5, 1e-10)
     B.SetMaxNumSteps(1000000)
+    if kpass:
+        assert B.SetStagesPerPass(kpass) == 0
     hip_sync()
-    log(f"batched: {nprob} problems, {sum(p['nx'] * p['ny'] for p in plist)} cells")
+    log(f"batched: {nprob} problems, {sum(p['nx'] * p['ny'] for p in plist)} cells, "
+        f"{kpass or 'default'} stages per pass")
     t0 = time.perf_counter()
     assert B.evolve(tout) == 0
     hip_sync()
     t_batch = time.perf_counter() - t0
     groups, launches = B.groups()
     log(f"batched done in {t_batch:.2f} s ({groups} step groups, {launches} phase launches)")
+    if batch_only:
+        nfe = sum(B.stats(k)["nfe"] for k in range(nprob))
+        cells_rhs = sum(B.stats(k)["nfe"] * p["nx"] * p["ny"] for k, p in enumerate(plist))
+        alg = B.algorithmic_bytes()
+        print(json.dumps({"metric": "batched STS solve (256 uneven problems)", "n_problems": nprob, "tout": tout,
+                          "batched_s": t_batch, "step_groups": groups, "phase_launches": launches,
+                          "rhs_evals": nfe, "batched_cell_rhs_per_s": cells_rhs / t_batch,
+                          "phase_kernels_algorithmic_bytes": alg,
+                          "algorithmic_GBs_over_the_solve": alg / t_batch / 1e9}))
+        B.release()
+        return
     # one after another, each with the fused+graph single-problem path
     probs, integs = [], []
     for prm in plist:
@@ -384,6 +401,25 @@ def implicit_workload(pk, args):
     for kind in ("learned", "learned-sym"):
         if kind in res and "identity" in res:
             out[f"krylov_iters_ratio_{names[kind]}_vs_untrained"] = res[kind]["lin_iters"] / res["identity"]["lin_iters"]
+    if "jacobi" in res:
+        # time to solution over the timed steps with the network's training
+        # (pairs + Adam) charged to the learned method, and the step count
+        # from which training pays off
+        tj = res["jacobi"]["s_per_step"]
+        tts = {"steps": n_steps, "t_span": n_steps * dt, "jacobi_pcg_s": tj * n_steps,
+               "note": "the first step (setup) is untimed for every method"}
+        for kind, tag in (("learned-sym", "symmetric"), ("learned-sym-pcg", "symmetric"), ("learned", "relu"),
+                          ("learned-fcg", "relu")):
+            if kind not in res or tag not in training:
+                continue
+            tr = training[tag]["data_s"] + training[tag]["train_s"]
+            tl = res[kind]["s_per_step"]
+            tts[f"{names[kind]}_s_incl_training"] = tl * n_steps + tr
+            tts[f"speedup_{names[kind]}_incl_training"] = tj * n_steps / (tl * n_steps + tr)
+            tts[f"break_even_steps_{names[kind]}"] = tr / (tj - tl) if tj > tl else None
+            # the whole solve to t = 1 (1000 steps) from the measured per-step times
+            tts[f"projected_speedup_{names[kind]}_incl_training_to_t1"] = 1000 * tj / (1000 * tl + tr)
+        out["time_to_solution"] = tts
     out["training"] = training
     out["fgmres_restart"] = args.mrestart
     out["fgmres_restart_symmetric"] = args.sym_mrestart
@@ -415,7 +451,8 @@ def learned_apply_workload(pk, args):
     hip_sync()
     ms = (time.perf_counter() - t0) / args.apply_reps * 1e3
     out = {"metric": "symmetric learned network application time", "value": ms, "unit": "ms",
-           "config": {"image": f"{n}x{n} FP64", "dilations": list(dil), "reps": args.apply_reps}}
+           "config": {"image": f"{n}x{n} FP64", "dilations": list(dil), "reps": args.apply_reps,
+                      "conv_kernel": "LDS row chains" if args.conv_kernel == 1 else "direct"}}
     ctx.close()
     print(json.dumps(out))
 
@@ -460,12 +497,13 @@ def c5_workload(pk, args):
     methods = args.c5_methods.split(",")
     t_end = dt * steps  # every method integrates each problem to the same time
     sdil = tuple(int(x) for x in args.sym_dil.split(","))
-    # network shape per learned method: (dilations, symmetric)
-    shapes = {"learned": ((1, 1, 1, 1), False), "learned-sym": (sdil, True)}
+    # network shape per learned method: (dilations, symmetric); learned-sym-pcg
+    # is PCG with the symmetric network (a fixed SPD preconditioner)
+    shapes = {"learned": ((1, 1, 1, 1), False), "learned-sym": (sdil, True), "learned-sym-pcg": (sdil, True)}
     shared = {}
     train = {}
     for kind in methods:
-        if kind not in shapes:
+        if kind not in shapes or (kind == "learned-sym-pcg" and "learned-sym" in shared):
             continue
         dil, sym = shapes[kind]
         ctx = pk.Context()
@@ -482,6 +520,8 @@ def c5_workload(pk, args):
         hist = net.train(args.nsamp, tn, ratio, R, E, train_steps, args.train_batch, lr, 0)
         hip_sync()
         shared[kind] = net.params()
+        if kind in ("learned-sym", "learned-sym-pcg"):  # one symmetric network for both
+            shared["learned-sym"] = shared["learned-sym-pcg"] = shared[kind]
         train[kind] = dict(ratio=ratio, pairs=args.nsamp, grid=tn, steps=train_steps, lr=lr, dilations=list(dil),
                            symmetric=sym, seconds=time.perf_counter() - t0,
                            loss_first20=float(hist[:20].mean()), loss_last50=float(hist[-50:].mean()))
@@ -498,7 +538,7 @@ def c5_workload(pk, args):
             dx2 = 1.0 / (p["nx"] + 1) ** 2
             net.set_output_scale(1.0 / (1.0 + 2.0 * gamma_a * dt * (p["kx"] + p["ky"]) / dx2))
             net.attach(G)
-            ls, prec, maxl = "fgmres", "learned", args.imaxl
+            ls, prec, maxl = ("cg" if kind == "learned-sym-pcg" else "fgmres"), "learned", args.imaxl
         I = pk.Integrator(G, "dirk", G.initial(), ls=ls, maxl=maxl, precond=prec)
         if ls == "fgmres":
             pk.lib().SUNLinSol_SPFGMRSetMaxRestartDim(I.LS, args.sym_mrestart if kind == "learned-sym" else args.mrestart)
@@ -595,23 +635,50 @@ def c5_workload(pk, args):
     print(json.dumps(out))
 
 
+def cpu_threads():
+    """Host threads for the OpenMP leg: the process's CPU share (OMP_NUM_THREADS
+    when set, as on the GPU hosts), else the cores it may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline():
-    """One adaptive RKL2 step of the same workload on the CPU oracle (serial)."""
+    """The CPU oracle (this repository's C restatement of the reference path,
+    built -O3 -march=x86-64-v3 with OpenMP loops) timed on the same workload
+    on this host: serially on one core, the first adaptive step (initial-step
+    estimate, one power iteration, the step's stages and error estimate); and
+    with OpenMP on every core of the process's share, the first four steps.
+    `value` is the OpenMP rate; the serial leg is reported beside it."""
     import oracle_lib as O
 
-    P = O.OracleProblem(O.MMS, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
-    R = O.OracleIntegrator(P, O.RKL, P.initial())
-    R.set_tolerances(1e-6, 1e-10)
-    R.set_internal_eig(1, 100, 0.01)
-    R.set_dom_eig_frequency(25)
-    R.set_max_num_steps(1)
-    t0 = time.perf_counter()
-    R.evolve(1.0)
-    dt = time.perf_counter() - t0
-    st = R.stats()
-    return {"value": st["nfe"] * P.n / dt, "unit": "cell-RHS evals/s", "cores": 1, "kind": "port",
-            "sample": f"first adaptive step of the 4096^2 RKL2 workload incl. initial-step "
-                      f"estimate and power iteration: {st['nfe']} RHS evals in {dt:.1f} s"}
+    def leg(threads, steps):
+        L = O.lib(O.TIMING_LIB_PATH)
+        got = L.orc_set_threads(threads)
+        P = O.OracleProblem(O.MMS, lib_path=O.TIMING_LIB_PATH, nx=N_SIDE, ny=N_SIDE, kx=1.0, ky=0.5)
+        R = O.OracleIntegrator(P, O.RKL, P.initial())
+        R.set_tolerances(1e-6, 1e-10)
+        R.set_internal_eig(1, 100, 0.01)
+        R.set_dom_eig_frequency(25)
+        R.set_max_num_steps(steps)
+        t0 = time.perf_counter()
+        R.evolve(1.0)
+        dt = time.perf_counter() - t0
+        st = R.stats()
+        return got, st, dt, st["nfe"] * P.n / dt
+
+    nthr = cpu_threads()
+    _, st1, dt1, v1 = leg(1, 1)
+    got, stn, dtn, vn = leg(nthr, 4)
+    log(f"cpu baseline: serial {v1:.3e}, {got} threads {vn:.3e} cell-RHS evals/s")
+    return {"value": vn, "unit": "cell-RHS evals/s", "cores": got, "kind": "port",
+            "sample": f"first 4 adaptive steps of the 4096^2 RKL2 workload (initial-step estimate, power "
+                      f"iteration, {stn['nst']} steps): {stn['nfe']} RHS evals in {dtn:.1f} s on {got} threads",
+            "build": "oracle/ (C restatement), gcc -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off",
+            "host_cpus": os.cpu_count(), "process_cpus": len(os.sched_getaffinity(0)),
+            "serial": {"value": v1, "cores": 1,
+                       "sample": f"first adaptive step: {st1['nfe']} RHS evals in {dt1:.1f} s"}}
 
 
 def parse_args(argv=None):
@@ -661,6 +728,11 @@ def parse_args(argv=None):
     ap.add_argument("--c5-workers", type=int, default=8, help="c5: concurrent host threads")
     ap.add_argument("--c5-methods", default="jacobi,mg")
     ap.add_argument("--tout", type=float, default=0.5)
+    ap.add_argument("--conv-kernel", type=int, default=1, choices=[0, 1],
+                    help="learned network's 16-channel layers: 1 LDS row chains, 0 the direct kernel")
+    ap.add_argument("--kpass", type=int, default=0, help="batch: stages per pass (0: the library default)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch: the batched solve alone, without the one-after-another comparison")
     return ap.parse_args(argv)
 
 
@@ -673,12 +745,14 @@ def main():
     from hiplib import package
 
     pk = package()
+    if args.conv_kernel != 1:
+        assert pk.lib().LearnedPrec_SetConvKernel(args.conv_kernel) == 0
     if args.kernels_only:
         pin = tuple(int(x) for x in args.pin_pass.split(",")) if args.pin_pass else None
         kernels_only(pk, args.kernels_only, args.sweep, pin)
         return
     if args.workload == "batch":
-        batch_workload(pk, args.nprob, args.tout)
+        batch_workload(pk, args.nprob, args.tout, args.batch_only, args.kpass)
         return
     if args.workload == "implicit":
         implicit_workload(pk, args)
@@ -762,13 +836,15 @@ def child_json(extra, keep, limit=300):
 
 def learned_vs_plain():
     """The implicit configuration's time-to-solution comparison (mms-dirichlet
-    2048^2, DIRK3, dt = 1e-3, 2 timed steps): fused Jacobi-PCG against FGMRES
-    with the symmetric learned network, trained on the GPU first."""
-    res = child_json(["--workload", "implicit", "--isteps", "2", "--jtimes", "--fused-pcg", "--imethods",
+    2048^2, DIRK3, dt = 1e-3, 50 timed steps): fused Jacobi-PCG against FGMRES
+    with the symmetric learned network, trained on the GPU first; the time to
+    solution is reported per step and with the training included."""
+    res = child_json(["--workload", "implicit", "--isteps", "50", "--jtimes", "--fused-pcg", "--imethods",
                       "jacobi,learned-sym", "--no-untrained"],
                      ("config", "steps_timed", "gamma_dt_over_dx2", "jacobi_pcg", "learned_symmetric_fgmres",
                       "speedup_learned_symmetric_fgmres_vs_jacobi_pcg",
-                      "rel_l2_diff_learned_symmetric_fgmres_vs_jacobi_pcg", "training", "fgmres_restart_symmetric"))
+                      "rel_l2_diff_learned_symmetric_fgmres_vs_jacobi_pcg", "time_to_solution", "training",
+                      "fgmres_restart_symmetric"))
     log(f"learned vs plain: speedup {res.get('speedup_learned_symmetric_fgmres_vs_jacobi_pcg')}")
     return res
 
@@ -778,7 +854,8 @@ def batched_vs_sequential():
     same problems solved one after another on the single-problem fast path."""
     res = child_json(["--workload", "batch"],
                      ("metric", "n_problems", "tout", "method", "rtol", "batched_s", "sequential_s", "speedup",
-                      "batched_solves_per_s", "sequential_solves_per_s", "rhs_evals_batched",
+                      "batched_solves_per_s", "sequential_solves_per_s", "step_groups", "phase_launches",
+                      "batched_cell_rhs_per_s", "rhs_evals_batched",
                       "rhs_evals_sequential", "abs_l2_error_vs_exact_max",
                       "max_rel_state_diff_batched_vs_sequential"))
     log(f"batched vs sequential: speedup {res.get('speedup')}")

@@ -104,6 +104,7 @@ void orc_set_jtimes_analytic(orc_integrator* I, int on);
 int orc_evolve(orc_integrator* I, double tout, double* yout, double* tret);
 int orc_get_stats(const orc_integrator* I, long* out, int nout);
 double orc_get_current_h(const orc_integrator* I);
+int orc_set_threads(int n);
 double orc_get_h0_used(const orc_integrator* I);
 double orc_get_last_dsm(const orc_integrator* I);
 int orc_test_controller(int ctype, int p_order, double C, double q, double h0, double ubound,

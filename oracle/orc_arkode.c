@@ -19,6 +19,9 @@
  * ------------------------------------------------------------------------- */
 #include <float.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -72,14 +75,19 @@ static void vcopy(double* z, const double* x, int64_t n)
 {
   memcpy(z, x, sizeof(double) * (size_t)n);
 }
+/* The OpenMP pragmas below only act in the CPU-baseline timing build
+   (liborc_omp.so, -fopenmp); the checker build ignores them and stays
+   serial, so its reductions keep their fixed serial order. */
 static void vlinsum(double a, const double* x, double b, const double* y,
                     double* z, int64_t n)
 {
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; i++) z[i] = a * x[i] + b * y[i];
 }
 void orc_vlinearcombination(int nv, const double* c, const double* const* X,
                             double* z, int64_t n)
 {
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; i++) {
     double acc = c[0] * X[0][i];
     for (int k = 1; k < nv; k++) acc = acc + c[k] * X[k][i];
@@ -89,6 +97,7 @@ void orc_vlinearcombination(int nv, const double* c, const double* const* X,
 double orc_vwrmsnorm(const double* x, const double* w, int64_t n)
 {
   double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
   for (int64_t i = 0; i < n; i++) {
     const double p = x[i] * w[i];
     s += p * p;
@@ -98,6 +107,7 @@ double orc_vwrmsnorm(const double* x, const double* w, int64_t n)
 double orc_vdot(const double* x, const double* y, int64_t n)
 {
   double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
   for (int64_t i = 0; i < n; i++) s += x[i] * y[i];
   return s;
 }
@@ -113,6 +123,7 @@ double orc_vmaxnorm(const double* x, int64_t n)
 static double vwl2(const double* x, const double* w, int64_t n)
 {
   double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
   for (int64_t i = 0; i < n; i++) {
     const double p = x[i] * w[i];
     s += p * p;
@@ -928,6 +939,18 @@ int orc_get_stats(const orc_integrator* I, long* out, int nout)
 }
 
 double orc_get_current_h(const orc_integrator* I) { return I->h; }
+
+/* threads of the OpenMP timing build (the checker build: always 1) */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
 double orc_get_h0_used(const orc_integrator* I) { return I->h0_used; }
 double orc_get_last_dsm(const orc_integrator* I) { return I->dsm_last; }
 double orc_get_spectral_radius(const orc_integrator* I) { return I->rho; }

@@ -177,8 +177,10 @@ void orc_stencil(const orc_problem* p, const double* u, double* f)
 {
   const int64_t nx = p->nx, ny = p->ny;
   const int per = (p->kind == ORC_CEDA_PERIODIC);
-  int64_t i, j;
+  int64_t j;
+#pragma omp parallel for schedule(static)
   for (j = 0; j < ny; j++) {
+    int64_t i;
     const double Ds = p->fy[j], Dn = p->fy[j + 1];
     const double* uc = u + j * nx;
     const double* us = (j > 0) ? u + (j - 1) * nx : (per ? u + (ny - 1) * nx : NULL);
@@ -204,6 +206,7 @@ int orc_rhs(orc_problem* p, double t, const double* u, double* f)
     double T1, T2;
     int64_t i, j;
     orc_mms_time(t, &T1, &T2);
+#pragma omp parallel for schedule(static) private(i)
     for (j = 0; j < p->ny; j++)
       for (i = 0; i < p->nx; i++) {
         const double b =

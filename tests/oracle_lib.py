@@ -28,15 +28,18 @@ class Problem(C.Structure):
                [("nrhs", C.c_long)]
 
 
-_lib = None
+# the -O3 / OpenMP build of the same sources, for timing the CPU baseline
+TIMING_LIB_PATH = os.path.join(ROOT, "oracle", "build", "liborc_omp.so")
+
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
-        L = C.CDLL(LIB_PATH)
+def lib(path=LIB_PATH):
+    """The checker (default), or the timing build (path=TIMING_LIB_PATH)."""
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+        L = C.CDLL(path)
         P = C.POINTER(Problem)
         d, i, vp, dp = C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_double)
         sig = {
@@ -78,13 +81,14 @@ def lib():
             "orc_vwrmsnorm": (d, [dp, dp, C.c_int64]),
             "orc_vdot": (d, [dp, dp, C.c_int64]),
             "orc_vmaxnorm": (d, [dp, C.c_int64]),
+            "orc_set_threads": (i, [i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def ptr(a):
@@ -95,19 +99,20 @@ def ptr(a):
 class OracleProblem:
     """Owns an orc_problem; kwargs override the reference defaults."""
 
-    def __init__(self, kind=CEDA, **kw):
+    def __init__(self, kind=CEDA, lib_path=LIB_PATH, **kw):
+        self.L = lib(lib_path)
         self.p = Problem()
-        lib().orc_problem_defaults(C.byref(self.p), kind)
+        self.L.orc_problem_defaults(C.byref(self.p), kind)
         for k, v in kw.items():
             setattr(self.p, k, v)
-        if lib().orc_problem_setup(C.byref(self.p)) != 0:
+        if self.L.orc_problem_setup(C.byref(self.p)) != 0:
             raise ValueError("oracle problem setup failed")
         self.n = self.p.n
         self.shape = (self.p.ny, self.p.nx)
 
     def __del__(self):
         try:
-            lib().orc_problem_free(C.byref(self.p))
+            self.L.orc_problem_free(C.byref(self.p))
         except Exception:
             pass
 
@@ -116,28 +121,28 @@ class OracleProblem:
 
     def initial(self):
         u = np.zeros(self.n)
-        lib().orc_initial(C.byref(self.p), ptr(u))
+        self.L.orc_initial(C.byref(self.p), ptr(u))
         return u
 
     def exact(self, t):
         u = np.zeros(self.n)
-        lib().orc_exact(C.byref(self.p), t, ptr(u))
+        self.L.orc_exact(C.byref(self.p), t, ptr(u))
         return u
 
     def rhs(self, t, u):
         u = np.ascontiguousarray(u, dtype=np.float64)
         f = np.zeros(self.n)
-        lib().orc_rhs(C.byref(self.p), t, ptr(u), ptr(f))
+        self.L.orc_rhs(C.byref(self.p), t, ptr(u), ptr(f))
         return f
 
     def psetup(self, gamma):
-        lib().orc_psetup(C.byref(self.p), gamma)
+        self.L.orc_psetup(C.byref(self.p), gamma)
         return np.ctypeslib.as_array(self.p.diag, shape=(self.n,)).copy()
 
     def psolve(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.zeros(self.n)
-        lib().orc_psolve(C.byref(self.p), ptr(r), ptr(z))
+        self.L.orc_psolve(C.byref(self.p), ptr(r), ptr(z))
         return z
 
     def mg_apply(self, gamma, r, relax=2, nu=2):
@@ -145,7 +150,7 @@ class OracleProblem:
         relax type 2 (symmetric red/black Gauss-Seidel) or 3, nu sweep pairs."""
         r = np.ascontiguousarray(r, dtype=np.float64)
         z = np.zeros(self.n)
-        f = lib().orc_mg_apply_opts
+        f = self.L.orc_mg_apply_opts
         f.restype = C.c_int
         f.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double),
                       C.POINTER(C.c_double)]
@@ -154,7 +159,7 @@ class OracleProblem:
 
     def dom_eig(self):
         lr, li = C.c_double(), C.c_double()
-        lib().orc_dom_eig(C.byref(self.p), C.byref(lr), C.byref(li))
+        self.L.orc_dom_eig(C.byref(self.p), C.byref(lr), C.byref(li))
         return lr.value, li.value
 
 
@@ -168,35 +173,36 @@ class OracleIntegrator:
     def __init__(self, prob, method, y0, t0=0.0):
         self.prob = prob
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
-        self.h = lib().orc_create(C.byref(prob.p), method, t0, ptr(y0))
+        self.L = prob.L
+        self.h = self.L.orc_create(C.byref(prob.p), method, t0, ptr(y0))
         self.n = prob.n
 
     def __del__(self):
         try:
-            lib().orc_free(self.h)
+            self.L.orc_free(self.h)
         except Exception:
             pass
 
     def __getattr__(self, name):
-        f = getattr(lib(), "orc_" + name)
+        f = getattr(self.L, "orc_" + name)
         return lambda *a: f(self.h, *a)
 
     def evolve(self, tout):
         y = np.zeros(self.n)
         t = C.c_double()
-        flag = lib().orc_evolve(self.h, tout, ptr(y), C.byref(t))
+        flag = self.L.orc_evolve(self.h, tout, ptr(y), C.byref(t))
         return flag, t.value, y
 
     def stats(self):
         out = (C.c_long * 32)()
-        m = lib().orc_get_stats(self.h, out, 32)
+        m = self.L.orc_get_stats(self.h, out, 32)
         return {k: out[v] for k, v in ST.items() if v < m}
 
     def linear_solve(self, t, gamma, b, reltol, maxl):
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros(self.n)
         it = C.c_long()
-        ret = lib().orc_test_linear_solve(self.h, t, gamma, ptr(b), ptr(x), reltol, maxl, C.byref(it))
+        ret = self.L.orc_test_linear_solve(self.h, t, gamma, ptr(b), ptr(x), reltol, maxl, C.byref(it))
         return ret, x, it.value
 
 
