@@ -179,6 +179,7 @@ _SIGS = {
     "DiffusionBatch_GetCurrentTime": (i32, [vp, i32, dp]),
     "DiffusionBatch_GetStatsArray": (i32, [vp, i32, C.POINTER(C.c_long), i32]),
     "DiffusionBatch_GetNumGroups": (i32, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    "DiffusionBatch_GetAlgorithmicBytes": (i32, [vp, C.POINTER(d)]),
 }
 
 
@@ -454,6 +455,11 @@ class Batch:
         g, l = C.c_long(), C.c_long()
         lib().DiffusionBatch_GetNumGroups(self.h, C.byref(g), C.byref(l))
         return g.value, l.value
+
+    def algorithmic_bytes(self):
+        v = C.c_double()
+        lib().DiffusionBatch_GetAlgorithmicBytes(self.h, C.byref(v))
+        return v.value
 
     def release(self):
         if self.h and self.ctx.h:

@@ -11,10 +11,12 @@
 // are a prefix of the list; a phase is one launch over that prefix.  Each
 // phase runs up to kpass stages of every problem in it (the multi-stage pass
 // kernel: intermediate stages stay in registers), or its first stage, or its
-// closing RHS.  A segmented reduction then produces every problem's error
-// norm (one device-to-host copy per group).  Phase launches of small
-// problems, which would each be limited by launch latency when the problems
-// are solved one after another, are amortised over the whole batch.
+// closing RHS.  Every problem's error norm is then reduced in the very tree
+// of the single-problem path (raw_sts_error_wsqr, as many virtual workgroups
+// of one launch), so every decision equals that path's bit for bit; one
+// device-to-host copy per group.  Phase launches of small problems, which
+// would each be limited by launch latency when the problems are solved one
+// after another, are amortised over the whole batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,7 +30,6 @@
 
 using namespace ceda_ark;
 using ceda::BatchDesc;
-using ceda::BatchErr;
 using ceda::BatchTile;
 using ceda::StageEntry;
 
@@ -58,17 +59,22 @@ struct DiffusionBatch {
   int smax = 200, stride = 202;
   int tile_rows = 0;
   int kpass = ceda::kBatchMaxK;
+  int layout = ceda::kLayoutAny;  // shared column layout and boundary of the problems
   bool tuned = false;
   long ngroups = 0, nlaunches = 0;
+  // algorithmic HBM bytes of the phase kernels (FP64 cells): stage 1 reads
+  // y_n, F_n and writes Y_1 (24 B); a pass of K stages reads Y_{j-1},
+  // Y_{j-2}, y_n, F_n and writes two stages (48 B); the closing RHS 16 B; the
+  // error norm reads y_n, Y_s, F_n, F_s (32 B)
+  double alg_bytes = 0.0;
   // device/pinned staging (grown on demand)
   BatchDesc* d_desc = nullptr;
   StageEntry *d_ent = nullptr, *h_ent = nullptr;
   BatchTile *d_tiles = nullptr, *h_tiles = nullptr;
-  BatchErr *d_err = nullptr, *h_err = nullptr;
-  int *d_s = nullptr, *h_s = nullptr, *d_tb = nullptr, *h_tb = nullptr, *d_tc = nullptr,
-      *h_tc = nullptr;
+  ceda::StsErrItem *d_err = nullptr, *h_err = nullptr;  // per slot: its error-norm reduction
+  int *d_s = nullptr, *h_s = nullptr;
   double *d_part = nullptr, *d_dsq = nullptr, *h_dsq = nullptr;
-  int cap_slots = 0, cap_tiles = 0;
+  int cap_slots = 0, cap_tiles = 0, cap_vblocks = 0;
 };
 
 namespace {
@@ -86,26 +92,27 @@ void host_alloc(T** h, size_t n)
 
 void free_staging(DiffusionBatch* B)
 {
-  void* ds[] = {B->d_ent, B->d_tiles, B->d_err, B->d_s, B->d_tb, B->d_tc, B->d_part, B->d_dsq};
+  void* ds[] = {B->d_ent, B->d_tiles, B->d_err, B->d_s, B->d_part, B->d_dsq};
   for (void* p : ds)
     if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
-  void* hs[] = {B->h_ent, B->h_tiles, B->h_err, B->h_s, B->h_tb, B->h_tc, B->h_dsq};
+  void* hs[] = {B->h_ent, B->h_tiles, B->h_err, B->h_s, B->h_dsq};
   for (void* p : hs)
     if (p) CEDA_HIP_CHECK(ceda::host_free(p));
   B->d_ent = B->h_ent = nullptr;
   B->d_tiles = B->h_tiles = nullptr;
   B->d_err = B->h_err = nullptr;
-  B->d_s = B->h_s = B->d_tb = B->h_tb = B->d_tc = B->h_tc = nullptr;
+  B->d_s = B->h_s = nullptr;
   B->d_part = B->d_dsq = B->h_dsq = nullptr;
 }
 
-void ensure_staging(DiffusionBatch* B, int slots, int tiles)
+void ensure_staging(DiffusionBatch* B, int slots, int tiles, int vblocks = 0)
 {
-  if (slots <= B->cap_slots && tiles <= B->cap_tiles) return;
+  if (slots <= B->cap_slots && tiles <= B->cap_tiles && vblocks <= B->cap_vblocks) return;
   CEDA_HIP_CHECK(hipStreamSynchronize(B->ex->stream));
   free_staging(B);
   B->cap_slots = std::max(slots, B->cap_slots);
   B->cap_tiles = std::max(tiles, B->cap_tiles);
+  B->cap_vblocks = std::max(vblocks, B->cap_vblocks);
   const size_t ns = B->cap_slots, nt = B->cap_tiles;
   dev_alloc(&B->d_ent, ns * B->stride);
   host_alloc(&B->h_ent, ns * B->stride);
@@ -116,11 +123,7 @@ void ensure_staging(DiffusionBatch* B, int slots, int tiles)
   host_alloc(&B->h_err, ns);
   dev_alloc(&B->d_s, ns);
   host_alloc(&B->h_s, ns);
-  dev_alloc(&B->d_tb, ns);
-  host_alloc(&B->h_tb, ns);
-  dev_alloc(&B->d_tc, ns);
-  host_alloc(&B->h_tc, ns);
-  dev_alloc(&B->d_part, nt);
+  dev_alloc(&B->d_part, B->cap_vblocks);  // the error norms' per-workgroup partials
   dev_alloc(&B->d_dsq, ns);
   host_alloc(&B->h_dsq, ns);
 }
@@ -198,10 +201,10 @@ void autotune_tiles(DiffusionBatch* B)
       CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(BatchTile) * nt,
                                     hipMemcpyHostToDevice, s));
       CEDA_HIP_CHECK(hipMemcpyAsync(B->d_s, B->h_s, sizeof(int) * ks.size(), hipMemcpyHostToDevice, s));
-      ceda::launch_batch_phase(B->d_desc, B->d_tiles, nt, B->d_ent, B->stride, B->d_s, 1, K, 0, s);
+      ceda::launch_batch_phase(B->d_desc, B->d_tiles, nt, B->d_ent, B->stride, B->d_s, 1, K, 0, B->layout, s);
       CEDA_HIP_CHECK(hipEventRecord(e0, s));
       for (int r = 0; r < 3; r++)
-        ceda::launch_batch_phase(B->d_desc, B->d_tiles, nt, B->d_ent, B->stride, B->d_s, 1, K, 0, s);
+        ceda::launch_batch_phase(B->d_desc, B->d_tiles, nt, B->d_ent, B->stride, B->d_s, 1, K, 0, B->layout, s);
       CEDA_HIP_CHECK(hipEventRecord(e1, s));
       CEDA_HIP_CHECK(hipEventSynchronize(e1));
       float ms = 0.0f;
@@ -262,11 +265,14 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
     return B->probs[a].SP.s > B->probs[c].SP.s;
   });
   const bool adaptive = !B->probs[planned[0]].A->fixedstep;
-  int ntiles = 0;
-  for (int k : planned) ntiles += tiles_of(B->probs[k]);
+  int ntiles = 0, nvblocks = 0;
+  for (int k : planned) {
+    ntiles += tiles_of(B->probs[k]);
+    nvblocks += ceda::sts_err_blocks(B->probs[k].P->n);
+  }
   const int nslots = (int)planned.size();
-  ensure_staging(B, nslots, ntiles);
-  int t = 0;
+  ensure_staging(B, nslots, ntiles, nvblocks);
+  int t = 0, vb = 0;
   std::vector<int> tile_end(nslots);
   for (int q = 0; q < nslots; q++) {
     BProb& b = B->probs[planned[q]];
@@ -274,13 +280,14 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
     StageEntry* T = B->h_ent + (size_t)q * B->stride;
     fused_fill(A, b.SP, T, B->kpass);
     B->h_s[q] = b.SP.s;
-    B->h_tb[q] = t;
     list_tiles(b, q, planned[q], B->h_tiles, t);
-    B->h_tc[q] = t - B->h_tb[q];
     tile_end[q] = t;
-    B->h_err[q] = BatchErr{N_VGetDeviceArrayPointer(A->yn), N_VGetDeviceArrayPointer(A->ycur),
-                           N_VGetDeviceArrayPointer(A->fn), N_VGetDeviceArrayPointer(A->tv2),
-                           0.4 * A->h, A->rtol, A->atol, 0.0};
+    // the single-problem path's reduction (raw_sts_error_wsqr), bit for bit
+    const int g = ceda::sts_err_blocks(A->n);
+    B->h_err[q] = ceda::StsErrItem{N_VGetDeviceArrayPointer(A->yn), N_VGetDeviceArrayPointer(A->ycur),
+                                   N_VGetDeviceArrayPointer(A->fn), N_VGetDeviceArrayPointer(A->tv2),
+                                   0.4 * A->h, A->rtol, A->atol, A->n, vb, g};
+    vb += g;
   }
   hipStream_t s = B->ex->stream;
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_ent, B->h_ent, sizeof(StageEntry) * nslots * B->stride,
@@ -288,15 +295,13 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(BatchTile) * ntiles,
                                 hipMemcpyHostToDevice, s));
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_s, B->h_s, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
-  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_err, B->h_err, sizeof(BatchErr) * nslots, hipMemcpyHostToDevice, s));
-  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tb, B->h_tb, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
-  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tc, B->h_tc, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
+  CEDA_HIP_CHECK(hipMemcpyAsync(B->d_err, B->h_err, sizeof(ceda::StsErrItem) * nslots, hipMemcpyHostToDevice, s));
   // phase 0: stage 1 of every slot; phase q >= 1 starts at stage
   // j = 2 + (q-1)*kpass and covers the slots with s >= j (their next pass) or,
   // adaptive, s >= j - kpass (their closing RHS): a prefix of the slots
   const int kp = B->kpass;
   ceda::launch_batch_phase(B->d_desc, B->d_tiles, ntiles, B->d_ent, B->stride, B->d_s, 0, kp,
-                           adaptive ? 1 : 0, s);
+                           adaptive ? 1 : 0, B->layout, s);
   B->nlaunches++;
   int active = nslots;
   for (int q = 1;; q++) {
@@ -305,16 +310,23 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
     while (active > 0 && B->h_s[active - 1] < need) active--;
     if (active == 0) break;
     ceda::launch_batch_phase(B->d_desc, B->d_tiles, tile_end[active - 1], B->d_ent, B->stride,
-                             B->d_s, q, kp, adaptive ? 1 : 0, s);
+                             B->d_s, q, kp, adaptive ? 1 : 0, B->layout, s);
     B->nlaunches++;
   }
   if (adaptive) {
-    ceda::launch_batch_err(B->d_desc, B->d_tiles, B->d_err, ntiles, B->d_part, B->d_tb, B->d_tc,
-                           nslots, B->d_dsq, s);
+    ceda::launch_sts_err_batch(B->d_err, nslots, nvblocks, B->d_part, B->d_dsq, s);
     CEDA_HIP_CHECK(hipMemcpyAsync(B->h_dsq, B->d_dsq, sizeof(double) * nslots,
                                   hipMemcpyDeviceToHost, s));
   }
   CEDA_HIP_CHECK(hipGetLastError());
+  for (int q = 0; q < nslots; q++) {
+    const BProb& b = B->probs[planned[q]];
+    const double cells = (double)b.P->n;
+    const int passes = (b.SP.s - 1 + kp - 1) / kp;
+    const int last_k = (b.SP.s - 1) - kp * (passes - 1);  // a one-stage pass writes one stage: 40 B
+    B->alg_bytes += cells * (24.0 + 48.0 * (passes - 1) + (last_k >= 2 ? 48.0 : 40.0) +
+                             (adaptive ? 16.0 + 32.0 : 0.0));
+  }
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
   B->ngroups++;
   for (int q = 0; q < nslots; q++) {
@@ -372,6 +384,16 @@ DiffusionBatch* DiffusionBatch_Create(int nprob, const DiffusionParams* params,
     b.A->tv3 = N_VClone(b.A->yn);
     set_geometry(b, 16);
   }
+  // one kernel instance per shared layout (two columns per lane: even sides)
+  bool c2 = true, all_per = true, all_mms = true;
+  for (const BProb& b : B->probs) {
+    c2 = c2 && b.cpl == 2;
+    const bool per = b.P->p.kind == DIFFUSION_CEDA_PERIODIC;
+    all_per = all_per && per;
+    all_mms = all_mms && !per;
+  }
+  if (c2 && all_mms) B->layout = ceda::kLayoutC2Mms;
+  else if (c2 && all_per) B->layout = ceda::kLayoutC2Per;
   upload_desc(B);
   return B;
 }
@@ -502,6 +524,13 @@ int DiffusionBatch_GetStatsArray(DiffusionBatch* B, int p, long* out, int nout)
 {
   if (!B || p < 0 || p >= (int)B->probs.size()) return ARK_ILL_INPUT;
   return ARKodeGetStatsArray(B->probs[p].A, out, nout);
+}
+
+int DiffusionBatch_GetAlgorithmicBytes(DiffusionBatch* B, double* bytes)
+{
+  if (!B || !bytes) return ARK_ILL_INPUT;
+  *bytes = B->alg_bytes;
+  return 0;
 }
 
 int DiffusionBatch_GetNumGroups(DiffusionBatch* B, long* ngroups, long* nlaunches)

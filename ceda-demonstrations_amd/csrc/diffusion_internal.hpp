@@ -86,7 +86,7 @@ int pass_halo_lanes(int K, int cpl);
 // ---- batched super-time-stepping (many independent problems, one launch
 // per pass index) ----
 constexpr int kMaxPassStages = 6;  // stages per multi-stage pass (one problem)
-constexpr int kBatchMaxK = 3;  // stages per pass the batch tile geometry covers
+constexpr int kBatchMaxK = 5;  // stages per pass the batch tile geometry covers
 struct BatchDesc {             // per problem, static
   const double *fx, *fy, *sx, *sy, *cx, *cy;
   int64_t nx, ny;
@@ -95,11 +95,10 @@ struct BatchDesc {             // per problem, static
 struct BatchTile {             // one wavefront tile, problems in slot order
   int slot, prob, strip, rb;
 };
-struct BatchErr {              // per slot: operands of the embedded error norm
-  const double *yn, *ys, *fn, *fs;
-  double ch, rtol, atol, pad;
-};
 int batch_halo_lanes(int cpl);
+// column layout and boundary shared by every problem of a batch, so that
+// the phase kernel instance holds only that layout's code and registers
+enum { kLayoutAny = 0, kLayoutC2Mms = 1, kLayoutC2Per = 2 };
 // phase 0: stage 1 of every slot; phase q >= 1: stages j = 2+(q-1)*kmax ..
 // j+kmax-1 (clipped at s = stages[slot]) of the slots with j <= s, and the RHS
 // at t_n + h (adaptive) of the slots whose last pass was phase q-1; entries of
@@ -107,11 +106,8 @@ int batch_halo_lanes(int cpl);
 // slots (a prefix of the slots when they are sorted by s, descending)
 void launch_batch_phase(const BatchDesc* desc, const BatchTile* tiles, int ntiles,
                         const StageEntry* ent, int stride, const int* stages, int phase, int kmax,
-                        int adaptive, hipStream_t s);
-// per-tile sums of (e*ewt)^2, then per-slot sums in tile order (deterministic)
-void launch_batch_err(const BatchDesc* desc, const BatchTile* tiles, const BatchErr* err,
-                      int ntiles, double* partial, const int* tile_begin, const int* tile_count,
-                      int nslots, double* dsq, hipStream_t s);
+                        int adaptive, int layout, hipStream_t s);
+
 
 // one STS stage of the LSRKStep recurrence, fused with the RHS evaluation:
 //   out = c0*F(yjm1) + c1*yjm2 + c2*yn + c3*yjm1 + c4*fn

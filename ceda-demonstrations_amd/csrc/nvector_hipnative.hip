@@ -514,6 +514,73 @@ double raw_sts_error_wsqr(Exec* ex, int64_t n, const double* yn, const double* y
   return run_reduce<RSUM, 1, RStsErr>(ex, n, 1, r)[0];
 }
 
+// Many problems' raw_sts_error_wsqr in two launches.  Problem q's reduction
+// is the grid of reduce_grid(n_q) workgroups of k_reduce<RSUM, 1, RStsErr>,
+// run as virtual workgroups vb0_q .. vb0_q + g_q - 1 of one launch (the same
+// grid-stride partition with g_q in place of gridDim.x, the same wavefront
+// and workgroup trees), then k_reduce_final's combine with one workgroup
+// per problem: every sum is bitwise the single-problem one.
+__global__ __launch_bounds__(kBlock) void k_sts_err_batch(const StsErrItem* __restrict__ items, int nitems,
+                                                          double* __restrict__ partials)
+{
+  // the item whose virtual workgroups contain this one (vb0 ascending)
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].vb0 <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const StsErrItem it = items[lo];
+  const int b = (int)blockIdx.x - it.vb0;
+  const RStsErr r{it.yn, it.ycur, it.fn, it.fs, it.ch, it.rtol, it.atol};
+  double acc[1] = {rinit<RSUM>()};
+  const int64_t n = it.n, np = n >> 1;
+  const int64_t stride = (int64_t)it.g * kBlock;
+  for (int64_t k = (int64_t)b * kBlock + threadIdx.x; k < np; k += stride) r.pair(acc, 2 * k);
+  if ((n & 1) && b == 0 && threadIdx.x == 0) r.one(acc, n - 1);
+  __shared__ double sm[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const double v = wave_reduce<RSUM>(acc[0]);
+  if (lane == 0) sm[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sm[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / kWave; w++) t = rcombine<RSUM>(t, sm[w]);
+    partials[it.vb0 + b] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sts_err_batch_final(const StsErrItem* __restrict__ items,
+                                                                const double* __restrict__ partials,
+                                                                double* __restrict__ out)
+{
+  const StsErrItem it = items[blockIdx.x];
+  __shared__ double sm[kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  double v = rinit<RSUM>();
+  for (int b = threadIdx.x; b < it.g; b += kBlock) v = rcombine<RSUM>(v, partials[it.vb0 + b]);
+  v = wave_reduce<RSUM>(v);
+  if (lane == 0) sm[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sm[0];
+    for (int w = 1; w < kBlock / kWave; w++) t = rcombine<RSUM>(t, sm[w]);
+    out[blockIdx.x] = t;
+  }
+}
+
+int sts_err_blocks(int64_t n) { return (int)reduce_grid(n); }
+
+void launch_sts_err_batch(const StsErrItem* items, int nitems, int nvblocks, double* partials, double* out,
+                          hipStream_t s)
+{
+  if (nitems <= 0 || nvblocks <= 0) return;
+  hipLaunchKernelGGL(k_sts_err_batch, dim3(nvblocks), dim3(kBlock), 0, s, items, nitems, partials);
+  hipLaunchKernelGGL(k_sts_err_batch_final, dim3(nitems), dim3(kBlock), 0, s, items, partials, out);
+  CEDA_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace ceda
 
 // DotProdMulti with more than 8 vectors: groups of 8 vectors in one launch

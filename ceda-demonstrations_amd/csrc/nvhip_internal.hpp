@@ -128,5 +128,16 @@ void raw_linear_sum(hipStream_t s, int64_t n, double a, const double* x, double 
 double raw_sts_error_wsqr(Exec* ex, int64_t n, const double* yn, const double* ycur,
                           const double* fn, const double* fs, double h, double rtol,
                           double atol);
+// the same sum for many problems in two launches, bitwise raw_sts_error_wsqr
+// of each: problem q owns the g = sts_err_blocks(n) partials from vb0 on
+struct StsErrItem {
+  const double *yn, *ycur, *fn, *fs;
+  double ch, rtol, atol;  // ch = 0.4 h
+  int64_t n;
+  int vb0, g;
+};
+int sts_err_blocks(int64_t n);
+void launch_sts_err_batch(const StsErrItem* items, int nitems, int nvblocks, double* partials, double* out,
+                          hipStream_t s);
 
 }  // namespace ceda

@@ -376,20 +376,35 @@ void launch_pass_t(const PassGeo& g, int K, bool nt, const StageEntry* e, hipStr
 }
 
 // ---------------------------------------------------------------- batch
-// Batch tiles use one geometry for every phase of a step group: 1 halo lane
-// (CPL 2) or 2 (CPL 1), which covers passes of up to kBatchMaxK stages.
+// Batch tiles use one geometry for every phase of a step group: the halo
+// lanes of a kBatchMaxK-stage pass (2 with two columns per lane, 4 with one).
 template <int CPL>
-constexpr int batch_halo() { return CPL == 2 ? 1 : 2; }
+constexpr int batch_halo() { return (kBatchMaxK - 1 + CPL - 1) / CPL; }
 
-template <int CPL, bool PER, bool MMS>
+// KMAX: the longest pass of the launch (one kernel instance per KMAX, so a
+// short-pass configuration keeps its own register allocation)
+template <int CPL, bool PER, bool MMS, int KMAX>
 __device__ void batch_wave(int mode, int K, const PassGeo& g, const StageEntry* e, int strip, int rb)
 {
   constexpr int L = batch_halo<CPL>();
-  if (mode == 1) axpy_wave<CPL, L>(g, e, strip, rb);
-  else if (mode == 3) pass_wave<CPL, PER, MMS, 1, L, false, false>(g, e, strip, rb);
-  else if (K == 1) pass_wave<CPL, PER, MMS, 1, L, true, false>(g, e, strip, rb);
-  else if (K == 2) pass_wave<CPL, PER, MMS, 2, L, true, false>(g, e, strip, rb);
-  else pass_wave<CPL, PER, MMS, 3, L, true, false>(g, e, strip, rb);
+  if (mode == 1) {
+    axpy_wave<CPL, L>(g, e, strip, rb);
+    return;
+  }
+  if (mode == 3) {
+    pass_wave<CPL, PER, MMS, 1, L, false, false>(g, e, strip, rb);
+    return;
+  }
+  // 1 <= K <= KMAX (the launch's longest pass)
+  if (K == 1) pass_wave<CPL, PER, MMS, 1, L, true, false>(g, e, strip, rb);
+  if constexpr (KMAX >= 2)
+    if (K == 2) pass_wave<CPL, PER, MMS, 2, L, true, false>(g, e, strip, rb);
+  if constexpr (KMAX >= 3)
+    if (K == 3) pass_wave<CPL, PER, MMS, 3, L, true, false>(g, e, strip, rb);
+  if constexpr (KMAX >= 4)
+    if (K == 4) pass_wave<CPL, PER, MMS, 4, L, true, false>(g, e, strip, rb);
+  if constexpr (KMAX >= 5)
+    if (K == 5) pass_wave<CPL, PER, MMS, 5, L, true, false>(g, e, strip, rb);
 }
 
 __device__ inline PassGeo geo_of(const BatchDesc& D)
@@ -408,12 +423,19 @@ __device__ inline PassGeo geo_of(const BatchDesc& D)
   return g;
 }
 
+// LAYOUT: the problems' column layout and boundary when the whole batch
+// shares them (kLayoutC2Mms: two columns per lane, Dirichlet with forcing;
+// kLayoutC2Per: two columns per lane, periodic), so that the instance holds
+// only that layout's code and registers; kLayoutAny dispatches per tile.
+
+template <int KMAX, int LAYOUT>
 __global__ __launch_bounds__(kBlock) void k_batch_pass(const BatchDesc* __restrict__ desc,
                                                        const BatchTile* __restrict__ tiles,
                                                        int ntiles, const StageEntry* __restrict__ ent,
                                                        int stride, const int* __restrict__ stages,
-                                                       int phase, int kmax, int adaptive)
+                                                       int phase, int adaptive)
 {
+  constexpr int kmax = KMAX;
   const int w = blockIdx.x * kWavesPerBlock +
                 __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
   if (w >= ntiles) return;
@@ -438,74 +460,21 @@ __global__ __launch_bounds__(kBlock) void k_batch_pass(const BatchDesc* __restri
   }
   const BatchDesc D = desc[t.prob];
   const PassGeo g = geo_of(D);
+  if constexpr (LAYOUT == kLayoutC2Mms) {
+    batch_wave<2, false, true, KMAX>(mode, K, g, e, t.strip, t.rb);
+    return;
+  }
+  if constexpr (LAYOUT == kLayoutC2Per) {
+    batch_wave<2, true, false, KMAX>(mode, K, g, e, t.strip, t.rb);
+    return;
+  }
   if (D.cpl == 2) {
-    if (D.per) batch_wave<2, true, false>(mode, K, g, e, t.strip, t.rb);
-    else batch_wave<2, false, true>(mode, K, g, e, t.strip, t.rb);
+    if (D.per) batch_wave<2, true, false, KMAX>(mode, K, g, e, t.strip, t.rb);
+    else batch_wave<2, false, true, KMAX>(mode, K, g, e, t.strip, t.rb);
   } else {
-    if (D.per) batch_wave<1, true, false>(mode, K, g, e, t.strip, t.rb);
-    else batch_wave<1, false, true>(mode, K, g, e, t.strip, t.rb);
+    if (D.per) batch_wave<1, true, false, KMAX>(mode, K, g, e, t.strip, t.rb);
+    else batch_wave<1, false, true, KMAX>(mode, K, g, e, t.strip, t.rb);
   }
-}
-
-__device__ inline double sts_err_term(const BatchErr& E, int64_t o)
-{
-  const double y = E.yn[o];
-  double e = 0.8 * y;
-  e = e + -0.8 * E.ys[o];
-  e = e + E.ch * E.fn[o];
-  e = e + E.ch * E.fs[o];
-  const double w = 1.0 / (E.rtol * fabs(y) + E.atol);
-  return e * w;
-}
-
-// per-tile sums of (e*ewt)^2 over the owned cells of each wavefront tile
-__global__ __launch_bounds__(kBlock) void k_batch_err(const BatchDesc* __restrict__ desc,
-                                                      const BatchTile* __restrict__ tiles,
-                                                      int ntiles, const BatchErr* __restrict__ err,
-                                                      double* __restrict__ partial)
-{
-  const int w = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (w >= ntiles) return;
-  const BatchTile t = tiles[w];
-  const BatchDesc D = desc[t.prob];
-  const BatchErr E = err[t.slot];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int L = (D.cpl == 2) ? batch_halo<2>() : batch_halo<1>();
-  const int64_t ow = (int64_t)(kWave - 2 * L) * D.cpl;
-  const int64_t c = (int64_t)t.strip * ow - (int64_t)L * D.cpl + (int64_t)lane * D.cpl;
-  const bool own = lane >= L && lane < kWave - L && c < D.nx;
-  const int64_t j0 = (int64_t)t.rb * D.rows;
-  const int64_t j1 = (j0 + D.rows < D.ny) ? j0 + D.rows : D.ny;
-  double acc = 0.0;
-  if (own) {
-    for (int64_t r = j0; r < j1; r++)
-      for (int k = 0; k < D.cpl; k++) {
-        const double p = sts_err_term(E, r * D.nx + c + k);
-        acc += p * p;
-      }
-  }
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-  if (lane == 0) partial[w] = acc;
-}
-
-// per-slot sums of the tile partials, in tile order (deterministic)
-__global__ __launch_bounds__(kBlock) void k_batch_err_final(const double* __restrict__ partial,
-                                                            const int* __restrict__ begin,
-                                                            const int* __restrict__ count,
-                                                            double* __restrict__ dsq)
-{
-  const int slot = blockIdx.x;
-  const int b = begin[slot], c = count[slot];
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < c; k += kBlock) acc += partial[b + k];
-  __shared__ double sh[kWavesPerBlock];
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-  if (lane == 0) sh[wave] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) dsq[slot] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
 }  // namespace
@@ -533,24 +502,30 @@ void launch_pass(DiffusionProblem* P, int K, int rows, int nt, const StageEntry*
 
 void launch_batch_phase(const BatchDesc* desc, const BatchTile* tiles, int ntiles,
                         const StageEntry* ent, int stride, const int* stages, int phase, int kmax,
-                        int adaptive, hipStream_t s)
+                        int adaptive, int layout, hipStream_t s)
 {
-  if (ntiles <= 0) return;
+  if (ntiles <= 0 || kmax < 1 || kmax > kBatchMaxK) return;
   const unsigned grid = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
-  hipLaunchKernelGGL(k_batch_pass, dim3(grid), dim3(kBlock), 0, s, desc, tiles, ntiles, ent,
-                     stride, stages, phase, kmax, adaptive);
-}
-
-void launch_batch_err(const BatchDesc* desc, const BatchTile* tiles, const BatchErr* err,
-                      int ntiles, double* partial, const int* tile_begin, const int* tile_count,
-                      int nslots, double* dsq, hipStream_t s)
-{
-  if (ntiles <= 0 || nslots <= 0) return;
-  const unsigned grid = (unsigned)((ntiles + kWavesPerBlock - 1) / kWavesPerBlock);
-  hipLaunchKernelGGL(k_batch_err, dim3(grid), dim3(kBlock), 0, s, desc, tiles, ntiles, err,
-                     partial);
-  hipLaunchKernelGGL(k_batch_err_final, dim3(nslots), dim3(kBlock), 0, s, partial, tile_begin,
-                     tile_count, dsq);
+#define CEDA_BATCH(KM, LY)                                                                          \
+  hipLaunchKernelGGL((k_batch_pass<KM, LY>), dim3(grid), dim3(kBlock), 0, s, desc, tiles, ntiles, ent, \
+                     stride, stages, phase, adaptive);
+#define CEDA_BATCH_K(LY)                                                                            \
+  switch (kmax) {                                                                                   \
+    case 1: CEDA_BATCH(1, LY) break;                                                                \
+    case 2: CEDA_BATCH(2, LY) break;                                                                \
+    case 3: CEDA_BATCH(3, LY) break;                                                                \
+    case 4: CEDA_BATCH(4, LY) break;                                                                \
+    default: CEDA_BATCH(5, LY) break;                                                               \
+  }
+  if (layout == kLayoutC2Mms) {
+    CEDA_BATCH_K(kLayoutC2Mms)
+  } else if (layout == kLayoutC2Per) {
+    CEDA_BATCH_K(kLayoutC2Per)
+  } else {
+    CEDA_BATCH_K(kLayoutAny)
+  }
+#undef CEDA_BATCH_K
+#undef CEDA_BATCH
 }
 
 }  // namespace ceda

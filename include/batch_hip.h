@@ -38,7 +38,7 @@ int DiffusionBatch_SetFixedStep(DiffusionBatch* B, sunrealtype h);
    124-column strip (62 for odd nx); rows 1..256, 0 = choose per size class by
    the tile autotuner */
 int DiffusionBatch_SetTileRows(DiffusionBatch* B, int rows);
-/* stages per pass of the batched multi-stage kernel (1..3, default 3): each
+/* stages per pass of the batched multi-stage kernel (1..5, default 5): each
    phase launch advances every active problem by up to this many stages */
 int DiffusionBatch_SetStagesPerPass(DiffusionBatch* B, int kpass);
 /* advance every problem to tout; returns 0, or the first failing problem's
@@ -51,6 +51,10 @@ int DiffusionBatch_GetCurrentTime(DiffusionBatch* B, int p, sunrealtype* t);
 int DiffusionBatch_GetStatsArray(DiffusionBatch* B, int p, long* out, int nout);
 /* step groups and stage-phase launches issued so far */
 int DiffusionBatch_GetNumGroups(DiffusionBatch* B, long* ngroups, long* nlaunches);
+/* algorithmic HBM bytes the phase kernels have moved so far (FP64 cells:
+   24 B per cell for a first stage, 48 for a pass of up to kpass stages, 16
+   for the closing RHS, 32 for the error norm), for roofline accounting */
+int DiffusionBatch_GetAlgorithmicBytes(DiffusionBatch* B, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -122,3 +122,29 @@ def test_batch_step_cap_reports_too_much_work(pk, ctx):
     B.SetMaxNumSteps(3)
     assert B.evolve(0.5) == -1
     B.release()
+
+
+def test_stages_per_pass_agree_bitwise(pk, ctx):
+    """The batch result does not depend on how many stages a phase launch
+    advances (1..5 per pass, the tile halo covering five): the passes are
+    bitwise equal to single stages."""
+    plist = problems(13, 6, [32, 64, 96])
+    ref = None
+    for kpass in (1, 2, 3, 4, 5):
+        B = pk.Batch(ctx, plist, "rkc")
+        B.SetTolerances(1e-5, 1e-10)
+        B.SetMaxNumSteps(100000)
+        assert B.SetStagesPerPass(kpass) == 0
+        assert B.evolve(0.05) == 0
+        sols = [B.solution(k) for k in range(len(plist))]
+        stats = [B.stats(k)["nfe"] for k in range(len(plist))]
+        B.release()
+        if ref is None:
+            ref = (sols, stats)
+            continue
+        assert stats == ref[1], kpass
+        for a, b in zip(sols, ref[0]):
+            assert np.array_equal(a, b), kpass
+    B = pk.Batch(ctx, plist, "rkc")
+    assert B.SetStagesPerPass(6) != 0 and B.SetStagesPerPass(0) != 0
+    B.release()
