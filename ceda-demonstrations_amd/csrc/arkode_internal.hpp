@@ -35,6 +35,7 @@ struct SUNLinearSolver_ {
   // the state's host copy (pinned); allocated on the first fused solve
   double* d_ws = nullptr;
   double* h_ws = nullptr;
+  double* d_p2 = nullptr;  // its second direction buffer (length of the vectors)
 };
 
 namespace ceda_ark {

@@ -1627,6 +1627,7 @@ int SUNLinSolFree(SUNLinearSolver S)
   for (N_Vector v : S->Z) N_VDestroy(v);
   if (S->d_ws) CEDA_HIP_CHECK(ceda::dev_free(S->d_ws));
   if (S->h_ws) CEDA_HIP_CHECK(ceda::host_free(S->h_ws));
+  if (S->d_p2) CEDA_HIP_CHECK(ceda::dev_free(S->d_p2));
   delete S;
   return 0;
 }

@@ -2,34 +2,37 @@
 // the 5-point stencil of the RHS (the analytic Jacobian-vector product,
 // exactly laplacian()'s arithmetic, diffusion.cpp:48-53) and the Jacobi or
 // multigrid preconditioner.  The algorithm is the integrator's PCG
-// (SUNLinSol_PCG, main.cpp:305; Hestenes-Stiefel with z = M r), with each
-// iteration in three kernels instead of about nine vector operations:
-//   k_pcg_ap:     Ap = p - gamma * J p, and the partial sums of p.Ap
-//   k_pcg_update: alpha = r.z / p.Ap; x += alpha p, r -= alpha Ap,
-//                 z = Dinv r (Jacobi), and the partial sums of the residual
-//                 norm and of r.z
-//   k_pcg_dir:    beta = r.z(new) / r.z(old); the convergence test; p = z + beta p
-// so an iteration streams about 104 bytes per cell (176 unfused).
+// (SUNLinSol_PCG, main.cpp:305; Hestenes-Stiefel with z = M r).
 //
-// The scalars never go to the host.  Every workgroup of k_pcg_update and
-// k_pcg_dir first sums the previous kernel's per-workgroup partials itself,
-// all in one fixed order (so every workgroup gets bitwise the same alpha or
-// beta, and the same value a separate one-workgroup combine would give).
-// k_pcg_dir also applies the residual test and records the outcome in a
-// device status word (running, converged at iteration l, non-finite); every
-// kernel of a later iteration reads the word first and returns at once when
-// the solve has ended.  The host therefore enqueues iterations in batches
-// and reads the status once per batch: the iteration count, x and the
-// statistics are those of a test after every iteration, and a NaN or
-// infinity stops the solve with an error instead of running to the cap.
-// Sums are deterministic; they are not bitwise equal to the separate
-// N_Vector reductions (a different summation order), which is why this path
-// is an option of the integrator (ARKodeSetDiffusionFastPCG) and the
-// N_Vector PCG stays the reference.
+// With Jacobi (or no preconditioner) an iteration l is two kernels:
+//   k_pcg_dirap:  the residual test of iteration l-1 and
+//                 beta = r.z(l-1) / r.z(l-2); the direction
+//                 p_l = z + beta p_{l-1}, formed at the five stencil points
+//                 and stored into the other of two direction buffers;
+//                 Ap = p_l - gamma * J p_l and the partial sums of p_l.Ap
+//   k_pcg_update: alpha = r.z / p.Ap; x += alpha p, r -= alpha Ap,
+//                 z = Dinv r, and the partial sums of |r|^2 and r.z
+// streaming 32 + 64 = 96 bytes per cell (176 for the separate vector
+// operations).  The scalars never go to the host: every workgroup first
+// sums the previous kernel's per-workgroup partials itself, in one fixed
+// order (every workgroup gets bitwise the same alpha or beta, the value a
+// separate one-workgroup combine would give), and the residual test is
+// recorded in a device status word (running, converged at iteration l,
+// non-finite) that every later kernel reads first, returning at once when
+// the solve has ended.  The host enqueues iterations in batches and reads the
+// status once per batch (after a one-workgroup kernel has tested the batch's
+// last iteration): the iteration count, x and the statistics are those of a
+// test after every iteration, and a NaN or infinity stops the solve with an
+// error instead of running to the cap.  The arithmetic of every value is the
+// one of the separate vector operations; only the reductions' summation
+// order differs from them (deterministic, not bitwise equal), which is why
+// this path is an option of the integrator (ARKodeSetDiffusionFastPCG) and
+// the N_Vector PCG stays the reference.
 //
 // With a preconditioner given as a callback (multigrid), z = M r is computed
-// on the host's call between iterations, so that path reads the residual
-// norm and r.z on the host (two waits per iteration, alpha on the device).
+// by the host's call between iterations: that path runs Ap (k_pcg_ap), the
+// update with alpha on the device, r.z (k_pcg_dot) and the direction
+// (k_pcg_dir), reading |r| and r.z on the host (two waits per iteration).
 #include <algorithm>
 #include <cmath>
 
@@ -199,6 +202,77 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, int nparts, co
   block_sums<2>(acc, partials);
 }
 
+// the residual test of iteration l-1 from k_pcg_update's partials (|r|^2,
+// r.z): converged, non-finite (|r| or beta), else running with
+// beta = r.z(l-1) / r.z(l-2).  Workgroup 0 records the outcome; the caller
+// returns when the solve has ended.
+__device__ inline bool test_iteration(int nparts, const double* __restrict__ parts_b, double* __restrict__ st,
+                                      int l, double* beta)
+{
+  const double rr = sum_partials(parts_b, nparts, 2, 0);
+  const double rznew = sum_partials(parts_b, nparts, 2, 1);
+  const double rnorm = sqrt(rr);
+  *beta = rznew / st[(l - 1) & 1];
+  int status = kRunning;
+  if (!isfinite(rnorm)) status = kNonFinite;
+  else if (rnorm <= st[kDelta]) status = kConverged;
+  else if (!isfinite(*beta)) status = kNonFinite;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[kRr] = rr;
+    st[kIter] = l - 1;
+    if (status == kRunning) st[l & 1] = rznew;  // the r.z iteration l uses
+    else st[kStatus] = status;
+  }
+  return status != kRunning;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_dirap(int64_t nx, int64_t ny, const double* __restrict__ fx,
+                                                      const double* __restrict__ fy, int per, double gamma,
+                                                      const double* __restrict__ z,
+                                                      const double* __restrict__ pold, double* __restrict__ pnew,
+                                                      double* __restrict__ ap, int nparts,
+                                                      const double* __restrict__ parts_b,
+                                                      double* __restrict__ parts_a, double* __restrict__ st, int l)
+{
+  if (solve_ended(st)) return;
+  double beta = 0.0;
+  if (l > 1 && test_iteration(nparts, parts_b, st, l, &beta)) return;
+  // p_l at a cell: the first direction is z itself, already in pold
+  auto pv = [&](int64_t k) { return (l > 1) ? z[k] + beta * pold[k] : pold[k]; };
+  double acc[1] = {0.0};
+  for (CellWalk c(nx * ny, nx); c.valid(); c.next()) {
+    const int64_t k = c.k, i = c.i, j = c.j;
+    double vw, ve, vs, vn;
+    if (per) {
+      vw = pv(j * nx + (i > 0 ? i - 1 : nx - 1));
+      ve = pv(j * nx + (i < nx - 1 ? i + 1 : 0));
+      vs = pv((j > 0 ? j - 1 : ny - 1) * nx + i);
+      vn = pv((j < ny - 1 ? j + 1 : 0) * nx + i);
+    } else {
+      vw = i > 0 ? pv(k - 1) : 0.0;
+      ve = i < nx - 1 ? pv(k + 1) : 0.0;
+      vs = j > 0 ? pv(k - nx) : 0.0;
+      vn = j < ny - 1 ? pv(k + nx) : 0.0;
+    }
+    const double pk = pv(k);
+    if (l > 1) pnew[k] = pk;
+    const double jp = cell(fx[i], fx[i + 1], fy[j], fy[j + 1], pk, vw, ve, vs, vn);
+    const double a = pk + (-gamma) * jp;  // N_VLinearSum(1, p, -gamma, Jp) order
+    ap[k] = a;
+    acc[0] += pk * a;
+  }
+  block_sums<1>(acc, parts_a);
+}
+
+// the residual test of the last iteration of a batch, so that the host sees it
+__global__ __launch_bounds__(kBlock) void k_pcg_check(int nparts, const double* __restrict__ parts_b,
+                                                      double* __restrict__ st, int l)
+{
+  if (solve_ended(st)) return;
+  double beta;
+  test_iteration(nparts, parts_b, st, l, &beta);
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcg_dot(int64_t n, const double* __restrict__ a,
                                                     const double* __restrict__ b, double* __restrict__ partials)
 {
@@ -206,42 +280,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dot(int64_t n, const double* __r
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
     acc[0] += a[k] * b[k];
   block_sums<1>(acc, partials);
-}
-
-// iteration l of a solve whose scalars stay on the device: |r|^2 and r.z
-// from k_pcg_update's partials, the residual test (status converged at l),
-// the non-finite test (status non-finite), else beta and p = z + beta p;
-// workgroup 0 records the state for the next iteration and the host
-__global__ __launch_bounds__(kBlock) void k_pcg_dir_dev(int64_t n, int nparts, const double* __restrict__ parts_b,
-                                                        double* __restrict__ st, int l,
-                                                        const double* __restrict__ z, double* __restrict__ p)
-{
-  if (solve_ended(st)) return;
-  const double rr = sum_partials(parts_b, nparts, 2, 0);
-  const double rznew = sum_partials(parts_b, nparts, 2, 1);
-  const double rnorm = sqrt(rr);
-  const double rz = st[l & 1];
-  const double beta = rznew / rz;
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  int status = kRunning;
-  if (!isfinite(rnorm)) status = kNonFinite;
-  else if (rnorm <= st[kDelta]) status = kConverged;
-  else if (!isfinite(beta)) status = kNonFinite;
-  if (status != kRunning) {
-    if (lead) {
-      st[kRr] = rr;
-      st[kIter] = l;
-      st[kStatus] = status;
-    }
-    return;
-  }
-  if (lead) {
-    st[(l + 1) & 1] = rznew;
-    st[kRr] = rr;
-    st[kIter] = l;
-  }
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
-    p[k] = z[k] + beta * p[k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t n, double beta, const double* __restrict__ z,
@@ -287,6 +325,7 @@ void ensure_ws(SUNLinearSolver S)
   if (S->d_ws) return;
   CEDA_HIP_CHECK(dev_malloc(&S->d_ws, sizeof(double) * kWsDoubles));
   CEDA_HIP_CHECK(host_malloc(&S->h_ws, sizeof(double) * kWsDoubles, hipHostMallocDefault));
+  CEDA_HIP_CHECK(dev_malloc(&S->d_p2, sizeof(double) * vlen(S->r)));
 }
 
 // the nacc sums of partials on the host
@@ -350,18 +389,21 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
   if (krylov_nonfinite("fused PCG", "r.z", rz)) return -1;
 
   if (pmode != 2) {
-    // scalars on the device; the host reads the state once per batch
+    // scalars on the device; the host reads the state once per batch.  The
+    // direction of iteration l lives in q[l & 1] (q[1] = S->p holds p_1 = z)
+    double* q[2] = {S->d_p2, dp};
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, st, rz, delta);
     int l = 1, batch = 4;
     while (l <= S->maxl) {
       const int last = std::min(S->maxl, l + batch - 1);
       for (int it = l; it <= last; it++) {
-        hipLaunchKernelGGL(k_pcg_ap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dp,
-                           dap, pa, st);
-        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, dp, dap, dx,
-                           dr, dz, dinv, w, pmode, pb);
-        hipLaunchKernelGGL(k_pcg_dir_dev, dim3(g), dim3(kBlock), 0, s, n, (int)g, pb, st, it, dz, dp);
+        // iteration 1 reads p_1 = z from q[1] and writes nothing
+        hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
+                           it == 1 ? q[1] : q[(it - 1) & 1], q[it & 1], dap, (int)g, pb, pa, st, it);
+        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
+                           dap, dx, dr, dz, dinv, w, pmode, pb);
       }
+      hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, (int)g, pb, st, last + 1);
       CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
       CEDA_HIP_CHECK(hipStreamSynchronize(s));
       const int status = (int)S->h_ws[kStatus];
