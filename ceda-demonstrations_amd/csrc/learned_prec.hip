@@ -373,6 +373,105 @@ __global__ __launch_bounds__(kBlock) void k_conv_chain(const ConvArgs a)
   }
 }
 
+// The first layer (one FP64 input channel, CIN = 1) on the same row chains:
+// the ring holds FP64 input rows of the strip; each lane converts its four
+// taps (4g..4g+3 of the 9) to BF16 after the image's input scale and runs
+// v_mfma_f32_16x16x16bf16_1k, exactly as k_conv<1, 16> does (the same
+// results bit for bit).
+template <bool RELU>
+__global__ __launch_bounds__(kBlock) void k_conv1_chain(const ConvArgs a)
+{
+  extern __shared__ double ring1[];
+  const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
+  const int nx = a.nx, ny = a.ny, d = a.dil;
+  const int span = kChainW + 2 * d;
+  const int bpr = (nx + 15) >> 4;
+  const int strips = (nx + kChainW - 1) / kChainW;
+  const int chains = min(d, ny);
+  const int segs = ((ny + d - 1) / d + kChainR - 1) / kChainR;
+  int64_t t = blockIdx.x;
+  const int strip = (int)(t % strips);
+  t /= strips;
+  const int c = (int)(t % chains);
+  t /= chains;
+  const int seg = (int)(t % segs);
+  const int bimg = (int)(t / segs);
+  const int64_t img = (int64_t)bimg * ny;
+  const int x0 = strip * kChainW;
+  const int y0 = c + seg * kChainR * d;
+  if (y0 >= ny) return;  // workgroup-uniform
+
+  const bfx4 A1 = reinterpret_cast<const bfx4*>(a.apack)[lane];
+  float bv[4] = {0.f, 0.f, 0.f, 0.f};
+  if (a.bias) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) bv[i] = a.bias[4 * g + i];
+  }
+  const double is = a.inv_scale ? a.inv_scale[bimg] : 1.0;
+  const double* in = static_cast<const double*>(a.in);
+  constexpr int kLoads1 = (kChainW + 2 * kChainMaxD + kBlock - 1) / kBlock;  // pixels per thread
+  auto fetch = [&](int yr, double (&v)[kLoads1]) {
+#pragma unroll
+    for (int k = 0; k < kLoads1; k++) {
+      const int px = threadIdx.x + k * kBlock;
+      const int x = x0 - d + px;
+      v[k] = 0.0;
+      if (px < span && yr >= 0 && yr < ny && x >= 0 && x < nx) v[k] = in[(img + yr) * nx + x];
+    }
+  };
+  auto store = [&](int slot, const double (&v)[kLoads1]) {
+#pragma unroll
+    for (int k = 0; k < kLoads1; k++) {
+      const int px = threadIdx.x + k * kBlock;
+      if (px < span) ring1[slot * span + px] = v[k];
+    }
+  };
+  // the ring of k_conv_chain: chain index m in slot m & 3, the next row
+  // fetched while the current one is computed
+  double v[kLoads1];
+  fetch(y0 - d, v);
+  store(3, v);
+  fetch(y0, v);
+  store(0, v);
+  fetch(y0 + d, v);
+  store(1, v);
+  __syncthreads();
+  for (int m = 0; m < kChainR; m++) {
+    const int y = y0 + m * d;
+    if (y >= ny) break;  // workgroup-uniform
+    const bool more = m + 1 < kChainR && y + d < ny;
+    if (more) fetch(y + 2 * d, v);
+#pragma unroll
+    for (int u = 0; u < kConvU; u++) {
+      const int bx = wave * kConvU + u;
+      const int x = x0 + bx * 16 + n;
+      bfx4 B;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int tp = 4 * g + j;
+        const int tc = min(tp, 8);
+        const int xs = x + (tc % 3 - 1) * d, ys = y + (tc / 3 - 1) * d;
+        const double val = ring1[((m + tc / 3 - 1) & 3) * span + bx * 16 + n + d + (tc % 3 - 1) * d];
+        const bool ok = tp < 9 && x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny;
+        B[j] = (short)(ok ? d2bf(val * is) : (unsigned short)0);
+      }
+      const fx4 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, B, fx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      if (x >= nx) continue;
+      bfx4 o;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        float r = acc[i] + bv[i];
+        if (RELU) r = fmaxf(r, 0.0f);
+        if (a.mask && !(bf2f(a.mask[act_off(img + y, bpr, x, g) + i]) > 0.0f)) r = 0.0f;
+        o[i] = (short)f2bf(r);
+      }
+      *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + act_off(img + y, bpr, x, g)) = o;
+    }
+    if (more) store((m + 2) & 3, v);
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------- weight gradient
 struct WGradArgs {
   int nb, nx, ny, dil;
@@ -955,8 +1054,8 @@ void lp_scale(LearnedPrec* L, int nb, int64_t npix, const double* in)
   hipLaunchKernelGGL(k_rms_final, dim3(nb), dim3(kBlock), 0, s, nb, P, npix, L->part, L->sc, L->isc);
 }
 
-// 1: the 16-input-channel layers use the LDS row-chain kernel (dilations up
-// to kChainMaxD), 0: every layer uses k_conv
+// 1: the 16-input-channel layers and the first layer use the LDS row-chain
+// kernels (dilations up to kChainMaxD), 0: every layer uses k_conv
 int g_conv_chain = 1;
 
 template <int CIN, int COUT, bool RELU>
@@ -966,6 +1065,12 @@ void conv(LearnedPrec* L, const ConvArgs& a)
     const int64_t tasks = chain_tasks(a.nb, a.nx, a.ny, a.dil);
     const size_t lds = sizeof(uint4) * 4 * 2 * (size_t)(kChainW + 2 * a.dil);
     hipLaunchKernelGGL((k_conv_chain<COUT, RELU>), dim3((unsigned)tasks), dim3(kBlock), lds, L->ex->stream, a);
+    return;
+  }
+  if (CIN == 1 && COUT == kC && g_conv_chain && a.dil >= 1 && a.dil <= kChainMaxD) {
+    const int64_t tasks = chain_tasks(a.nb, a.nx, a.ny, a.dil);
+    const size_t lds = sizeof(double) * 4 * (size_t)(kChainW + 2 * a.dil);
+    hipLaunchKernelGGL((k_conv1_chain<RELU>), dim3((unsigned)tasks), dim3(kBlock), lds, L->ex->stream, a);
     return;
   }
   // one row segment (kWavesPB * kConvU blocks of 16 pixels) per workgroup

@@ -264,6 +264,116 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirap(int64_t nx, int64_t ny, co
   block_sums<1>(acc, parts_a);
 }
 
+// The same two kernels on pairs of cells (k, k + 1), k even, for an even
+// nx (a pair never straddles a row): 16-byte loads and stores, half the
+// load instructions of the stencil kernel, which reads z and p_{l-1} at the
+// five points.  Every cell value is the one the cell-wise kernels compute;
+// the dot products group their terms by pairs (a thread adds cell 2q, then
+// 2q + 1), so they differ from the cell-wise kernels' sums in the last bits
+// but are as deterministic.
+__device__ inline double2 ld2(const double* __restrict__ a, int64_t k)
+{
+  return *reinterpret_cast<const double2*>(a + k);
+}
+__device__ inline void st2(double* __restrict__ a, int64_t k, double2 v) { *reinterpret_cast<double2*>(a + k) = v; }
+
+template <bool PER>
+__global__ __launch_bounds__(kBlock) void k_pcg_dirap_pairs(int64_t nx, int64_t ny, const double* __restrict__ fx,
+                                                            const double* __restrict__ fy, double gamma,
+                                                            const double* __restrict__ z,
+                                                            const double* __restrict__ pold,
+                                                            double* __restrict__ pnew, double* __restrict__ ap,
+                                                            int nparts, const double* __restrict__ parts_b,
+                                                            double* __restrict__ parts_a, double* __restrict__ st,
+                                                            int l)
+{
+  if (solve_ended(st)) return;
+  double beta = 0.0;
+  if (l > 1 && test_iteration(nparts, parts_b, st, l, &beta)) return;
+  const bool first = (l == 1);
+  auto pv1 = [&](int64_t k) { return first ? pold[k] : z[k] + beta * pold[k]; };
+  auto pv2 = [&](int64_t k) {
+    const double2 pp = ld2(pold, k);
+    if (first) return pp;
+    const double2 zz = ld2(z, k);
+    return make_double2(zz.x + beta * pp.x, zz.y + beta * pp.y);
+  };
+  const double2 zero2 = make_double2(0.0, 0.0);
+  double acc[1] = {0.0};
+  // pair q is cells 2q, 2q + 1 of row j = 2q / nx, columns i, i + 1
+  const int64_t hnx = nx >> 1, npairs = (nx * ny) >> 1;
+  const int64_t step = (int64_t)gridDim.x * kBlock;
+  int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t j = q / hnx, i2 = q - j * hnx;
+  const int64_t dj = step / hnx, di = step - dj * hnx;
+  for (; q < npairs; q += step) {
+    const int64_t k = 2 * q, i = 2 * i2;
+    const double2 c = pv2(k);
+    const double w = i > 0 ? pv1(k - 1) : (PER ? pv1(k + nx - 1) : 0.0);
+    const double e = i + 2 < nx ? pv1(k + 2) : (PER ? pv1(k + 2 - nx) : 0.0);
+    const double2 sv = j > 0 ? pv2(k - nx) : (PER ? pv2(k + (ny - 1) * nx) : zero2);
+    const double2 nv = j < ny - 1 ? pv2(k + nx) : (PER ? pv2(k - (ny - 1) * nx) : zero2);
+    if (!first) st2(pnew, k, c);
+    const double fyj = fy[j], fyj1 = fy[j + 1];
+    const double j0 = cell(fx[i], fx[i + 1], fyj, fyj1, c.x, w, c.y, sv.x, nv.x);
+    const double j1 = cell(fx[i + 1], fx[i + 2], fyj, fyj1, c.y, c.x, e, sv.y, nv.y);
+    const double a0 = c.x + (-gamma) * j0;  // N_VLinearSum(1, p, -gamma, Jp) order
+    const double a1 = c.y + (-gamma) * j1;
+    st2(ap, k, make_double2(a0, a1));
+    acc[0] += c.x * a0;
+    acc[0] += c.y * a1;
+    i2 += di;
+    j += dj;
+    if (i2 >= hnx) {
+      i2 -= hnx;
+      j++;
+    }
+  }
+  block_sums<1>(acc, parts_a);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t n, int nparts,
+                                                             const double* __restrict__ parts_a,
+                                                             const double* __restrict__ st, int l,
+                                                             const double* __restrict__ p,
+                                                             const double* __restrict__ ap, double* __restrict__ x,
+                                                             double* __restrict__ r, double* __restrict__ z,
+                                                             const double* __restrict__ dinv,
+                                                             const double* __restrict__ w, int mode,
+                                                             double* __restrict__ partials)
+{
+  if (solve_ended(st)) return;
+  const double pap = sum_partials(parts_a, nparts, 1, 0);
+  const double alpha = st[l & 1] / pap;
+  double acc[2] = {0.0, 0.0};
+  const int64_t npairs = n >> 1;
+  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npairs; q += (int64_t)gridDim.x * kBlock) {
+    const int64_t k = 2 * q;
+    const double2 xv = ld2(x, k), pv = ld2(p, k), rv = ld2(r, k), av = ld2(ap, k);
+    st2(x, k, make_double2(xv.x + alpha * pv.x, xv.y + alpha * pv.y));
+    const double r0 = rv.x + (-alpha) * av.x, r1 = rv.y + (-alpha) * av.y;
+    st2(r, k, make_double2(r0, r1));
+    double rw0 = r0, rw1 = r1;
+    if (w) {
+      const double2 wv = ld2(w, k);
+      rw0 = r0 * wv.x;
+      rw1 = r1 * wv.y;
+    }
+    acc[0] += rw0 * rw0;
+    acc[0] += rw1 * rw1;
+    double z0 = r0, z1 = r1;
+    if (mode == 1) {
+      const double2 dv = ld2(dinv, k);
+      z0 = dv.x * r0;
+      z1 = dv.y * r1;
+    }
+    st2(z, k, make_double2(z0, z1));
+    acc[1] += r0 * z0;
+    acc[1] += r1 * z1;
+  }
+  block_sums<2>(acc, partials);
+}
+
 // the residual test of the last iteration of a batch, so that the host sees it
 __global__ __launch_bounds__(kBlock) void k_pcg_check(int nparts, const double* __restrict__ parts_b,
                                                       double* __restrict__ st, int l)
@@ -392,16 +502,31 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
     // scalars on the device; the host reads the state once per batch.  The
     // direction of iteration l lives in q[l & 1] (q[1] = S->p holds p_1 = z)
     double* q[2] = {S->d_p2, dp};
+    // pairs of cells with 16-byte accesses: an even row length (every pair in
+    // one row, every pair's first cell 16-byte aligned)
+    const bool pairs = (nx % 2 == 0);
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, st, rz, delta);
     int l = 1, batch = 4;
     while (l <= S->maxl) {
       const int last = std::min(S->maxl, l + batch - 1);
       for (int it = l; it <= last; it++) {
         // iteration 1 reads p_1 = z from q[1] and writes nothing
-        hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
-                           it == 1 ? q[1] : q[(it - 1) & 1], q[it & 1], dap, (int)g, pb, pa, st, it);
-        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
-                           dap, dx, dr, dz, dinv, w, pmode, pb);
+        const double* pold = it == 1 ? q[1] : q[(it - 1) & 1];
+        if (pairs) {
+          if (per)
+            hipLaunchKernelGGL(k_pcg_dirap_pairs<true>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
+                               gamma, dz, pold, q[it & 1], dap, (int)g, pb, pa, st, it);
+          else
+            hipLaunchKernelGGL(k_pcg_dirap_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
+                               gamma, dz, pold, q[it & 1], dap, (int)g, pb, pa, st, it);
+          hipLaunchKernelGGL(k_pcg_update_pairs, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, st, it, q[it & 1],
+                             dap, dx, dr, dz, dinv, w, pmode, pb);
+        } else {
+          hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
+                             pold, q[it & 1], dap, (int)g, pb, pa, st, it);
+          hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
+                             dap, dx, dr, dz, dinv, w, pmode, pb);
+        }
       }
       hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, (int)g, pb, st, last + 1);
       CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));

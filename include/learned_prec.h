@@ -67,9 +67,10 @@ int LearnedPrec_InitIdentity(LearnedPrec* lp, double noise, unsigned long seed);
 int LearnedPrec_SetOutputScale(LearnedPrec* lp, double out_scale);
 double LearnedPrec_GetOutputScale(const LearnedPrec* lp);
 
-/* the convolution kernel of the 16-input-channel layers (every network in
-   the process): 1 (default) sweeps rows of one residue class mod the
-   dilation with the input rows staged in LDS (dilations up to 64), 0 the
+/* the convolution kernel of the 16-input-channel layers and of the first
+   layer (every network in the process): 1 (default) sweeps rows of one
+   residue class mod the dilation with the input rows staged in LDS
+   (dilations up to 64), 0 the
    direct kernel that reads every tap through the caches.  Both give the
    same results bit for bit; -1 for another value. */
 int LearnedPrec_SetConvKernel(int kind);
