@@ -546,8 +546,9 @@ class Integrator:
         return lib().ARKodeSetJacTimes(self.h, None, fn_ptr("JTimes"))
 
     def fused_pcg(self, on=True):
-        """PCG in three fused kernels per iteration (needs analytic_jtimes and
-        the Jacobi or multigrid preconditioner; otherwise the N_Vector PCG)."""
+        """PCG in fused kernels (two per iteration with Jacobi; needs
+        analytic_jtimes and the Jacobi or multigrid preconditioner; otherwise
+        the N_Vector PCG)."""
         return lib().ARKodeSetDiffusionFastPCG(self.h, self.prob.h if on else None)
 
     def fallbacks(self):

@@ -105,11 +105,12 @@ int LSRKStepGetMaxNumStages(void* arkode_mem, int* stage_max);
 /* [ext] fused stencil+stage kernels for a DiffusionProblem RHS; bitwise equal
    to the N_Vector path; use_graphs replays each stage loop as a HIP graph */
 int LSRKStepSetDiffusionFastPath(void* arkode_mem, DiffusionProblem* prob, int use_graphs);
-/* [ext] PCG (SUNLinSol_PCG, main.cpp:305) in three fused kernels per
-   iteration for a DiffusionProblem that is also the user data, when J v is
-   the analytic JTimes (otherwise the N_Vector PCG runs); Jacobi (PSolve) is
-   fused into the update kernel, any other preconditioner is called through
-   its callback.  Same algorithm and stopping test as
+/* [ext] PCG (SUNLinSol_PCG, main.cpp:305) in fused kernels for a
+   DiffusionProblem that is also the user data, when J v is the analytic
+   JTimes (otherwise the N_Vector PCG runs).  With Jacobi (PSolve) or no
+   preconditioner an iteration is two kernels with the scalars kept on the
+   device (the host checks the status every few iterations); any other
+   preconditioner is called through its callback between iterations.  Same algorithm and stopping test as
    the N_Vector PCG, not bitwise equal to it (the dot products are summed in
    another order).  prob = NULL switches it off. */
 int ARKodeSetDiffusionFastPCG(void* arkode_mem, DiffusionProblem* prob);
