@@ -453,7 +453,9 @@ def learned_apply_workload(pk, args):
     ms = (time.perf_counter() - t0) / args.apply_reps * 1e3
     out = {"metric": "symmetric learned network application time", "value": ms, "unit": "ms",
            "config": {"image": f"{n}x{n} FP64", "dilations": list(dil), "reps": args.apply_reps,
-                      "conv_kernel": "LDS row chains" if args.conv_kernel == 1 else "direct"}}
+                      "conv_kernel": {0: "direct", 1: "LDS row chains",
+                                      2: "LDS row chains, two layers per kernel"}[args.conv_kernel],
+                      "pair_rows": args.pair_rows, "pair_ahead": args.pair_ahead}}
     ctx.close()
     print(json.dumps(out))
 
@@ -733,8 +735,11 @@ def parse_args(argv=None):
     ap.add_argument("--c5-workers", type=int, default=8, help="c5: concurrent host threads")
     ap.add_argument("--c5-methods", default="jacobi,mg")
     ap.add_argument("--tout", type=float, default=0.5)
-    ap.add_argument("--conv-kernel", type=int, default=1, choices=[0, 1],
-                    help="learned network's 16-channel layers: 1 LDS row chains, 0 the direct kernel")
+    ap.add_argument("--conv-kernel", type=int, default=1, choices=[0, 1, 2],
+                    help="learned network's convolutions: 1 LDS row chains, 2 LDS row chains and two layers per "
+                         "kernel in the symmetric network's application, 0 the direct kernel")
+    ap.add_argument("--pair-rows", type=int, default=16, help="chain rows per workgroup of the two-layer kernel")
+    ap.add_argument("--pair-ahead", type=int, default=2, help="input rows in flight per workgroup, two-layer kernel")
     ap.add_argument("--kpass", type=int, default=0, help="batch: stages per pass (0: the library default)")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch: the batched solve alone, without the one-after-another comparison")
@@ -752,6 +757,8 @@ def main():
     pk = package()
     if args.conv_kernel != 1:
         assert pk.lib().LearnedPrec_SetConvKernel(args.conv_kernel) == 0
+    if (args.pair_rows, args.pair_ahead) != (16, 2):
+        assert pk.lib().LearnedPrec_SetPairTuning(args.pair_rows, args.pair_ahead) == 0
     if args.kernels_only:
         pin = tuple(int(x) for x in args.pin_pass.split(",")) if args.pin_pass else None
         kernels_only(pk, args.kernels_only, args.sweep, pin)

@@ -146,6 +146,7 @@ _SIGS = {
     "LearnedPrec_Apply": (i32, [vp, i32, i32, i32, vp, vp, i32]),
     "LearnedPrec_ApplyNoAlloc": (i32, [vp, i32, i32, i32, vp, vp, i32]),
     "LearnedPrec_SetConvKernel": (i32, [i32]),
+    "LearnedPrec_SetPairTuning": (i32, [i32, i32]),
     "LearnedPrec_LossGrad": (i32, [vp, i32, i32, i32, d, vp, vp, dp, C.POINTER(C.c_float)]),
     "LearnedPrec_MakeTrainingSet": (i32, [i32, i32, d, d, C.c_ulong, vp, vp, C.POINTER(i32), vp]),
     "LearnedPrec_Train": (i32, [vp, i32, i32, d, vp, vp, i32, i32, d, C.c_ulong, dp]),

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -58,15 +59,9 @@ typedef short bfx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float fx4 __attribute__((ext_vector_type(4)));
 
-__device__ inline unsigned short f2bf(float f)
-{
-  unsigned u = __float_as_uint(f);
-  // a NaN keeps its sign and stays a (quiet) NaN: rounding its payload
-  // could carry into the exponent and give an infinity
-  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (unsigned short)((u >> 16) | 0x0040u);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-  return (unsigned short)(u >> 16);
-}
+// FP32 -> BF16 rounded to nearest even, in one instruction
+// (v_cvt_pk_bf16_f32); a NaN stays a NaN
+__device__ inline unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 __device__ inline float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
 // Blocked activation layout: image rows (row = b*ny + y) of bpr blocks of 16
@@ -329,11 +324,37 @@ __global__ __launch_bounds__(kBlock) void k_conv_chain(const ConvArgs a)
   store(1, v);
   __syncthreads();
   const int h = g & 1;
+  // Per-lane tap constants: operand p of lane group g is tap
+  // tp = min(2p + g/2, 8), in ring row m + tp/3 - 1 at the element offset
+  // col (its half and column).  Pixels past the image's last column only feed
+  // their own (unstored) outputs, and out-of-image taps are zeros in the
+  // ring, so the only operand replaced by zero is the missing 10th tap.
+  int rowt[5], colt[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    const int tp = min(2 * p + (g >> 1), 8);
+    rowt[p] = tp / 3;
+    colt[p] = h * span + n + d + (tp % 3 - 1) * d;
+  }
+  const bool tap9 = (g >> 1) == 1;
+  const uint4 zero = {0u, 0u, 0u, 0u};
+  // the lane's output element offsets within an image row (kConvU pixels)
+  int oofs[kConvU];
+#pragma unroll
+  for (int u = 0; u < kConvU; u++) {
+    const int x = x0 + (wave * kConvU + u) * 16 + n;
+    oofs[u] = COUT == kC ? (int)act_off(0, bpr, x, g) : x;
+  }
   for (int m = 0; m < kChainR; m++) {
     const int y = y0 + m * d;
     if (y >= ny) break;  // workgroup-uniform
     const bool more = m + 1 < kChainR && y + d < ny;
     if (more) fetch(y + 2 * d, v);
+    const int rb[3] = {((m + 3) & 3) * 2 * span, (m & 3) * 2 * span, ((m + 1) & 3) * 2 * span};
+    int off[5];
+#pragma unroll
+    for (int p = 0; p < 5; p++) off[p] = (rowt[p] == 0 ? rb[0] : (rowt[p] == 1 ? rb[1] : rb[2])) + colt[p];
+    const int64_t row = img + y;
 #pragma unroll
     for (int u = 0; u < kConvU; u++) {
       const int bx = wave * kConvU + u;
@@ -341,28 +362,31 @@ __global__ __launch_bounds__(kBlock) void k_conv_chain(const ConvArgs a)
       fx4 acc = fx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int p = 0; p < 5; p++) {
-        const int tp = min(2 * p + (g >> 1), 8);
-        const int slot = (m + tp / 3 - 1) & 3;
-        const int px = bx * 16 + n + d + (tp % 3 - 1) * d;
-        const uint4 bt = ring[(slot * 2 + h) * span + px];
-        const bool ok = 2 * p + (g >> 1) < 9 && x < nx;
-        const uint4 zero = {0u, 0u, 0u, 0u};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A8[p], __builtin_bit_cast(bf16x8, ok ? bt : zero), acc, 0,
-                                                      0, 0);
+        uint4 bt = ring[off[p] + bx * 16];
+        if (p == 4 && tap9) bt = zero;
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A8[p], __builtin_bit_cast(bf16x8, bt), acc, 0, 0, 0);
       }
       if (x >= nx) continue;
-      const int64_t pix = (img + y) * nx + x;
       if (COUT == kC) {
-        bfx4 o;
+        float r[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-          float val = acc[i] + bv[i];
-          if (RELU) val = fmaxf(val, 0.0f);
-          if (a.mask && !(bf2f(a.mask[act_off(img + y, bpr, x, g) + i]) > 0.0f)) val = 0.0f;
-          o[i] = (short)f2bf(val);
+          r[i] = acc[i] + bv[i];
+          if (RELU) r[i] = fmaxf(r[i], 0.0f);
         }
-        *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + act_off(img + y, bpr, x, g)) = o;
+        const int64_t o = row * bpr * 256 + oofs[u];
+        if (a.mask) {  // workgroup-uniform
+          const bfx4 mk = *reinterpret_cast<const bfx4*>(a.mask + o);
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            if (!(bf2f((unsigned short)mk[i]) > 0.0f)) r[i] = 0.0f;
+        }
+        bfx4 ob;
+#pragma unroll
+        for (int i = 0; i < 4; i++) ob[i] = (short)f2bf(r[i]);
+        *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + o) = ob;
       } else if (g == 0) {
+        const int64_t pix = row * nx + oofs[u];
         const double val = (double)(acc[0] + bv[0]) * a.out_mul * (a.scale ? a.scale[bimg] : 1.0);
         double* out = static_cast<double*>(a.out);
         out[pix] = a.addto ? out[pix] + val : val;
@@ -436,11 +460,28 @@ __global__ __launch_bounds__(kBlock) void k_conv1_chain(const ConvArgs a)
   fetch(y0 + d, v);
   store(1, v);
   __syncthreads();
+  // per-lane tap constants (taps 4g .. 4g + 3; out-of-image taps are zeros
+  // in the ring, and the K entries past the 9th tap are zero)
+  int rowt[4], colt[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int tc = min(4 * g + j, 8);
+    rowt[j] = tc / 3;
+    colt[j] = n + d + (tc % 3 - 1) * d;
+  }
+  int oofs[kConvU];
+#pragma unroll
+  for (int u = 0; u < kConvU; u++) oofs[u] = (int)act_off(0, bpr, x0 + (wave * kConvU + u) * 16 + n, g);
   for (int m = 0; m < kChainR; m++) {
     const int y = y0 + m * d;
     if (y >= ny) break;  // workgroup-uniform
     const bool more = m + 1 < kChainR && y + d < ny;
     if (more) fetch(y + 2 * d, v);
+    const int rb[3] = {((m + 3) & 3) * span, (m & 3) * span, ((m + 1) & 3) * span};
+    int off[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) off[j] = (rowt[j] == 0 ? rb[0] : (rowt[j] == 1 ? rb[1] : rb[2])) + colt[j];
+    const int64_t rowo = (img + y) * bpr * 256;
 #pragma unroll
     for (int u = 0; u < kConvU; u++) {
       const int bx = wave * kConvU + u;
@@ -448,27 +489,339 @@ __global__ __launch_bounds__(kBlock) void k_conv1_chain(const ConvArgs a)
       bfx4 B;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const int tp = 4 * g + j;
-        const int tc = min(tp, 8);
-        const int xs = x + (tc % 3 - 1) * d, ys = y + (tc / 3 - 1) * d;
-        const double val = ring1[((m + tc / 3 - 1) & 3) * span + bx * 16 + n + d + (tc % 3 - 1) * d];
-        const bool ok = tp < 9 && x < nx && xs >= 0 && xs < nx && ys >= 0 && ys < ny;
-        B[j] = (short)(ok ? d2bf(val * is) : (unsigned short)0);
+        const double val = ring1[off[j] + bx * 16];
+        B[j] = (short)(4 * g + j < 9 ? d2bf(val * is) : (unsigned short)0);
       }
       const fx4 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, B, fx4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       if (x >= nx) continue;
-      bfx4 o;
+      float r[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        float r = acc[i] + bv[i];
-        if (RELU) r = fmaxf(r, 0.0f);
-        if (a.mask && !(bf2f(a.mask[act_off(img + y, bpr, x, g) + i]) > 0.0f)) r = 0.0f;
-        o[i] = (short)f2bf(r);
+        r[i] = acc[i] + bv[i];
+        if (RELU) r[i] = fmaxf(r[i], 0.0f);
       }
-      *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + act_off(img + y, bpr, x, g)) = o;
+      const int64_t o = rowo + oofs[u];
+      if (a.mask) {  // workgroup-uniform
+        const bfx4 mk = *reinterpret_cast<const bfx4*>(a.mask + o);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if (!(bf2f((unsigned short)mk[i]) > 0.0f)) r[i] = 0.0f;
+      }
+      bfx4 ob;
+#pragma unroll
+      for (int i = 0; i < 4; i++) ob[i] = (short)f2bf(r[i]);
+      *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + o) = ob;
     }
     if (more) store((m + 2) & 3, v);
     __syncthreads();
+  }
+}
+
+// ------------------------------- two layers per sweep (network application)
+// Two consecutive convolutions A (dilation da) and B (dilation db) of the
+// symmetric network's application (no bias, no activation function) in one
+// kernel, the intermediate rows kept in LDS instead of HBM: 64 bytes per
+// pixel less traffic per fused pair.  When one dilation divides the other,
+// both layers act along the rows of one residue class mod g = min(da, db):
+// A couples chain steps k, k +- sa (sa = da / g), B couples k, k +- sb.  A
+// workgroup takes kPairW output columns and `rows` chain steps of B; it
+// sweeps the chain with two rings in LDS,
+//   input rows of the strip (the B columns plus db + da halo pixels on each
+//   side), 2 sa + 2 slots, and
+//   A's output rows (the B columns plus db halo pixels on each side), BF16
+//   exactly as k_conv(_chain) stores them to HBM, 2 sb + 2 slots.
+// Step k computes A's row k and B's row k - sb - 1 (whose A rows k - 2sb - 1
+// .. k - 1 were written in earlier steps) and stores the input row k + sa + 1
+// fetched into registers at the start of the step; the slot counts keep the
+// rows written in a step apart from the rows read in it, so that one barrier
+// per step suffices.  B takes A's output as zero outside the image, as the
+// separate layers' zero padding does.  The operands, the MFMA sequences and
+// the roundings are those of the separate layers: the same results bit for
+// bit.  Workgroups are laid out XCD-major (workgroup b runs on XCD b % 8 and
+// takes the (b / 8)-th task of that XCD's contiguous share), so that the
+// strips of one row segment, which share halo columns, meet in one L2.
+constexpr int kPairThreads = 512;                // 8 wavefronts: one 16-pixel block of B each
+constexpr int kPairWaves = kPairThreads / kWave;
+constexpr int kPairW = kPairWaves * 16;          // output pixels per strip (B)
+constexpr int kPairMaxD = 32;                    // larger dilations use the separate layers
+constexpr int kPairLoads16 =
+    (2 * (16 * ((kPairW + 2 * kPairMaxD + 15) / 16) + 2 * kPairMaxD) + kPairThreads - 1) / kPairThreads;
+constexpr int kPairLoads1 = (16 * ((kPairW + 2 * kPairMaxD + 15) / 16) + 2 * kPairMaxD + kPairThreads - 1) / kPairThreads;
+
+struct PairArgs {
+  int nb, nx, ny, da, db, rows;
+  int64_t tasks;                 // workgroups with work (the grid is rounded up to 8)
+  const void* in;                // FP64 image (CIN_A = 1) or NHWC16 BF16
+  const double* inv_scale;       // CIN_A = 1: per-image input multiplier
+  const unsigned short* apack_a; // packed A operands of layer A
+  const unsigned short* apack_b; // and of layer B
+  void* out;                     // NHWC16 BF16, or an FP64 image (COUT_B = 1)
+  const double* scale;           // COUT_B = 1: per-image output multiplier (nullptr: 1)
+  double out_mul;                // COUT_B = 1: constant output multiplier
+  int addto;                     // COUT_B = 1: out += result
+};
+
+struct PairGeom {
+  int g, sa, sb;  // chain modulus, A and B offsets in chain steps
+  int nba, wa;    // layer A's row: 16-pixel blocks, pixels (kPairW + 2 db rounded up)
+  int wi;         // input row pixels (wa + 2 da)
+  int mi, ma;     // ring slots
+};
+__host__ __device__ inline PairGeom pair_geom(int da, int db)
+{
+  PairGeom q;
+  q.g = da < db ? da : db;
+  q.sa = da / q.g;
+  q.sb = db / q.g;
+  q.nba = (kPairW + 2 * db + 15) / 16;
+  q.wa = 16 * q.nba;
+  q.wi = q.wa + 2 * da;
+  q.mi = 2 * q.sa + 2;
+  q.ma = 2 * q.sb + 2;
+  return q;
+}
+inline size_t pair_lds(int da, int db, int cin_a)
+{
+  const PairGeom q = pair_geom(da, db);
+  return sizeof(uint4) * 2 * (size_t)q.ma * q.wa + (cin_a == 1 ? sizeof(double) : 2 * sizeof(uint4)) * (size_t)q.mi * q.wi;
+}
+// chain steps of a segment: every residue class has ceil(ny / g) rows at most
+inline int64_t pair_tasks(int nb, int nx, int ny, int da, int db, int rows)
+{
+  const PairGeom q = pair_geom(da, db);
+  const int64_t strips = (nx + kPairW - 1) / kPairW;
+  const int64_t chains = std::min(q.g, ny);
+  const int64_t segs = ((ny + q.g - 1) / q.g + rows - 1) / rows;
+  return (int64_t)nb * strips * chains * segs;
+}
+__device__ inline int ring_slot(int k, int m) { return ((k % m) + m) % m; }
+
+template <int CIN_A, int COUT_B, int AHEAD>
+__global__ __launch_bounds__(kPairThreads) void k_conv_pair(const PairArgs a)
+{
+  extern __shared__ uint4 lds[];
+  const int lane = threadIdx.x & (kWave - 1), g = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
+  const int h = g & 1;
+  const int nx = a.nx, ny = a.ny, da = a.da, db = a.db, R = a.rows;
+  const PairGeom q = pair_geom(da, db);
+  const int bpr = (nx + 15) >> 4;
+  const int64_t per_xcd = (a.tasks + 7) >> 3;
+  int64_t t = (int64_t)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if ((blockIdx.x >> 3) >= per_xcd || t >= a.tasks) return;  // workgroup-uniform
+  // task: strip fastest, then chain, then segment, then image
+  const int strips = (nx + kPairW - 1) / kPairW;
+  const int chains = min(q.g, ny);
+  const int segs = ((ny + q.g - 1) / q.g + R - 1) / R;
+  const int strip = (int)(t % strips);
+  t /= strips;
+  const int c = (int)(t % chains);
+  t /= chains;
+  const int seg = (int)(t % segs);
+  const int bimg = (int)(t / segs);
+  const int64_t img = (int64_t)bimg * ny;
+  const int x0 = strip * kPairW;
+  const int y0 = c + seg * R * q.g;  // row of chain step 0 of the segment
+  if (y0 >= ny) return;              // workgroup-uniform: an empty tail segment of a short chain
+  const int xi0 = x0 - db - da;      // first input column of the strip
+
+  uint4* ringA = lds;                                                      // [ma][half][wa]
+  uint4* ringI16 = lds + 2 * q.ma * q.wa;                                  // CIN_A = 16: [mi][half][wi]
+  double* ringI1 = reinterpret_cast<double*>(lds + 2 * q.ma * q.wa);       // CIN_A = 1: [mi][wi]
+
+  bf16x8 A8a[5], A8b[5];
+  bfx4 A1a = {0, 0, 0, 0};
+  if (CIN_A == kC) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) A8a[p] = __builtin_bit_cast(bf16x8, reinterpret_cast<const uint4*>(a.apack_a)[p * kWave + lane]);
+  } else {
+    A1a = reinterpret_cast<const bfx4*>(a.apack_a)[lane];
+  }
+#pragma unroll
+  for (int p = 0; p < 5; p++) A8b[p] = __builtin_bit_cast(bf16x8, reinterpret_cast<const uint4*>(a.apack_b)[p * kWave + lane]);
+  const double is = (CIN_A == 1 && a.inv_scale) ? a.inv_scale[bimg] : 1.0;
+
+  // input row of chain step k (zeros outside the image) into one of AHEAD
+  // register sets: rows are fetched AHEAD steps before they are stored
+  constexpr int kLoads = CIN_A == kC ? kPairLoads16 : kPairLoads1;
+  uint4 v16s[AHEAD][kLoads];
+  double v1s[AHEAD][kLoads];
+  auto fetch = [&](int k, uint4 (&v16)[kLoads], double (&v1)[kLoads]) {
+    const int yr = y0 + k * q.g;
+    const bool row_in = yr >= 0 && yr < ny;
+#pragma unroll
+    for (int i = 0; i < kLoads; i++) {
+      const int ci = threadIdx.x + i * kPairThreads;
+      if (CIN_A == kC) {
+        const int hh = ci >= q.wi ? 1 : 0, px = ci - hh * q.wi;
+        const int x = xi0 + px;
+        v16[i] = uint4{0u, 0u, 0u, 0u};
+        if (ci < 2 * q.wi && row_in && x >= 0 && x < nx)
+          v16[i] = *reinterpret_cast<const uint4*>(static_cast<const unsigned short*>(a.in) + chunk_off(img + yr, bpr, x, hh));
+      } else {
+        const int x = xi0 + ci;
+        v1[i] = 0.0;
+        if (ci < q.wi && row_in && x >= 0 && x < nx) v1[i] = static_cast<const double*>(a.in)[(img + yr) * nx + x];
+      }
+    }
+  };
+  auto store = [&](int slot, const uint4 (&v16)[kLoads], const double (&v1)[kLoads]) {
+#pragma unroll
+    for (int i = 0; i < kLoads; i++) {
+      const int ci = threadIdx.x + i * kPairThreads;
+      if (CIN_A == kC) {
+        if (ci < 2 * q.wi) ringI16[slot * 2 * q.wi + ci] = v16[i];
+      } else {
+        if (ci < q.wi) ringI1[slot * q.wi + ci] = v1[i];
+      }
+    }
+  };
+
+  // A's output row at chain step k (row yA inside the image), columns
+  // x0 - db .. x0 - db + wa - 1, into ring slot k mod ma
+  // Per-lane tap constants, the same in every step: the ring row (0, 1, 2
+  // for k - s, k, k + s) and the element offset within the two halves of a
+  // slot of each operand a lane reads.  Out-of-image input columns and rows
+  // are zeros in the input ring, and A writes zeros for its pixels outside
+  // the image, so the only operand ever replaced by zero is the missing 10th
+  // tap (p = 4, lane groups 2 and 3).
+  int rowB[5], colB[5], rowA[5], colA[5];
+#pragma unroll
+  for (int p = 0; p < 5; p++) {
+    const int tp = min(2 * p + (g >> 1), 8);
+    rowB[p] = tp / 3;
+    colB[p] = h * q.wa + n + db + (tp % 3 - 1) * db;
+    rowA[p] = tp / 3;
+    colA[p] = h * q.wi + n + da + (tp % 3 - 1) * da;
+  }
+  int rowA1[4], colA1[4];  // CIN_A = 1: taps 4g .. 4g + 3
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int tc = min(4 * g + j, 8);
+    rowA1[j] = tc / 3;
+    colA1[j] = n + da + (tc % 3 - 1) * da;
+  }
+  const bool tap9 = (g >> 1) == 1;  // lane groups whose operand p = 4 is the missing tap
+  const uint4 zero = {0u, 0u, 0u, 0u};
+  auto pick = [](int r, const int (&b)[3]) { return r == 0 ? b[0] : (r == 1 ? b[1] : b[2]); };
+
+  // A's output row at chain step k, columns x0 - db .. x0 - db + wa - 1, into
+  // ring slot sA; sI: the ring slots of input rows k - sa, k, k + sa.  Rows
+  // outside the image are written as zeros.
+  auto layer_a = [&](int yA, const int (&sI)[3], int sA) {
+    unsigned short* dst = reinterpret_cast<unsigned short*>(ringA + (size_t)sA * 2 * q.wa) + (g >> 1) * q.wa * 8 +
+                          n * 8 + 4 * (g & 1);
+    if (yA < 0 || yA >= ny) {  // workgroup-uniform
+      for (int bA = wave; bA < q.nba; bA += kPairWaves) *reinterpret_cast<bfx4*>(dst + bA * 128) = bfx4{0, 0, 0, 0};
+      return;
+    }
+    const int bI[3] = {sI[0] * 2 * q.wi, sI[1] * 2 * q.wi, sI[2] * 2 * q.wi};
+    const int bI1[3] = {sI[0] * q.wi, sI[1] * q.wi, sI[2] * q.wi};
+    int off[5], off1[4];
+#pragma unroll
+    for (int p = 0; p < 5; p++) off[p] = pick(rowA[p], bI) + colA[p];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) off1[jj] = pick(rowA1[jj], bI1) + colA1[jj];
+    for (int bA = wave; bA < q.nba; bA += kPairWaves) {  // wavefront-uniform
+      const int xa = x0 - db + 16 * bA + n;
+      fx4 acc = fx4{0.f, 0.f, 0.f, 0.f};
+      if (CIN_A == kC) {
+#pragma unroll
+        for (int p = 0; p < 5; p++) {
+          uint4 bt = ringI16[off[p] + 16 * bA];
+          if (p == 4 && tap9) bt = zero;
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A8a[p], __builtin_bit_cast(bf16x8, bt), acc, 0, 0, 0);
+        }
+      } else {
+        bfx4 B;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          const double val = ringI1[off1[jj] + 16 * bA];
+          B[jj] = (short)(4 * g + jj < 9 ? d2bf(val * is) : (unsigned short)0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1a, B, acc, 0, 0, 0);
+      }
+      // "+ 0.0f": the separate layers add their (zero) bias, which turns a
+      // -0 into +0
+      bfx4 o;
+#pragma unroll
+      for (int i = 0; i < 4; i++) o[i] = (short)f2bf(acc[i] + 0.0f);
+      if (xa < 0 || xa >= nx) o = bfx4{0, 0, 0, 0};
+      *reinterpret_cast<bfx4*>(dst + bA * 128) = o;
+    }
+  };
+
+  // B's output row at chain step j (row yB inside the image); sR: the ring
+  // slots of A's rows j - sb, j, j + sb
+  auto layer_b = [&](int yB, const int (&sR)[3]) {
+    const int bR[3] = {sR[0] * 2 * q.wa, sR[1] * 2 * q.wa, sR[2] * 2 * q.wa};
+    int off[5];
+#pragma unroll
+    for (int p = 0; p < 5; p++) off[p] = pick(rowB[p], bR) + colB[p];
+    {
+      const int bx = wave;
+      const int x = x0 + bx * 16 + n;
+      fx4 acc = fx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int p = 0; p < 5; p++) {
+        uint4 bt = ringA[off[p] + 16 * bx];
+        if (p == 4 && tap9) bt = zero;
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A8b[p], __builtin_bit_cast(bf16x8, bt), acc, 0, 0, 0);
+      }
+      if (x < nx && COUT_B == kC) {
+        bfx4 o;
+#pragma unroll
+        for (int i = 0; i < 4; i++) o[i] = (short)f2bf(acc[i] + 0.0f);
+        *reinterpret_cast<bfx4*>(static_cast<unsigned short*>(a.out) + act_off(img + yB, bpr, x, g)) = o;
+      } else if (x < nx && g == 0) {
+        const int64_t pix = (img + yB) * nx + x;
+        const double val = (double)(acc[0] + 0.0f) * a.out_mul * (a.scale ? a.scale[bimg] : 1.0);
+        double* out = static_cast<double*>(a.out);
+        out[pix] = a.addto ? out[pix] + val : val;
+      }
+    }
+  };
+
+  // B's rows of the segment: chain steps 0 .. nB - 1 (the chain may end
+  // first); A's rows -sb .. nB - 1 + sb; input rows up to kF1
+  const int nB = min(R, (ny - y0 + q.g - 1) / q.g);
+  const int kA0 = -q.sb, kA1 = nB - 1 + q.sb, kLast = nB + q.sb, kF1 = kA1 + q.sa;
+  for (int k = kA0 - q.sa; k <= kA0 + q.sa; k++) {
+    fetch(k, v16s[0], v1s[0]);
+    store(ring_slot(k, q.mi), v16s[0], v1s[0]);
+  }
+#pragma unroll
+  for (int j = 1; j < AHEAD; j++)
+    if (kA0 + q.sa + j <= kF1) fetch(kA0 + q.sa + j, v16s[j], v1s[j]);
+  __syncthreads();
+  // ring slots, kept as k mod mi and k mod ma and moved along with k (no
+  // division in the loop); the offsets below are taken mod the ring sizes
+  // (mi = 2 sa + 2, ma = 2 sb + 2)
+  auto addm = [](int x, int c, int m) { return x + c >= m ? x + c - m : x + c; };
+  int ki = ring_slot(kA0, q.mi), ka = ring_slot(kA0, q.ma);
+  // step s = k - kA0 fetches row k + sa + AHEAD into set s % AHEAD and stores
+  // row k + sa + 1 (fetched AHEAD - 1 steps earlier) from set (s + 1) % AHEAD
+  for (int k0 = kA0; k0 <= kLast; k0 += AHEAD) {
+#pragma unroll
+    for (int i = 0; i < AHEAD; i++) {
+      const int k = k0 + i;
+      if (k <= kLast) {  // workgroup-uniform
+        const int kf = k + q.sa + AHEAD;
+        if (kf <= kF1) fetch(kf, v16s[i], v1s[i]);
+        if (k <= kA1) {
+          const int sI[3] = {addm(ki, q.mi - q.sa, q.mi), ki, addm(ki, q.sa, q.mi)};
+          layer_a(y0 + k * q.g, sI, ka);
+        }
+        const int jB = k - q.sb - 1;  // A's rows jB - sb, jB, jB + sb are k - 2sb - 1, k - sb - 1, k - 1
+        if (jB >= 0) {
+          const int sR[3] = {addm(ka, 1, q.ma), addm(ka, q.sb + 1, q.ma), addm(ka, q.ma - 1, q.ma)};
+          layer_b(y0 + jB * q.g, sR);
+        }
+        if (k + q.sa + 1 <= kF1) store(addm(ki, q.sa + 1, q.mi), v16s[(i + 1) % AHEAD], v1s[(i + 1) % AHEAD]);
+        __syncthreads();
+        ki = addm(ki, 1, q.mi);
+        ka = addm(ka, 1, q.ma);
+      }
+    }
   }
 }
 
@@ -1055,8 +1408,14 @@ void lp_scale(LearnedPrec* L, int nb, int64_t npix, const double* in)
 }
 
 // 1: the 16-input-channel layers and the first layer use the LDS row-chain
-// kernels (dilations up to kChainMaxD), 0: every layer uses k_conv
+// kernels (dilations up to kChainMaxD); 2: as 1, and the symmetric network's
+// application runs two layers per kernel (k_conv_pair) where their dilations
+// allow; 0: every layer uses k_conv.  1 is the fastest: the row-chain kernels
+// stream near the HBM limit, and k_conv_pair, which saves 64 bytes of traffic
+// per pixel and pair, is held back by the occupancy its two LDS rings allow.
 int g_conv_chain = 1;
+int g_pair_rows = 16;  // chain steps per workgroup of k_conv_pair
+int g_pair_ahead = 2;  // input rows in flight per workgroup of k_conv_pair (1..3)
 
 template <int CIN, int COUT, bool RELU>
 void conv(LearnedPrec* L, const ConvArgs& a)
@@ -1078,6 +1437,62 @@ void conv(LearnedPrec* L, const ConvArgs& a)
   const int64_t segs = (int64_t)a.nb * a.ny * (((a.nx + 15) / 16 + kWavesPB * kConvU - 1) / (kWavesPB * kConvU));
   const unsigned grid = (unsigned)std::min<int64_t>((segs + 7) / 8 * 8, 4096);
   hipLaunchKernelGGL((k_conv<CIN, COUT, RELU>), dim3(grid), dim3(kBlock), 0, L->ex->stream, a);
+}
+
+// LDS a workgroup may use on the current device
+size_t lds_per_block()
+{
+  static int v = -1;
+  if (v < 0) {
+    int dev = 0, x = 0;
+    CEDA_HIP_CHECK(hipGetDevice(&dev));
+    CEDA_HIP_CHECK(hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    v = x;
+  }
+  return (size_t)v;
+}
+
+// whether k_conv_pair<CIN_A, COUT_B> can run layers of dilations da, db:
+// one divides the other, both at most kPairMaxD, the rings fit in LDS (above
+// 64 KiB once the kernel has been allowed that much dynamic LDS)
+template <int CIN_A, int COUT_B>
+bool pair_ok(int da, int db)
+{
+  if (da < 1 || db < 1 || da > kPairMaxD || db > kPairMaxD) return false;
+  if ((da < db ? db % da : da % db) != 0) return false;
+  const size_t lds = pair_lds(da, db, CIN_A);
+  if (lds > lds_per_block()) return false;
+  if (lds <= 65536) return true;
+  static std::atomic<int> state{0};  // 0 not tried, 1 allowed, -1 refused
+  if (state.load() == 0) {
+    const void* fs[] = {(const void*)k_conv_pair<CIN_A, COUT_B, 1>, (const void*)k_conv_pair<CIN_A, COUT_B, 2>,
+                        (const void*)k_conv_pair<CIN_A, COUT_B, 3>};
+    bool allowed = true;
+    for (const void* f : fs) {
+      const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_per_block());
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        allowed = false;
+      }
+    }
+    state.store(allowed ? 1 : -1);
+  }
+  return state.load() == 1;
+}
+
+template <int CIN_A, int COUT_B>
+void conv_pair(LearnedPrec* L, PairArgs a)
+{
+  a.rows = g_pair_rows;
+  a.tasks = pair_tasks(a.nb, a.nx, a.ny, a.da, a.db, a.rows);
+  const int64_t grid = (a.tasks + 7) / 8 * 8;
+  const size_t lds = pair_lds(a.da, a.db, CIN_A);
+  if (g_pair_ahead == 1)
+    hipLaunchKernelGGL((k_conv_pair<CIN_A, COUT_B, 1>), dim3((unsigned)grid), dim3(kPairThreads), lds, L->ex->stream, a);
+  else if (g_pair_ahead == 2)
+    hipLaunchKernelGGL((k_conv_pair<CIN_A, COUT_B, 2>), dim3((unsigned)grid), dim3(kPairThreads), lds, L->ex->stream, a);
+  else
+    hipLaunchKernelGGL((k_conv_pair<CIN_A, COUT_B, 3>), dim3((unsigned)grid), dim3(kPairThreads), lds, L->ex->stream, a);
 }
 
 // symmetric network: h_0 = in/s, h_{l+1} = W_l * h_l (l < nl), then the
@@ -1183,32 +1598,72 @@ void lp_apply_in(LearnedPrec* L, int nb, int nx, int ny, const double* in, doubl
     dst = (dst == buf0) ? buf1 : buf0;
   };
   if (L->sym) {
-    for (int l = 0; l < nl; l++) {
-      a.dil = L->dil[l];
-      a.apack = L->d_afw + (int64_t)l * kPack;
-      a.in = src;
-      a.inv_scale = (l == 0) ? isc : nullptr;
-      a.out = dst;
-      if (l == 0) conv<1, kC, false>(L, a);
-      else conv<kC, kC, false>(L, a);
-      advance();
-    }
-    a.inv_scale = nullptr;
-    for (int l = nl - 1; l >= 0; l--) {
-      a.dil = L->dil[l];
-      a.apack = L->d_abw + (int64_t)l * kPack;
-      a.in = src;
-      if (l == 0) {
-        a.out = out;
-        a.scale = sc;
-        a.out_mul = L->out_scale;
-        a.addto = addto;
-        conv<kC, 1, false>(L, a);
+    // the 2 nl layers of the application: layer i < nl is the forward layer
+    // i, layer i >= nl the adjoint of layer 2 nl - 1 - i
+    const int nops = 2 * nl;
+    auto op_dil = [&](int i) { return L->dil[i < nl ? i : nops - 1 - i]; };
+    auto op_pack = [&](int i) {
+      return i < nl ? L->d_afw + (int64_t)i * kPack : L->d_abw + (int64_t)(nops - 1 - i) * kPack;
+    };
+    for (int i = 0; i < nops;) {
+      const bool first = (i == 0);
+      if (g_conv_chain == 2 && i + 1 < nops) {
+        const bool last = (i + 2 == nops);
+        const int da = op_dil(i), db = op_dil(i + 1);
+        const bool ok = first ? (last ? pair_ok<1, 1>(da, db) : pair_ok<1, kC>(da, db))
+                              : (last ? pair_ok<kC, 1>(da, db) : pair_ok<kC, kC>(da, db));
+        if (ok) {
+          PairArgs pa{};
+          pa.nb = nb;
+          pa.nx = nx;
+          pa.ny = ny;
+          pa.da = da;
+          pa.db = db;
+          pa.in = src;
+          pa.inv_scale = first ? isc : nullptr;
+          pa.apack_a = op_pack(i);
+          pa.apack_b = op_pack(i + 1);
+          if (last) {
+            pa.out = out;
+            pa.scale = sc;
+            pa.out_mul = L->out_scale;
+            pa.addto = addto;
+          } else {
+            pa.out = dst;
+          }
+          if (first) {
+            if (last) conv_pair<1, 1>(L, pa);
+            else conv_pair<1, kC>(L, pa);
+          } else {
+            if (last) conv_pair<kC, 1>(L, pa);
+            else conv_pair<kC, kC>(L, pa);
+          }
+          if (!last) advance();
+          i += 2;
+          continue;
+        }
+      }
+      ConvArgs c{};
+      c.nb = nb;
+      c.nx = nx;
+      c.ny = ny;
+      c.dil = op_dil(i);
+      c.apack = op_pack(i);
+      c.in = src;
+      c.inv_scale = first ? isc : nullptr;
+      if (i == nops - 1) {
+        c.out = out;
+        c.scale = sc;
+        c.out_mul = L->out_scale;
+        c.addto = addto;
+        conv<kC, 1, false>(L, c);
       } else {
-        a.out = dst;
-        conv<kC, kC, false>(L, a);
+        c.out = dst;
+        if (first) conv<1, kC, false>(L, c);
+        else conv<kC, kC, false>(L, c);
         advance();
       }
+      i += 1;
     }
   } else {
     for (int l = 0; l < nl; l++) {
@@ -1496,18 +1951,26 @@ double LearnedPrec_GetOutputScale(const LearnedPrec* L) { return L ? L->out_scal
 
 int LearnedPrec_SetConvKernel(int kind)
 {
-  if (kind != 0 && kind != 1) return -1;
+  if (kind < 0 || kind > 2) return -1;
   g_conv_chain = kind;
+  return 0;
+}
+
+int LearnedPrec_SetPairTuning(int rows, int ahead)
+{
+  if (rows < 1 || rows > 1024 || ahead < 1 || ahead > 3) return -1;
+  g_pair_rows = rows;
+  g_pair_ahead = ahead;
   return 0;
 }
 
 int LearnedPrec_Apply(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto)
 {
   if (!L || nb < 1 || nx < 1 || ny < 1 || !in || !out) return -1;
-  const int64_t npix = (int64_t)nx * ny;
   lp_workspace(L, nb, nx, ny);
-  lp_scale(L, nb, npix, in);
-  lp_forward(L, nb, nx, ny, in, out, addto, false);
+  // the application keeps no activations: two of the training buffers serve
+  // as the ping-pong pair
+  lp_apply_in(L, nb, nx, ny, in, out, addto, L->gA, L->gB, L->sc, L->isc, L->part);
   return 0;
 }
 

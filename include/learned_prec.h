@@ -67,13 +67,19 @@ int LearnedPrec_InitIdentity(LearnedPrec* lp, double noise, unsigned long seed);
 int LearnedPrec_SetOutputScale(LearnedPrec* lp, double out_scale);
 double LearnedPrec_GetOutputScale(const LearnedPrec* lp);
 
-/* the convolution kernel of the 16-input-channel layers and of the first
-   layer (every network in the process): 1 (default) sweeps rows of one
-   residue class mod the dilation with the input rows staged in LDS
-   (dilations up to 64), 0 the
-   direct kernel that reads every tap through the caches.  Both give the
-   same results bit for bit; -1 for another value. */
+/* the convolution kernels (every network in the process): 1 (default)
+   sweeps rows of one residue class mod the dilation with the input rows
+   staged in LDS (dilations up to 64; the 16-input-channel layers and the
+   first layer), 2 does the same and, in the symmetric network's
+   application, runs two consecutive layers per kernel with the intermediate
+   rows in LDS where one dilation divides the other (both at most 32), 0 the
+   direct kernel that reads every tap through the caches.  All give the same
+   results bit for bit; -1 for another value. */
 int LearnedPrec_SetConvKernel(int kind);
+/* tuning of the two-layer kernel: chain rows per workgroup (default 16;
+   1..1024) and input rows in flight per workgroup (default 2; 1..3); -1 for
+   other values.  The results do not depend on them. */
+int LearnedPrec_SetPairTuning(int rows, int ahead);
 
 /* out = N(in) for nb images of nx*ny pixels (image-major, x fastest), FP64
    device arrays; addto = 1 adds N(in) to out */

@@ -354,15 +354,21 @@ def _set_conv(pk, kind):
     assert pk.lib().LearnedPrec_SetConvKernel(kind) == 0
 
 
-@pytest.mark.parametrize("symmetric,dil", [(True, BENCH_DIL), (True, (1, 2, 6, 18)), (False, DIL)])
+@pytest.mark.parametrize("symmetric,dil", [(True, BENCH_DIL), (True, (1, 2, 6, 18)), (True, (2, 3, 33, 1)),
+                                           (True, (5,)), (False, DIL)])
 @pytest.mark.parametrize("nb,nx,ny", [(1, 300, 211), (2, 70, 83), (3, 37, 21), (1, 128, 96), (1, 16, 130)])
 def test_row_chain_convolution_is_bitwise_the_direct_one(pk, ctx, symmetric, dil, nb, nx, ny):
     """LearnedPrec_SetConvKernel(1) (input rows of one residue class mod the
-    dilation staged in LDS) and (0) (every tap read through the caches) must
-    give the same network output and training gradient bit for bit: the
-    same operands in the same MFMA sequence.  Shapes cover strips narrower
-    than one image, images narrower than a strip, fewer rows than the
-    dilation (21 < 27) and a batch of images."""
+    dilation staged in LDS), (2) (as 1, and two layers per kernel in the
+    symmetric network's application) and (0) (every tap read through the
+    caches) must give the same network output and training gradient bit for
+    bit: the same operands in the same MFMA sequence.  Shapes cover strips
+    narrower than one image, images narrower than a strip, fewer rows than
+    the dilation (21 < 27) and a batch of images; the dilations cover layer
+    pairs whose chain offsets are 1 and 3 in either order (1,3 / 9,27 and
+    their adjoints), a pair of equal dilations and the first and last layer
+    in one kernel (a one-layer network), and pairs that cannot be fused (2,3;
+    33 > 32) next to one that can (1,1)."""
     net = pk.LearnedPrec(ctx, dil, symmetric=symmetric)
     p = random_params(net.nparams, 31 + nx, 0.2)
     net.set_params(p)
@@ -375,7 +381,7 @@ def test_row_chain_convolution_is_bitwise_the_direct_one(pk, ctx, symmetric, dil
     E.set(e)
     out = {}
     try:
-        for kind in (0, 1):
+        for kind in (0, 1, 2):
             _set_conv(pk, kind)
             assert net.apply(nb, nx, ny, R, Z) == 0
             loss, g = net.loss_grad(nb, nx, ny, 40.0, R, E)
@@ -383,12 +389,40 @@ def test_row_chain_convolution_is_bitwise_the_direct_one(pk, ctx, symmetric, dil
     finally:
         _set_conv(pk, 1)
     assert np.all(np.isfinite(out[1][0])) and np.abs(out[1][0]).max() > 0
-    assert np.array_equal(out[0][0], out[1][0])
-    assert out[0][1] == out[1][1]
-    assert np.array_equal(out[0][2], out[1][2])
+    for kind in (1, 2):
+        # equal bits, including the sign of zeros
+        assert np.array_equal(out[0][0].view(np.uint64), out[kind][0].view(np.uint64)), kind
+        assert out[0][1] == out[kind][1]
+        assert np.array_equal(out[0][2], out[kind][2])
+    net.release()
+
+
+@pytest.mark.parametrize("rows,ahead", [(1, 1), (5, 3), (64, 2), (16, 1), (16, 3), (3, 2)])
+def test_pair_kernel_tuning_does_not_change_the_result(pk, ctx, rows, ahead):
+    """The two-layer kernel's chain rows per workgroup and rows in flight
+    split and schedule the work only."""
+    net = pk.LearnedPrec(ctx, BENCH_DIL, symmetric=True)
+    net.set_params(random_params(net.nparams, 5, 0.2))
+    rng = np.random.default_rng(rows)
+    nx, ny = 200, 173
+    R, Z = pk.Vector(ctx, n=nx * ny), pk.Vector(ctx, n=nx * ny)
+    R.set(rng.standard_normal(nx * ny))
+    out = []
+    _set_conv(pk, 2)
+    try:
+        for r, a in ((16, 2), (rows, ahead)):
+            assert pk.lib().LearnedPrec_SetPairTuning(r, a) == 0
+            assert net.apply(1, nx, ny, R, Z) == 0
+            out.append(Z.get())
+    finally:
+        assert pk.lib().LearnedPrec_SetPairTuning(16, 2) == 0
+        _set_conv(pk, 1)
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
     net.release()
 
 
 def test_conv_kernel_switch_rejects_other_values(pk):
-    assert pk.lib().LearnedPrec_SetConvKernel(2) == -1
+    assert pk.lib().LearnedPrec_SetConvKernel(3) == -1
     assert pk.lib().LearnedPrec_SetConvKernel(-1) == -1
+    assert pk.lib().LearnedPrec_SetPairTuning(0, 2) == -1
+    assert pk.lib().LearnedPrec_SetPairTuning(16, 4) == -1
