@@ -906,6 +906,8 @@ int PSetup(sunrealtype, N_Vector, N_Vector, sunbooleantype, sunbooleantype* jcur
   hipLaunchKernelGGL(k_psetup, dim3(stream_grid(2 * P->n)), dim3(kBlock), 0, vstream(P->diag),
                      P->p.nx, P->p.ny, per ? P->d_px : P->d_fx, per ? P->d_py : P->d_fy, gamma,
                      dptr(P->diag));
+  P->diag_gamma = gamma;
+  P->diag_gamma_set = true;
   if (jcurPtr) *jcurPtr = SUNTRUE;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -35,6 +35,8 @@ struct DiffusionProblem {
   double* d_tf = nullptr;   // device time factors (T1, T2) for the plain RHS
   double* h_tf = nullptr;   // pinned staging for d_tf
   N_Vector diag = nullptr;  // Jacobi inverse diagonal (udata->diag, main.cpp:223)
+  double diag_gamma = 0.0;       // the gamma of the last PSetup, which computed diag
+  bool diag_gamma_set = false;   // diag was last written by PSetup
   SUNContext ctx = nullptr;
   ceda::Exec* exec = nullptr;
   long nrhs = 0;

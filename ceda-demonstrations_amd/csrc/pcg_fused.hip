@@ -11,8 +11,9 @@
 //                 and stored into the other of two direction buffers;
 //                 Ap = p_l - gamma * J p_l and the partial sums of p_l.Ap
 //   k_pcg_update: alpha = r.z / p.Ap; x += alpha p, r -= alpha Ap,
-//                 z = Dinv r, and the partial sums of |r|^2 and r.z
-// streaming 32 + 64 = 96 bytes per cell (176 for the separate vector
+//                 z = Dinv r (Dinv formed from the face coefficients, not
+//                 read), and the partial sums of |r|^2 and r.z
+// streaming 32 + 56 = 88 bytes per cell (176 for the separate vector
 // operations).  The scalars never go to the host: every workgroup first
 // sums the previous kernel's per-workgroup partials itself, in one fixed
 // order (every workgroup gets bitwise the same alpha or beta, the value a
@@ -271,11 +272,48 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirap(int64_t nx, int64_t ny, co
 // the dot products group their terms by pairs (a thread adds cell 2q, then
 // 2q + 1), so they differ from the cell-wise kernels' sums in the last bits
 // but are as deterministic.
+// XCD-aware walk over [0, npairs): workgroup b runs on XCD b % 8 and, when
+// the grid is a multiple of 8, walks the (b % 8)-th of 8 contiguous ranges,
+// so that the rows above and below a pair are read by the same XCD at about
+// the same time and come from its L2
+struct PairWalk {
+  int64_t q, end, step;
+};
+__device__ inline PairWalk xcd_pairs(int64_t npairs)
+{
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const int parts = (G % 8 == 0) ? 8 : 1;
+  const int64_t per = (npairs + parts - 1) / parts;
+  const int part = (int)(b % parts);
+  PairWalk w;
+  w.q = part * per + (int64_t)(b / parts) * kBlock + threadIdx.x;
+  w.end = min(npairs, (part + 1) * per);
+  w.step = (int64_t)(G / parts) * kBlock;
+  return w;
+}
+
 __device__ inline double2 ld2(const double* __restrict__ a, int64_t k)
 {
   return *reinterpret_cast<const double2*>(a + k);
 }
 __device__ inline void st2(double* __restrict__ a, int64_t k, double2 v) { *reinterpret_cast<double2*>(a + k) = v; }
+
+// With JAC the update kernel forms the Jacobi inverse diagonal itself
+// instead of reading it: dinv = 1 / (1 - gp d), d = -((ax[i] + ax[i+1]) +
+// (ay[j] + ay[j+1])), exactly k_psetup's value for the gamma gp of the last
+// preconditioner setup: 88 bytes per cell and iteration instead of 96.  (Not
+// storing z at all and forming it at the stencil's five points instead
+// saves 16 more bytes, but the eight divisions per pair of cells make the
+// stencil kernel slower than the traffic saved.)
+struct JacobiDiag {
+  const double *ax, *ay;
+  double gp;
+};
+__device__ inline double jac_dinv(const JacobiDiag& J, double sxi, double syj)
+{
+  const double d = -(sxi + syj);
+  return 1.0 / (1.0 - J.gp * d);
+}
 
 template <bool PER>
 __global__ __launch_bounds__(kBlock) void k_pcg_dirap_pairs(int64_t nx, int64_t ny, const double* __restrict__ fx,
@@ -302,11 +340,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirap_pairs(int64_t nx, int64_t 
   double acc[1] = {0.0};
   // pair q is cells 2q, 2q + 1 of row j = 2q / nx, columns i, i + 1
   const int64_t hnx = nx >> 1, npairs = (nx * ny) >> 1;
-  const int64_t step = (int64_t)gridDim.x * kBlock;
-  int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const PairWalk wk = xcd_pairs(npairs);
+  const int64_t step = wk.step;
+  int64_t q = wk.q;
   int64_t j = q / hnx, i2 = q - j * hnx;
   const int64_t dj = step / hnx, di = step - dj * hnx;
-  for (; q < npairs; q += step) {
+  for (; q < wk.end; q += step) {
     const int64_t k = 2 * q, i = 2 * i2;
     const double2 c = pv2(k);
     const double w = i > 0 ? pv1(k - 1) : (PER ? pv1(k + nx - 1) : 0.0);
@@ -332,13 +371,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirap_pairs(int64_t nx, int64_t 
   block_sums<1>(acc, parts_a);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t n, int nparts,
+template <bool JAC>
+__global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t nx, int64_t ny, int nparts,
                                                              const double* __restrict__ parts_a,
                                                              const double* __restrict__ st, int l,
                                                              const double* __restrict__ p,
                                                              const double* __restrict__ ap, double* __restrict__ x,
                                                              double* __restrict__ r, double* __restrict__ z,
-                                                             const double* __restrict__ dinv,
+                                                             const double* __restrict__ dinv, JacobiDiag J,
                                                              const double* __restrict__ w, int mode,
                                                              double* __restrict__ partials)
 {
@@ -346,8 +386,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t n, int npar
   const double pap = sum_partials(parts_a, nparts, 1, 0);
   const double alpha = st[l & 1] / pap;
   double acc[2] = {0.0, 0.0};
-  const int64_t npairs = n >> 1;
-  for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npairs; q += (int64_t)gridDim.x * kBlock) {
+  const int64_t hnx = nx >> 1, npairs = (nx * ny) >> 1;
+  const PairWalk wk = xcd_pairs(npairs);
+  const int64_t step = wk.step;
+  int64_t q = wk.q;
+  int64_t j = q / hnx, i2 = q - j * hnx;
+  const int64_t dj = step / hnx, di = step - dj * hnx;
+  for (; q < wk.end; q += step) {
     const int64_t k = 2 * q;
     const double2 xv = ld2(x, k), pv = ld2(p, k), rv = ld2(r, k), av = ld2(ap, k);
     st2(x, k, make_double2(xv.x + alpha * pv.x, xv.y + alpha * pv.y));
@@ -362,7 +407,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t n, int npar
     acc[0] += rw0 * rw0;
     acc[0] += rw1 * rw1;
     double z0 = r0, z1 = r1;
-    if (mode == 1) {
+    if (JAC) {
+      const int64_t i = 2 * i2;
+      const double syj = J.ay[j] + J.ay[j + 1];
+      z0 = jac_dinv(J, J.ax[i] + J.ax[i + 1], syj) * r0;
+      z1 = jac_dinv(J, J.ax[i + 1] + J.ax[i + 2], syj) * r1;
+    } else if (mode == 1) {
       const double2 dv = ld2(dinv, k);
       z0 = dv.x * r0;
       z1 = dv.y * r1;
@@ -370,6 +420,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t n, int npar
     st2(z, k, make_double2(z0, z1));
     acc[1] += r0 * z0;
     acc[1] += r1 * z1;
+    i2 += di;
+    j += dj;
+    if (i2 >= hnx) {
+      i2 -= hnx;
+      j++;
+    }
   }
   block_sums<2>(acc, partials);
 }
@@ -505,6 +561,10 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
     // pairs of cells with 16-byte accesses: an even row length (every pair in
     // one row, every pair's first cell 16-byte aligned)
     const bool pairs = (nx % 2 == 0);
+    // the Jacobi inverse diagonal formed in the update kernel (from the gamma
+    // of its last setup, which is what P->diag holds)
+    const bool jac = (pmode == 1 && P->diag_gamma_set);
+    const JacobiDiag J{per ? P->d_px : P->d_fx, per ? P->d_py : P->d_fy, P->diag_gamma};
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, st, rz, delta);
     int l = 1, batch = 4;
     while (l <= S->maxl) {
@@ -519,8 +579,12 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
           else
             hipLaunchKernelGGL(k_pcg_dirap_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
                                gamma, dz, pold, q[it & 1], dap, (int)g, pb, pa, st, it);
-          hipLaunchKernelGGL(k_pcg_update_pairs, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, st, it, q[it & 1],
-                             dap, dx, dr, dz, dinv, w, pmode, pb);
+          if (jac)
+            hipLaunchKernelGGL(k_pcg_update_pairs<true>, dim3(g), dim3(kBlock), 0, s, nx, ny, (int)g, pa, st, it,
+                               q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
+          else
+            hipLaunchKernelGGL(k_pcg_update_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, (int)g, pa, st, it,
+                               q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
         } else {
           hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
                              pold, q[it & 1], dap, (int)g, pb, pa, st, it);

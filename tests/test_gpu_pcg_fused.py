@@ -1,5 +1,5 @@
-"""The fused PCG of the implicit path (three kernels per iteration,
-ARKodeSetDiffusionFastPCG) solves the same systems as the N_Vector PCG
+"""The fused PCG of the implicit path (two kernels per iteration with
+Jacobi, ARKodeSetDiffusionFastPCG) solves the same systems as the N_Vector PCG
 (SUNLinSol_PCG with the analytic J v): the same steps and Newton iterations,
 Krylov iterations within 1 %, final states within 1e-10, for the Jacobi and
 multigrid preconditioners and none, both problems and both residual norms
@@ -27,11 +27,14 @@ def ctx(pk):
     c.close()
 
 
-def dirk(pk, ctx, kind, side, dt, tout, precond, fused, reltol=1e-8, jtimes=True):
-    G = pk.Problem(ctx, kind, nx=side, ny=side, kx=1.0, ky=0.7)
+def dirk(pk, ctx, kind, side, dt, tout, precond, fused, reltol=1e-8, jtimes=True, ny=None):
+    G = pk.Problem(ctx, kind, nx=side, ny=ny or side, kx=1.0, ky=0.7)
     I = pk.Integrator(G, "dirk", G.initial(), ls="cg", maxl=20000, precond=precond)
     I.ARKodeSetOrder(3)
-    I.ARKodeSetFixedStep(dt)
+    if dt:
+        I.ARKodeSetFixedStep(dt)
+    else:
+        I.ARKodeSStolerances(1e-6, 1e-10)
     if reltol:
         I.ARKodeSetLSRelativeTolerance(reltol)
     if jtimes:
@@ -64,6 +67,26 @@ def test_fused_pcg_matches_nvector_pcg(pk, ctx, kind, precond, reltol):
     if precond is True:
         assert got["stats"]["npsolve"] == ref["stats"]["npsolve"] or \
             abs(got["stats"]["npsolve"] - ref["stats"]["npsolve"]) <= max(2, 0.02 * ref["stats"]["npsolve"])
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_fused_pcg_with_adaptive_steps(pk, ctx, kind):
+    """Adaptive steps change gamma between preconditioner setups; the fused
+    Jacobi PCG must apply the inverse diagonal of the last setup (what the
+    N_Vector PCG's PSolve uses), not one of the current gamma."""
+    ref = dirk(pk, ctx, kind, 64, None, 0.02, True, False)
+    got = dirk(pk, ctx, kind, 64, None, 0.02, True, True)
+    assert ref["stats"]["nst"] > 3
+    same_solve(got, ref)
+
+
+@pytest.mark.parametrize("side,ny", [(63, 40), (50, 37)])
+def test_fused_pcg_odd_and_even_rows(pk, ctx, side, ny):
+    """Rows of odd length take the cell-wise kernels, even ones the pair
+    kernels."""
+    ref = dirk(pk, ctx, pk.MMS_DIRICHLET, side, 4e-3, 1.2e-2, True, False, ny=ny)
+    got = dirk(pk, ctx, pk.MMS_DIRICHLET, side, 4e-3, 1.2e-2, True, True, ny=ny)
+    same_solve(got, ref)
 
 
 def test_fused_pcg_matches_oracle(pk, ctx):
