@@ -36,6 +36,7 @@ struct SUNLinearSolver_ {
   double* d_ws = nullptr;
   double* h_ws = nullptr;
   double* d_p2 = nullptr;  // its second direction buffer (length of the vectors)
+  long last_iters = 0;     // iterations of the last fused solve (sizes its first batch)
 };
 
 namespace ceda_ark {

@@ -30,10 +30,10 @@
 // this path is an option of the integrator (ARKodeSetDiffusionFastPCG) and
 // the N_Vector PCG stays the reference.
 //
-// With a preconditioner given as a callback (multigrid), z = M r is computed
-// by the host's call between iterations: that path runs Ap (k_pcg_ap), the
-// update with alpha on the device, r.z (k_pcg_dot) and the direction
-// (k_pcg_dir), reading |r| and r.z on the host (two waits per iteration).
+// With a preconditioner given as a callback (multigrid, the learned
+// network), the update leaves z alone, the callback's z = M r is enqueued on
+// the same stream after it, and k_pcg_dot sums r.z for the next iteration's
+// test and beta: the scalars stay on the device there too.
 #include <algorithm>
 #include <cmath>
 
@@ -49,6 +49,7 @@ namespace {
 
 constexpr int kPcgBlocks = 1024;   // workgroups of the reduction kernels
 constexpr int kPcgBatchMax = 32;   // iterations enqueued between host checks
+constexpr int kPcgChunk = 4;       // callback preconditioner: iterations between host waits in the bulk
 // workspace layout (doubles): partials of p.Ap, of (|r|^2, r.z), of r.z
 // (callback preconditioner), then the solve state
 constexpr int kOffA = 0, kOffB = kPcgBlocks, kOffC = 3 * kPcgBlocks, kOffState = 4 * kPcgBlocks;
@@ -58,27 +59,6 @@ constexpr int kWsDoubles = kOffState + 8;
 // slot l & 1, the status word, the iteration it was set at, the last |r|^2
 enum { kRz0 = 0, kRz1 = 1, kStatus = 2, kIter = 3, kRr = 4, kDelta = 5 };
 enum { kRunning = 0, kConverged = 1, kNonFinite = 2 };
-
-// one cell of J v with the periodic wrap or zero Dirichlet ghosts
-__device__ inline double jv_cell(const double* __restrict__ v, const double* __restrict__ fx,
-                                 const double* __restrict__ fy, int64_t nx, int64_t ny, int64_t i,
-                                 int64_t j, bool per)
-{
-  const int64_t k = j * nx + i;
-  double vw, ve, vs, vn;
-  if (per) {
-    vw = v[j * nx + (i > 0 ? i - 1 : nx - 1)];
-    ve = v[j * nx + (i < nx - 1 ? i + 1 : 0)];
-    vs = v[(j > 0 ? j - 1 : ny - 1) * nx + i];
-    vn = v[(j < ny - 1 ? j + 1 : 0) * nx + i];
-  } else {
-    vw = i > 0 ? v[k - 1] : 0.0;
-    ve = i < nx - 1 ? v[k + 1] : 0.0;
-    vs = j > 0 ? v[k - nx] : 0.0;
-    vn = j < ny - 1 ? v[k + nx] : 0.0;
-  }
-  return cell(fx[i], fx[i + 1], fy[j], fy[j + 1], v[k], vw, ve, vs, vn);
-}
 
 // workgroup sum of NACC accumulators into partials[blockIdx.x * NACC + a]
 template <int NACC>
@@ -105,7 +85,7 @@ __device__ inline void block_sums(double (&acc)[NACC], double* __restrict__ part
 // the sum of accumulator a over nblocks per-workgroup partials, returned to
 // every thread of the workgroup: thread t adds partials t, t + 256, ... in
 // order, then the wavefront butterfly, then the four wavefront sums in order
-// (the combine order of k_pcg_final, so the value is bitwise the same)
+// (every workgroup gets bitwise the same value)
 __device__ inline double sum_partials(const double* __restrict__ partials, int nblocks, int nacc, int a)
 {
   __shared__ double s[kBlock / kWave];
@@ -151,24 +131,7 @@ struct CellWalk {
 
 __device__ inline bool solve_ended(const double* __restrict__ st) { return st[kStatus] != kRunning; }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_ap(int64_t nx, int64_t ny, const double* __restrict__ fx,
-                                                   const double* __restrict__ fy, int per, double gamma,
-                                                   const double* __restrict__ p, double* __restrict__ ap,
-                                                   double* __restrict__ partials, const double* __restrict__ st)
-{
-  if (st && solve_ended(st)) return;
-  double acc[1] = {0.0};
-  for (CellWalk c(nx * ny, nx); c.valid(); c.next()) {
-    const double jp = jv_cell(p, fx, fy, nx, ny, c.i, c.j, per != 0);
-    const double pk = p[c.k];
-    const double a = pk + (-gamma) * jp;  // N_VLinearSum(1, p, -gamma, Jp) order
-    ap[c.k] = a;
-    acc[0] += pk * a;
-  }
-  block_sums<1>(acc, partials);
-}
-
-// alpha = rz / p.Ap with p.Ap from k_pcg_ap's partials (parts_a), then
+// alpha = rz / p.Ap with p.Ap from the stencil kernel's partials (parts_a), then
 // x += alpha p, r -= alpha Ap and
 // mode 0: no preconditioner (z = r), 1: Jacobi (z = dinv r), 2: z computed
 // elsewhere (only x and r are updated; acc[1] = 0).  wl2: the residual norm
@@ -207,11 +170,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, int nparts, co
 // r.z): converged, non-finite (|r| or beta), else running with
 // beta = r.z(l-1) / r.z(l-2).  Workgroup 0 records the outcome; the caller
 // returns when the solve has ended.
-__device__ inline bool test_iteration(int nparts, const double* __restrict__ parts_b, double* __restrict__ st,
-                                      int l, double* beta)
+// r.z comes from parts_b's second accumulator, or, when parts_z is given
+// (a preconditioner callback computed z after the update), from k_pcg_dot's
+// partials parts_z.
+__device__ inline bool test_iteration(int nparts, const double* __restrict__ parts_b,
+                                      const double* __restrict__ parts_z, double* __restrict__ st, int l,
+                                      double* beta)
 {
   const double rr = sum_partials(parts_b, nparts, 2, 0);
-  const double rznew = sum_partials(parts_b, nparts, 2, 1);
+  const double rznew = parts_z ? sum_partials(parts_z, nparts, 1, 0) : sum_partials(parts_b, nparts, 2, 1);
   const double rnorm = sqrt(rr);
   *beta = rznew / st[(l - 1) & 1];
   int status = kRunning;
@@ -233,11 +200,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirap(int64_t nx, int64_t ny, co
                                                       const double* __restrict__ pold, double* __restrict__ pnew,
                                                       double* __restrict__ ap, int nparts,
                                                       const double* __restrict__ parts_b,
+                                                      const double* __restrict__ parts_z,
                                                       double* __restrict__ parts_a, double* __restrict__ st, int l)
 {
   if (solve_ended(st)) return;
   double beta = 0.0;
-  if (l > 1 && test_iteration(nparts, parts_b, st, l, &beta)) return;
+  if (l > 1 && test_iteration(nparts, parts_b, parts_z, st, l, &beta)) return;
   // p_l at a cell: the first direction is z itself, already in pold
   auto pv = [&](int64_t k) { return (l > 1) ? z[k] + beta * pold[k] : pold[k]; };
   double acc[1] = {0.0};
@@ -322,12 +290,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirap_pairs(int64_t nx, int64_t 
                                                             const double* __restrict__ pold,
                                                             double* __restrict__ pnew, double* __restrict__ ap,
                                                             int nparts, const double* __restrict__ parts_b,
+                                                            const double* __restrict__ parts_z,
                                                             double* __restrict__ parts_a, double* __restrict__ st,
                                                             int l)
 {
   if (solve_ended(st)) return;
   double beta = 0.0;
-  if (l > 1 && test_iteration(nparts, parts_b, st, l, &beta)) return;
+  if (l > 1 && test_iteration(nparts, parts_b, parts_z, st, l, &beta)) return;
   const bool first = (l == 1);
   auto pv1 = [&](int64_t k) { return first ? pold[k] : z[k] + beta * pold[k]; };
   auto pv2 = [&](int64_t k) {
@@ -406,20 +375,22 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t nx, int64_t
     }
     acc[0] += rw0 * rw0;
     acc[0] += rw1 * rw1;
-    double z0 = r0, z1 = r1;
-    if (JAC) {
-      const int64_t i = 2 * i2;
-      const double syj = J.ay[j] + J.ay[j + 1];
-      z0 = jac_dinv(J, J.ax[i] + J.ax[i + 1], syj) * r0;
-      z1 = jac_dinv(J, J.ax[i + 1] + J.ax[i + 2], syj) * r1;
-    } else if (mode == 1) {
-      const double2 dv = ld2(dinv, k);
-      z0 = dv.x * r0;
-      z1 = dv.y * r1;
+    if (mode != 2) {  // 2: z is computed elsewhere
+      double z0 = r0, z1 = r1;
+      if (JAC) {
+        const int64_t i = 2 * i2;
+        const double syj = J.ay[j] + J.ay[j + 1];
+        z0 = jac_dinv(J, J.ax[i] + J.ax[i + 1], syj) * r0;
+        z1 = jac_dinv(J, J.ax[i + 1] + J.ax[i + 2], syj) * r1;
+      } else if (mode == 1) {
+        const double2 dv = ld2(dinv, k);
+        z0 = dv.x * r0;
+        z1 = dv.y * r1;
+      }
+      st2(z, k, make_double2(z0, z1));
+      acc[1] += r0 * z0;
+      acc[1] += r1 * z1;
     }
-    st2(z, k, make_double2(z0, z1));
-    acc[1] += r0 * z0;
-    acc[1] += r1 * z1;
     i2 += di;
     j += dj;
     if (i2 >= hnx) {
@@ -432,27 +403,43 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_pairs(int64_t nx, int64_t
 
 // the residual test of the last iteration of a batch, so that the host sees it
 __global__ __launch_bounds__(kBlock) void k_pcg_check(int nparts, const double* __restrict__ parts_b,
-                                                      double* __restrict__ st, int l)
+                                                      const double* __restrict__ parts_z, double* __restrict__ st,
+                                                      int l)
 {
   if (solve_ended(st)) return;
   double beta;
-  test_iteration(nparts, parts_b, st, l, &beta);
+  test_iteration(nparts, parts_b, parts_z, st, l, &beta);
+}
+
+// the residual test alone of iteration l (callback preconditioner: before
+// z = M r is computed for it): converged or non-finite, recorded as
+// test_iteration does; otherwise the state is left for the next iteration's
+// full test (beta needs r.z)
+__global__ __launch_bounds__(kBlock) void k_pcg_rtest(int nparts, const double* __restrict__ parts_b,
+                                                      double* __restrict__ st, int l)
+{
+  if (solve_ended(st)) return;
+  const double rr = sum_partials(parts_b, nparts, 2, 0);
+  const double rnorm = sqrt(rr);
+  int status = kRunning;
+  if (!isfinite(rnorm)) status = kNonFinite;
+  else if (rnorm <= st[kDelta]) status = kConverged;
+  if (threadIdx.x == 0 && status != kRunning) {
+    st[kRr] = rr;
+    st[kIter] = l;
+    st[kStatus] = status;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_dot(int64_t n, const double* __restrict__ a,
-                                                    const double* __restrict__ b, double* __restrict__ partials)
+                                                    const double* __restrict__ b, double* __restrict__ partials,
+                                                    const double* __restrict__ st)
 {
+  if (st && solve_ended(st)) return;
   double acc[1] = {0.0};
   for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
     acc[0] += a[k] * b[k];
   block_sums<1>(acc, partials);
-}
-
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t n, double beta, const double* __restrict__ z,
-                                                    double* __restrict__ p)
-{
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock)
-    p[k] = z[k] + beta * p[k];
 }
 
 __global__ void k_pcg_init(double* __restrict__ st, double rz, double delta)
@@ -464,16 +451,6 @@ __global__ void k_pcg_init(double* __restrict__ st, double rz, double delta)
     st[kIter] = 0;
     st[kRr] = 0.0;
     st[kDelta] = delta;
-  }
-}
-
-// sum of the per-workgroup partials (nacc accumulators), in a fixed order
-__global__ __launch_bounds__(kBlock) void k_pcg_final(int nblocks, int nacc, const double* __restrict__ partials,
-                                                      double* __restrict__ out)
-{
-  for (int a = 0; a < nacc; a++) {
-    const double t = sum_partials(partials, nblocks, nacc, a);
-    if (threadIdx.x == 0) out[a] = t;
   }
 }
 
@@ -492,16 +469,6 @@ void ensure_ws(SUNLinearSolver S)
   CEDA_HIP_CHECK(dev_malloc(&S->d_ws, sizeof(double) * kWsDoubles));
   CEDA_HIP_CHECK(host_malloc(&S->h_ws, sizeof(double) * kWsDoubles, hipHostMallocDefault));
   CEDA_HIP_CHECK(dev_malloc(&S->d_p2, sizeof(double) * vlen(S->r)));
-}
-
-// the nacc sums of partials on the host
-const double* finish(hipStream_t s, SUNLinearSolver S, const double* partials, unsigned g, int nacc)
-{
-  double* out = S->d_ws + kOffState;
-  hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, s, (int)g, nacc, partials, out);
-  CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, out, sizeof(double) * nacc, hipMemcpyDeviceToHost, s));
-  CEDA_HIP_CHECK(hipStreamSynchronize(s));
-  return S->h_ws;
 }
 
 }  // namespace
@@ -554,93 +521,132 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
   double rz = N_VDotProd(S->r, S->z);
   if (krylov_nonfinite("fused PCG", "r.z", rz)) return -1;
 
-  if (pmode != 2) {
-    // scalars on the device; the host reads the state once per batch.  The
-    // direction of iteration l lives in q[l & 1] (q[1] = S->p holds p_1 = z)
-    double* q[2] = {S->d_p2, dp};
-    // pairs of cells with 16-byte accesses: an even row length (every pair in
-    // one row, every pair's first cell 16-byte aligned)
-    const bool pairs = (nx % 2 == 0);
-    // the Jacobi inverse diagonal formed in the update kernel (from the gamma
-    // of its last setup, which is what P->diag holds)
-    const bool jac = (pmode == 1 && P->diag_gamma_set);
-    const JacobiDiag J{per ? P->d_px : P->d_fx, per ? P->d_py : P->d_fy, P->diag_gamma};
-    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, st, rz, delta);
-    int l = 1, batch = 4;
-    while (l <= S->maxl) {
-      const int last = std::min(S->maxl, l + batch - 1);
-      for (int it = l; it <= last; it++) {
-        // iteration 1 reads p_1 = z from q[1] and writes nothing
-        const double* pold = it == 1 ? q[1] : q[(it - 1) & 1];
-        if (pairs) {
-          if (per)
-            hipLaunchKernelGGL(k_pcg_dirap_pairs<true>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
-                               gamma, dz, pold, q[it & 1], dap, (int)g, pb, pa, st, it);
-          else
-            hipLaunchKernelGGL(k_pcg_dirap_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
-                               gamma, dz, pold, q[it & 1], dap, (int)g, pb, pa, st, it);
-          if (jac)
-            hipLaunchKernelGGL(k_pcg_update_pairs<true>, dim3(g), dim3(kBlock), 0, s, nx, ny, (int)g, pa, st, it,
-                               q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
-          else
-            hipLaunchKernelGGL(k_pcg_update_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, (int)g, pa, st, it,
-                               q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
-        } else {
-          hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
-                             pold, q[it & 1], dap, (int)g, pb, pa, st, it);
-          hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
-                             dap, dx, dr, dz, dinv, w, pmode, pb);
+  // Scalars on the device; the host reads the state once per batch.  The
+  // direction of iteration l lives in q[l & 1] (q[1] = S->p holds p_1 = z).
+  // With a preconditioner callback (multigrid, the learned network) an
+  // iteration is the stencil/direction kernel, the update, the callback's
+  // z = M r (enqueued on the same stream) and r.z (k_pcg_dot).  Every kernel
+  // returns at once after convergence, but the callback's own work would
+  // still be spent, so the host enqueues the first 3/4 of the previous
+  // solve's iteration count without waiting and then tests each residual
+  // (k_pcg_rtest, one wait) before it calls the preconditioner.
+  double* q[2] = {S->d_p2, dp};
+  // pairs of cells with 16-byte accesses: an even row length (every pair in
+  // one row, every pair's first cell 16-byte aligned)
+  const bool pairs = (nx % 2 == 0);
+  // the Jacobi inverse diagonal formed in the update kernel (from the gamma
+  // of its last setup, which is what P->diag holds)
+  const bool jac = (pmode == 1 && P->diag_gamma_set);
+  const JacobiDiag J{per ? P->d_px : P->d_fx, per ? P->d_py : P->d_fy, P->diag_gamma};
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, st, rz, delta);
+  if (pmode == 2) {
+    const long bulk = (S->last_iters > 4) ? S->last_iters * 3 / 4 : 0;
+    for (int it = 1; it <= S->maxl; it++) {
+      const double* pold = it == 1 ? q[1] : q[(it - 1) & 1];
+      if (pairs) {
+        if (per)
+          hipLaunchKernelGGL(k_pcg_dirap_pairs<true>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
+                             gamma, dz, pold, q[it & 1], dap, (int)g, pb, pc, pa, st, it);
+        else
+          hipLaunchKernelGGL(k_pcg_dirap_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
+                             gamma, dz, pold, q[it & 1], dap, (int)g, pb, pc, pa, st, it);
+        hipLaunchKernelGGL(k_pcg_update_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, (int)g, pa, st, it,
+                           q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
+      } else {
+        hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
+                           pold, q[it & 1], dap, (int)g, pb, pc, pa, st, it);
+        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
+                           dap, dx, dr, dz, dinv, w, pmode, pb);
+      }
+      // after the bulk, and every kPcgChunk iterations within it, the host
+      // waits for the residual test (chunks keep streams that share a
+      // hardware queue interleaved)
+      if (it > bulk || it % kPcgChunk == 0) {
+        hipLaunchKernelGGL(k_pcg_rtest, dim3(1), dim3(kBlock), 0, s, (int)g, pb, st, it);
+        CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+        CEDA_HIP_CHECK(hipStreamSynchronize(s));
+        const int status = (int)S->h_ws[kStatus];
+        if (status != kRunning) {
+          const long iters = (long)S->h_ws[kIter];
+          A->nli += iters;
+          A->njtimes += iters;
+          A->npsolve += iters - 1;  // z of the converged iterate is not computed
+          S->last_iters = iters;
+          if (status == kNonFinite) {
+            fprintf(stderr, "[ARKLS ERROR] fused PCG: non-finite residual or direction at iteration %ld "
+                            "(|r|^2 = %g); linear solve failed\n", iters, S->h_ws[kRr]);
+            return -1;
+          }
+          return 0;
         }
       }
-      hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, (int)g, pb, st, last + 1);
-      CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
-      CEDA_HIP_CHECK(hipStreamSynchronize(s));
-      const int status = (int)S->h_ws[kStatus];
-      const long iters = (long)S->h_ws[kIter];  // iterations run in this solve
-      if (status != kRunning) {
-        A->nli += iters;
-        A->njtimes += iters;
-        if (pmode == 1) A->npsolve += iters - 1;  // z of the converged iterate is not used
-        if (status == kNonFinite) {
-          fprintf(stderr, "[ARKLS ERROR] fused PCG: non-finite residual or direction at iteration %ld "
-                          "(|r|^2 = %g); linear solve failed\n", iters, S->h_ws[kRr]);
-          return -1;
-        }
-        return 0;
+      if (ps(t, A->zcur, A->fz, S->r, S->z, gamma, delta, 1, ud)) {
+        fprintf(stderr, "[ARKLS ERROR] fused PCG: the preconditioner solve failed at iteration %d\n", it);
+        return -1;
       }
-      l = last + 1;
-      batch = std::min(2 * batch, kPcgBatchMax);
+      hipLaunchKernelGGL(k_pcg_dot, dim3(g), dim3(kBlock), 0, s, n, dr, dz, pc, st);
     }
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, (int)g, pb, pc, st, S->maxl + 1);
+    CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    CEDA_HIP_CHECK(hipStreamSynchronize(s));
     A->nli += S->maxl;
     A->njtimes += S->maxl;
-    if (pmode == 1) A->npsolve += S->maxl;
+    A->npsolve += S->maxl;
+    S->last_iters = S->maxl;
     return (sqrt(S->h_ws[kRr]) < bnorm) ? 1 : 2;
   }
 
-  // callback preconditioner: alpha on the device, the residual test and
-  // beta on the host
-  double rnorm = bnorm;
-  for (int l = 1; l <= S->maxl; l++) {
-    hipLaunchKernelGGL(k_pcg_ap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dp, dap,
-                       pa, nullptr);
-    A->njtimes++;
-    hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, rz, nullptr, l, dp, dap, dx,
-                       dr, dz, dinv, w, pmode, pb);
-    const double* sums = finish(s, S, pb, g, 2);
-    A->nli++;
-    rnorm = sqrt(sums[0]);
-    if (krylov_nonfinite("fused PCG", "|r|", rnorm)) return -1;
-    if (rnorm <= delta) return 0;
-    A->npsolve++;
-    if (ps(t, A->zcur, A->fz, S->r, S->z, gamma, delta, 1, ud)) return -1;
-    hipLaunchKernelGGL(k_pcg_dot, dim3(g), dim3(kBlock), 0, s, n, dr, dz, pc);
-    const double rznew = finish(s, S, pc, g, 1)[0];
-    const double beta = rznew / rz;
-    if (krylov_nonfinite("fused PCG", "beta", beta)) return -1;
-    rz = rznew;
-    hipLaunchKernelGGL(k_pcg_dir, dim3(g), dim3(kBlock), 0, s, n, beta, dz, dp);
+  int l = 1, batch = 4;
+  while (l <= S->maxl) {
+    const int last = std::min(S->maxl, l + batch - 1);
+    for (int it = l; it <= last; it++) {
+      // iteration 1 reads p_1 = z from q[1] and writes nothing
+      const double* pold = it == 1 ? q[1] : q[(it - 1) & 1];
+      if (pairs) {
+        if (per)
+          hipLaunchKernelGGL(k_pcg_dirap_pairs<true>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
+                             gamma, dz, pold, q[it & 1], dap, (int)g, pb, nullptr, pa, st, it);
+        else
+          hipLaunchKernelGGL(k_pcg_dirap_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy,
+                             gamma, dz, pold, q[it & 1], dap, (int)g, pb, nullptr, pa, st, it);
+        if (jac)
+          hipLaunchKernelGGL(k_pcg_update_pairs<true>, dim3(g), dim3(kBlock), 0, s, nx, ny, (int)g, pa, st, it,
+                             q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
+        else
+          hipLaunchKernelGGL(k_pcg_update_pairs<false>, dim3(g), dim3(kBlock), 0, s, nx, ny, (int)g, pa, st, it,
+                             q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
+      } else {
+        hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
+                           pold, q[it & 1], dap, (int)g, pb, nullptr, pa, st, it);
+        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
+                           dap, dx, dr, dz, dinv, w, pmode, pb);
+      }
+    }
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, (int)g, pb, nullptr, st, last + 1);
+    CEDA_HIP_CHECK(hipMemcpyAsync(S->h_ws, st, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+    CEDA_HIP_CHECK(hipStreamSynchronize(s));
+    const int status = (int)S->h_ws[kStatus];
+    const long iters = (long)S->h_ws[kIter];  // iterations run in this solve
+    if (status != kRunning) {
+      A->nli += iters;
+      A->njtimes += iters;
+      if (pmode == 1) A->npsolve += iters - 1;  // z of the converged iterate is not used
+      S->last_iters = iters;
+      if (status == kNonFinite) {
+        fprintf(stderr, "[ARKLS ERROR] fused PCG: non-finite residual or direction at iteration %ld "
+                        "(|r|^2 = %g); linear solve failed\n", iters, S->h_ws[kRr]);
+        return -1;
+      }
+      return 0;
+    }
+    l = last + 1;
+    batch = std::min(2 * batch, kPcgBatchMax);
   }
-  return (rnorm < bnorm) ? 1 : 2;
+  A->nli += S->maxl;
+  A->njtimes += S->maxl;
+  if (pmode == 1) A->npsolve += S->maxl;
+  S->last_iters = S->maxl;
+  return (sqrt(S->h_ws[kRr]) < bnorm) ? 1 : 2;
 }
 
 }  // namespace ceda_ark
