@@ -849,13 +849,16 @@ def child_json(extra, keep, limit=300):
 def learned_vs_plain():
     """The implicit configuration's time-to-solution comparison (mms-dirichlet
     2048^2, DIRK3, dt = 1e-3, 50 timed steps): fused Jacobi-PCG against FGMRES
-    with the symmetric learned network, trained on the GPU first; the time to
-    solution is reported per step and with the training included."""
+    and against PCG with the symmetric learned network (one network, trained
+    on the GPU first); the time to solution is reported per step and with the
+    training included."""
     res = child_json(["--workload", "implicit", "--isteps", "50", "--jtimes", "--fused-pcg", "--imethods",
-                      "jacobi,learned-sym", "--no-untrained"],
+                      "jacobi,learned-sym,learned-sym-pcg", "--no-untrained"],
                      ("config", "steps_timed", "gamma_dt_over_dx2", "jacobi_pcg", "learned_symmetric_fgmres",
-                      "speedup_learned_symmetric_fgmres_vs_jacobi_pcg",
-                      "rel_l2_diff_learned_symmetric_fgmres_vs_jacobi_pcg", "time_to_solution", "training",
+                      "learned_symmetric_pcg", "speedup_learned_symmetric_fgmres_vs_jacobi_pcg",
+                      "speedup_learned_symmetric_pcg_vs_jacobi_pcg",
+                      "rel_l2_diff_learned_symmetric_fgmres_vs_jacobi_pcg",
+                      "rel_l2_diff_learned_symmetric_pcg_vs_jacobi_pcg", "time_to_solution", "training",
                       "fgmres_restart_symmetric"))
     log(f"learned vs plain: speedup {res.get('speedup_learned_symmetric_fgmres_vs_jacobi_pcg')}")
     return res
