@@ -29,8 +29,10 @@ def ctx(pk):
 
 def run_pair(pk, ctx, kind, method, tout, prob_kw, rtol=1e-5, atol=1e-10, controller=0,
              fixed=0.0, internal=False, deefreq=25, fast=0, maxsteps=100000, dirk=None,
-             stagemax=200, jtimes=False):
+             stagemax=200, jtimes=False, pass_cfg=None):
     G = pk.Problem(ctx, kind, **prob_kw)
+    if pass_cfg:
+        assert G.set_config(2, *pass_cfg) == 0
     P = O.OracleProblem(kind, **prob_kw)
     y0 = G.initial()
     I = pk.Integrator(G, method, y0, ls=(dirk or {}).get("ls", "cg"),
@@ -164,6 +166,48 @@ def test_full_size_config2_steps(pk, ctx):
     assert o["fg"] == -1 and o["fr"] == -1  # stopped by the step cap, as requested
     check(o, fields=("nst", "attempts", "nfe", "ndomeig", "ndee_iters", "stage_max"))
     assert o["sg"]["nfe"] > 30  # initial-step estimate, power iteration and stages
+
+
+@pytest.mark.timeout(600)
+def test_config2_reduced_100_steps(pk, ctx):
+    """The benchmark configuration's method at 512^2 (MMS, kx=1, ky=0.5,
+    RKL2, power-iteration spectral radius every 25 steps, rtol 1e-6) for 100
+    steps (four estimates of the spectral radius, stage counts up to ~100) on
+    the benchmark's own path: fused kernels, HIP graphs and 5-stage passes of
+    48 rows.  The GPU takes the oracle's steps (same counts, state to 1e-10),
+    or, if a last bit of a reduction flips a step decision, stays within the
+    oracle's own spread under one- and two-ulp changes of rtol."""
+    kw = dict(nx=512, ny=512, kx=1.0, ky=0.5)
+    fields = ("nst", "attempts", "nfe", "netf", "nstlim", "ndomeig", "stage_max")
+    o = run_pair(pk, ctx, 1, "rkl", 1.0, kw, rtol=1e-6, internal=True, fast=2, maxsteps=100,
+                 pass_cfg=(5, 48, 0))
+    assert o["fg"] == o["fr"] == -1  # stopped by the step cap, as requested
+    assert o["sr"]["ndomeig"] >= 4 and o["sr"]["stage_max"] > 50
+    if all(o["sg"][f] == o["sr"][f] for f in fields):
+        check(o, fields=fields)
+        return
+    env = []
+    for u in (1, -1, 2, -2):
+        r = 1e-6
+        for _ in range(abs(u)):
+            r = float(np.nextafter(r, np.inf if u > 0 else -np.inf))
+        P = O.OracleProblem(1, **kw)
+        R = O.OracleIntegrator(P, O.RKL, P.initial())
+        R.set_tolerances(r, 1e-10)
+        R.set_max_num_steps(100)
+        R.set_internal_eig(1, 100, 0.01)
+        R.set_dom_eig_frequency(25)
+        R.set_max_num_stages(200)
+        R.set_controller(0)
+        fr, tr, yr = R.evolve(1.0)
+        env.append((R.stats(), tr, yr))
+    for f in ("nfe",):
+        lo = min([e[0][f] for e in env] + [o["sr"][f]])
+        hi = max([e[0][f] for e in env] + [o["sr"][f]])
+        assert 0.9 * lo <= o["sg"][f] <= 1.1 * hi, (f, o["sg"][f], lo, hi)
+    tlo = min([e[1] for e in env] + [o["tr"]])
+    thi = max([e[1] for e in env] + [o["tr"]])
+    assert 0.9 * tlo <= o["tg"] <= 1.1 * thi, (o["tg"], tlo, thi)
 
 
 @pytest.mark.parametrize("order", [2, 3])
