@@ -148,3 +148,36 @@ def test_stages_per_pass_agree_bitwise(pk, ctx):
     B = pk.Batch(ctx, plist, "rkc")
     assert B.SetStagesPerPass(6) != 0 and B.SetStagesPerPass(0) != 0
     B.release()
+
+
+@pytest.mark.parametrize("method", ["rkc", "rkl"])
+def test_mixed_batch_equals_single_problems(pk, ctx, method):
+    """40 mixed problems (both kinds, odd and even sides: the per-tile layout
+    dispatch) end bitwise where the single-problem fused path ends, with the
+    same counts, after two evolve calls."""
+    plist = problems(21, 20, [16, 24, 33, 48]) + problems(22, 20, [16, 31, 40], kind=0)
+    B = pk.Batch(ctx, plist, method)
+    B.SetTolerances(1e-5, 1e-10)
+    B.SetMaxNumSteps(100000)
+    assert B.evolve(0.03) == 0
+    assert B.evolve(0.06) == 0
+    g, launches = B.groups()
+    assert g > 0 and launches > g
+    for k, prm in enumerate(plist):
+        kw = {kk: v for kk, v in prm.items() if kk != "kind"}
+        G = pk.Problem(ctx, prm["kind"], **kw)
+        I = pk.Integrator(G, method, G.initial())
+        I.ARKodeSStolerances(1e-5, 1e-10)
+        I.ARKodeSetMaxNumSteps(100000)
+        assert I.LSRKStepSetDiffusionFastPath(G.h, 1) == 0
+        for tout in (0.03, 0.06):
+            flag, t = I.evolve(tout)
+            assert flag == 0
+        assert B.time(k) == t
+        si, sb = I.stats(), B.stats(k)
+        for f in ("nst", "attempts", "nfe", "netf", "nstlim", "stage_max"):
+            assert si[f] == sb[f], (k, f, si[f], sb[f])
+        assert np.array_equal(I.y.get(), B.solution(k)), k
+        I.free()
+        G.destroy()
+    B.release()
