@@ -608,7 +608,7 @@ int DiffusionProblem_SetStencilRows(DiffusionProblem* P, int rows)
 
 int DiffusionProblem_SetStencilConfig(DiffusionProblem* P, int which, int variant, int rows, int nt)
 {
-  if (!P || which < 0 || which > 2 || rows < 1 || rows > 1024 || (nt != 0 && nt != 1)) return -1;
+  if (!P || which < 0 || which > 2 || rows < 1 || rows > 1024 || nt < 0 || nt > (which == 2 ? 3 : 1)) return -1;
   if (which < 2 && (variant < 0 || variant > 1 || rows > 256 || (variant == 0 && nt))) return -1;
   if (which == 2 && (variant < 0 || variant > kMaxPassStages)) return -1;
   P->cfg[which].variant = variant;
@@ -726,7 +726,7 @@ int DiffusionProblem_Autotune(DiffusionProblem* P, int reps)
   for (int K : {2, 3, 4, 5, 6}) {
     StageEntry* dtab = pass_table(buf, K, s);
     for (int rows : {8, 16, 24, 32, 40, 48, 64, 96, 128})
-      for (int nt : {0, 1}) {
+      for (int nt : {0, 1, 2, 3}) {
         if (rows > 2 * P->p.ny) continue;
         launch_pass(P, K, rows, nt, dtab, s);
         CEDA_HIP_CHECK(hipEventRecord(e0, s));

@@ -115,9 +115,15 @@ __device__ inline void gstore(gdouble* p, const double* v, int cpl)
 // re-read into a scalar register every row step through an empty asm, so the
 // compiler cannot hoist the K x CPL products T1*sx out of the march: they
 // would hold 4K more VGPRs, which at K = 5 costs a wavefront per SIMD.
-template <int CPL, bool PER, bool MMS, int K, int L, bool FUSED, bool NT, bool SCALAR_E = false>
+// CLDS: the stage coefficients (5 per level, with T1 and T2) are read from
+// LDS (lc[8m + k]) at every row step instead of being held in scalar
+// registers for the whole march, where K = 5 spills 46 of them to vector
+// lanes (read back by v_readlane on the VALU at every use).
+typedef __attribute__((address_space(3))) const double lcdouble;
+
+template <int CPL, bool PER, bool MMS, int K, int L, bool FUSED, bool NT, bool SCALAR_E = false, bool CLDS = false>
 __device__ __forceinline__ void pass_wave(const PassGeo& g, const StageEntry* __restrict__ e,
-                                          int strip, int rb)
+                                          int strip, int rb, lcdouble* __restrict__ lc = nullptr)
 {
   static_assert(K >= 1 && K <= kMaxPassStages, "1..6 stages per pass");
   static_assert(L * CPL >= K - 1, "halo lanes must cover the shrinking levels");
@@ -148,13 +154,14 @@ __device__ __forceinline__ void pass_wave(const PassGeo& g, const StageEntry* __
       }
     }
   }
-  double cc[K][5], T1[K], T2[K];
+  constexpr int KC = CLDS ? 1 : K;  // coefficient registers (CLDS: unused)
+  double cc[KC][5], T1[KC], T2[KC];
 #pragma unroll
-  for (int m = 0; m < K; m++) {
+  for (int m = 0; m < KC; m++) {
 #pragma unroll
-    for (int k = 0; k < 5; k++) cc[m][k] = FUSED ? e[m].c[k] : 0.0;
-    T1[m] = e[m].T1;
-    T2[m] = e[m].T2;
+    for (int k = 0; k < 5; k++) cc[m][k] = (FUSED && !CLDS) ? e[m].c[k] : 0.0;
+    T1[m] = CLDS ? 0.0 : e[m].T1;
+    T2[m] = CLDS ? 0.0 : e[m].T2;
   }
   gcdouble* __restrict__ y0 = (gcdouble*)e[0].yjm1;
   gcdouble* __restrict__ ym = (gcdouble*)e[0].yjm2;
@@ -210,6 +217,7 @@ __device__ __forceinline__ void pass_wave(const PassGeo& g, const StageEntry* __
   }
 
   for (int64_t t = ts; t <= te; t++) {
+    if (CLDS) asm volatile("" ::: "memory");  // the coefficients are read again at every row step
     double A1[CPL];
 #pragma unroll
     for (int k = 0; k < CPL; k++) {
@@ -268,15 +276,19 @@ __device__ __forceinline__ void pass_wave(const PassGeo& g, const StageEntry* __
           f[CPL - 1] = cell(De0, De1, Ds, Dn, C[CPL - 1], C[0], ev, S[CPL - 1], N[CPL - 1]);
         } else f[0] = cell(Dw0, De0, Ds, Dn, C[0], wv, ev, S[0], N[0]);
         if (MMS) {
-          double t1 = T1[m - 1];
-          if (SCALAR_E) asm volatile("" : "+s"(t1));
-          f[0] = f[0] + forcing(t1, T2[m - 1], sx0, cx0, syj, cyj);
-          if (CPL == 2) f[CPL - 1] = f[CPL - 1] + forcing(t1, T2[m - 1], sx1, cx1, syj, cyj);
+          double t1 = CLDS ? lc[8 * (m - 1) + 5] : T1[CLDS ? 0 : m - 1];
+          const double t2 = CLDS ? lc[8 * (m - 1) + 6] : T2[CLDS ? 0 : m - 1];
+          if (SCALAR_E && !CLDS) asm volatile("" : "+s"(t1));
+          f[0] = f[0] + forcing(t1, t2, sx0, cx0, syj, cyj);
+          if (CPL == 2) f[CPL - 1] = f[CPL - 1] + forcing(t1, t2, sx1, cx1, syj, cyj);
         }
         if (FUSED) {
           const double* Am = (m == 1) ? A1 : W[m >= 2 ? m - 2 : 0][0];
+          double cm[5];
 #pragma unroll
-          for (int k = 0; k < CPL; k++) f[k] = lc5(cc[m - 1], f[k], Am[k], YN[m - 1][k], C[k], FN[m - 1][k]);
+          for (int k = 0; k < 5; k++) cm[k] = CLDS ? lc[8 * (m - 1) + k] : cc[CLDS ? 0 : m - 1][k];
+#pragma unroll
+          for (int k = 0; k < CPL; k++) f[k] = lc5(cm, f[k], Am[k], YN[m - 1][k], C[k], FN[m - 1][k]);
         }
       }
       if (own && rho >= j0 && rho < j1 && (m == K || m == K - 1))
@@ -321,10 +333,19 @@ __device__ __forceinline__ void axpy_wave(const PassGeo& g, const StageEntry* __
 }
 
 // ---------------------------------------------------------- one problem
-template <int CPL, bool PER, bool MMS, int K, bool NT>
+template <int CPL, bool PER, bool MMS, int K, bool NT, bool CLDS>
 __global__ __launch_bounds__(kBlock) void k_pass(const PassGeo g, const StageEntry* __restrict__ e)
 {
   constexpr int L = (K - 1 + CPL - 1) / CPL;
+  __shared__ double scoef[CLDS ? 8 * K : 1];
+  if (CLDS) {  // every wavefront of the workgroup reads the same table
+    const int i = threadIdx.x;
+    if (i < 8 * K) {
+      const int m = i >> 3, k = i & 7;
+      scoef[i] = k < 5 ? e[m].c[k] : (k == 5 ? e[m].T1 : (k == 6 ? e[m].T2 : 0.0));
+    }
+    __syncthreads();
+  }
   // XCD-aware: the 8 XCDs receive workgroups round-robin, so workgroup b is
   // given tile (b mod 8)*ceil(T/8) + (b div 8) and each XCD sweeps a
   // contiguous band of row blocks (their halo rows stay in its L2)
@@ -336,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_pass(const PassGeo g, const StageEnt
   const int64_t nrb = (g.ny + g.rows - 1) / g.rows;
   if (wt >= (int64_t)g.strips * nrb) return;
   const int strip = (int)(wt % g.strips), rb = (int)(wt / g.strips);
-  pass_wave<CPL, PER, MMS, K, L, true, NT, true>(g, e, strip, rb);
+  pass_wave<CPL, PER, MMS, K, L, true, NT, true, CLDS>(g, e, strip, rb, (lcdouble*)scoef);
 }
 
 PassGeo geo_of(const DiffusionProblem* P, int cpl, int L, int rows)
@@ -356,14 +377,19 @@ PassGeo geo_of(const DiffusionProblem* P, int cpl, int L, int rows)
   return g;
 }
 
+// variant: bit 0 non-temporal operand streams, bit 1 stage coefficients in LDS
 template <int CPL, bool PER, bool MMS>
-void launch_pass_t(const PassGeo& g, int K, bool nt, const StageEntry* e, hipStream_t s)
+void launch_pass_t(const PassGeo& g, int K, int variant, const StageEntry* e, hipStream_t s)
 {
   const int64_t waves = (int64_t)g.strips * ((g.ny + g.rows - 1) / g.rows);
   const unsigned grid = (unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock);
-#define CEDA_PASS(KK)                                                                              \
-  if (nt) hipLaunchKernelGGL((k_pass<CPL, PER, MMS, KK, true>), dim3(grid), dim3(kBlock), 0, s, g, e); \
-  else hipLaunchKernelGGL((k_pass<CPL, PER, MMS, KK, false>), dim3(grid), dim3(kBlock), 0, s, g, e);
+#define CEDA_PASS(KK)                                                                                     \
+  switch (variant & 3) {                                                                                  \
+    case 0: hipLaunchKernelGGL((k_pass<CPL, PER, MMS, KK, false, false>), dim3(grid), dim3(kBlock), 0, s, g, e); break; \
+    case 1: hipLaunchKernelGGL((k_pass<CPL, PER, MMS, KK, true, false>), dim3(grid), dim3(kBlock), 0, s, g, e); break;  \
+    case 2: hipLaunchKernelGGL((k_pass<CPL, PER, MMS, KK, false, true>), dim3(grid), dim3(kBlock), 0, s, g, e); break;  \
+    default: hipLaunchKernelGGL((k_pass<CPL, PER, MMS, KK, true, true>), dim3(grid), dim3(kBlock), 0, s, g, e); break; \
+  }
   switch (K) {
     case 1: CEDA_PASS(1) break;
     case 2: CEDA_PASS(2) break;
@@ -492,11 +518,11 @@ void launch_pass(DiffusionProblem* P, int K, int rows, int nt, const StageEntry*
   const PassGeo g = geo_of(P, cpl, pass_halo_lanes(K, cpl), rows);
   const bool per = (P->p.kind == DIFFUSION_CEDA_PERIODIC);
   if (cpl == 2) {
-    if (per) launch_pass_t<2, true, false>(g, K, nt != 0, e, s);
-    else launch_pass_t<2, false, true>(g, K, nt != 0, e, s);
+    if (per) launch_pass_t<2, true, false>(g, K, nt, e, s);
+    else launch_pass_t<2, false, true>(g, K, nt, e, s);
   } else {
-    if (per) launch_pass_t<1, true, false>(g, K, nt != 0, e, s);
-    else launch_pass_t<1, false, true>(g, K, nt != 0, e, s);
+    if (per) launch_pass_t<1, true, false>(g, K, nt, e, s);
+    else launch_pass_t<1, false, true>(g, K, nt, e, s);
   }
 }
 

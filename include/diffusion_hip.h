@@ -57,8 +57,10 @@ int DiffusionProblem_SetStencilRows(DiffusionProblem* prob, int rows);
    y-marching (rows = consecutive rows per wavefront, nt = non-temporal
    streamed operands).  which = 2 is the multi-stage pass of the fused path:
    variant = stages per pass K (2..6; 0 = one kernel per stage), rows = output
-   rows per wavefront tile (1..1024), nt as above.  Every shape computes
-   bitwise-identical results. */
+   rows per wavefront tile (1..1024), nt a bit mask: 1 non-temporal streamed
+   operands, 2 the stage coefficients read from LDS at every row step instead
+   of held in scalar registers.  Every shape computes bitwise-identical
+   results. */
 int DiffusionProblem_SetStencilConfig(DiffusionProblem* prob, int which, int variant, int rows,
                                       int nt);
 int DiffusionProblem_GetStencilConfig(const DiffusionProblem* prob, int which, int* variant,

@@ -73,7 +73,7 @@ def test_stage_pass_bitwise(pk, ctx, K, kind, nx, ny):
     ra, rb = reference_pass(P, kind, K, coef, times, y1, y2, yn, fn)
     V = [G.vector(v) for v in (y1, y2, yn, fn)]
     for rows in (1, 3, 8, 32, 200):
-        for nt in (0, 1):
+        for nt in (0, 1, 2, 3):  # bit 0 non-temporal, bit 1 stage coefficients in LDS
             assert G.set_config(2, K, rows, nt) == 0
             outa, outb = G.vector(), G.vector()
             assert G.stage_pass(K, coef, *V, outa, outb) == 0
@@ -116,7 +116,7 @@ def test_fused_path_with_passes_is_bitwise(pk, ctx, method, kind, nx, ny, intern
     base = fused_run(pk, ctx, kind, method, kw, (0, 32, 0), 1, 0.05, internal)
     assert base[0] == 0
     for cfg, graphs in [((2, 16, 0), 1), ((3, 8, 1), 1), ((4, 32, 0), 0), ((3, 200, 0), 1), ((5, 16, 1), 1),
-                        ((6, 24, 0), 1)]:
+                        ((6, 24, 0), 1), ((5, 48, 2), 1), ((4, 16, 3), 0)]:
         got = fused_run(pk, ctx, kind, method, kw, cfg, graphs, 0.05, internal)
         assert got[0] == 0 and got[1] == base[1]
         for f in ("nst", "attempts", "nfe", "netf", "stage_max"):
