@@ -96,8 +96,8 @@ def roofline(pk, ctx, G):
         assert rc == 0
         launch_ms = avg.value
         bytes_alg = 48.0 * G.n
-        kname = (f"k_pass<2,false,true,{K},{'true' if nt else 'false'}> rows={rows} "
-                 f"({K} fused STS stages per launch)")
+        kname = (f"k_pass<2,false,true,{K},{'true' if nt & 1 else 'false'},{'true' if nt & 2 else 'false'}> "
+                 f"rows={rows} ({K} fused STS stages per launch)")
         traffic = read_traffic("stage_pass_4096_pmc.txt", (K, rows, nt))
     else:
         launch_ms = stage_ms
