@@ -512,7 +512,7 @@ def c5_workload(pk, args):
             continue
         dil, sym = shapes[kind]
         ctx = pk.Context()
-        ratio = float(np.median(ratios))
+        ratio = args.c5_train_ratio if args.c5_train_ratio > 0 else float(np.median(ratios))
         tn = args.sym_train_n if sym else args.train_n
         train_steps = args.sym_train_steps if sym else args.train_steps
         lr = args.sym_train_lr if sym else args.train_lr
@@ -736,6 +736,8 @@ def parse_args(argv=None):
     ap.add_argument("--c5-steps", type=int, default=3, help="c5: DIRK3 steps per problem")
     ap.add_argument("--c5-workers", type=int, default=8, help="c5: concurrent host threads")
     ap.add_argument("--c5-methods", default="jacobi,mg")
+    ap.add_argument("--c5-train-ratio", type=float, default=0.0,
+                    help="c5: gamma*dt*k/dx^2 the shared network is trained at (0: the batch median)")
     ap.add_argument("--tout", type=float, default=0.5)
     ap.add_argument("--conv-kernel", type=int, default=1, choices=[0, 1, 2],
                     help="learned network's convolutions: 1 LDS row chains, 2 LDS row chains and two layers per "
