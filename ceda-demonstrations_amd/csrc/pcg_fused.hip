@@ -105,14 +105,22 @@ __device__ inline double sum_partials(const double* __restrict__ partials, int n
 
 // a grid-stride walk over the cells that keeps (i, j) without a division
 // per cell: one division per thread for the start and the stride
+// XCD-aware (as xcd_pairs): when the grid is a multiple of 8, workgroup b
+// walks the (b % 8)-th of 8 contiguous ranges of cells, so that a cell's
+// rows above and below are read by the same XCD at about the same time
 struct CellWalk {
   int64_t k, i, j, n, nx, step, di, dj;
   __device__ CellWalk(int64_t n_, int64_t nx_) : n(n_), nx(nx_)
   {
-    k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    const int parts = (G % 8 == 0) ? 8 : 1;
+    const int64_t per = (n_ + parts - 1) / parts;
+    const int part = (int)(b % parts);
+    k = part * per + (int64_t)(b / parts) * kBlock + threadIdx.x;
+    n = min(n_, (part + 1) * per);
     j = k / nx;
     i = k - j * nx;
-    step = (int64_t)gridDim.x * kBlock;
+    step = (int64_t)(G / parts) * kBlock;
     dj = step / nx;
     di = step - dj * nx;
   }
@@ -137,12 +145,33 @@ __device__ inline bool solve_ended(const double* __restrict__ st) { return st[kS
 // elsewhere (only x and r are updated; acc[1] = 0).  wl2: the residual norm
 // is the weighted one, sum (r w)^2, else sum r^2.  rz comes from the state
 // (slot l & 1) when st is given, else from the argument.
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, int nparts, const double* __restrict__ parts_a,
-                                                       double rz_arg, const double* __restrict__ st, int l,
+// With JAC the update kernel forms the Jacobi inverse diagonal itself
+// instead of reading it: dinv = 1 / (1 - gp d), d = -((ax[i] + ax[i+1]) +
+// (ay[j] + ay[j+1])), exactly k_psetup's value for the gamma gp of the last
+// preconditioner setup: 88 bytes per cell and iteration instead of 96.  (Not
+// storing z at all and forming it at the stencil's five points instead
+// saves 16 more bytes, but the eight divisions per pair of cells make the
+// stencil kernel slower than the traffic saved.)
+struct JacobiDiag {
+  const double *ax, *ay;
+  double gp;
+};
+__device__ inline double jac_dinv(const JacobiDiag& J, double sxi, double syj)
+{
+  const double d = -(sxi + syj);
+  return 1.0 / (1.0 - J.gp * d);
+}
+
+// (the cell-wise update; JAC: mode 1 with the inverse diagonal formed from
+// the face coefficients as above)
+template <bool JAC>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, int64_t nx, int nparts,
+                                                       const double* __restrict__ parts_a, double rz_arg,
+                                                       const double* __restrict__ st, int l,
                                                        const double* __restrict__ p,
                                                        const double* __restrict__ ap, double* __restrict__ x,
                                                        double* __restrict__ r, double* __restrict__ z,
-                                                       const double* __restrict__ dinv,
+                                                       const double* __restrict__ dinv, JacobiDiag J,
                                                        const double* __restrict__ w, int mode,
                                                        double* __restrict__ partials)
 {
@@ -151,14 +180,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, int nparts, co
   const double rz = st ? st[l & 1] : rz_arg;
   const double alpha = rz / pap;
   double acc[2] = {0.0, 0.0};
-  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBlock) {
+  for (CellWalk c(n, nx); c.valid(); c.next()) {
+    const int64_t k = c.k;
     x[k] = x[k] + alpha * p[k];
     const double rk = r[k] + (-alpha) * ap[k];
     r[k] = rk;
     const double rw = w ? rk * w[k] : rk;
     acc[0] += rw * rw;
     if (mode != 2) {
-      const double zk = (mode == 1) ? dinv[k] * rk : rk;
+      double zk = rk;
+      if (JAC) zk = jac_dinv(J, J.ax[c.i] + J.ax[c.i + 1], J.ay[c.j] + J.ay[c.j + 1]) * rk;
+      else if (mode == 1) zk = dinv[k] * rk;
       z[k] = zk;
       acc[1] += rk * zk;
     }
@@ -265,23 +297,6 @@ __device__ inline double2 ld2(const double* __restrict__ a, int64_t k)
   return *reinterpret_cast<const double2*>(a + k);
 }
 __device__ inline void st2(double* __restrict__ a, int64_t k, double2 v) { *reinterpret_cast<double2*>(a + k) = v; }
-
-// With JAC the update kernel forms the Jacobi inverse diagonal itself
-// instead of reading it: dinv = 1 / (1 - gp d), d = -((ax[i] + ax[i+1]) +
-// (ay[j] + ay[j+1])), exactly k_psetup's value for the gamma gp of the last
-// preconditioner setup: 88 bytes per cell and iteration instead of 96.  (Not
-// storing z at all and forming it at the stencil's five points instead
-// saves 16 more bytes, but the eight divisions per pair of cells make the
-// stencil kernel slower than the traffic saved.)
-struct JacobiDiag {
-  const double *ax, *ay;
-  double gp;
-};
-__device__ inline double jac_dinv(const JacobiDiag& J, double sxi, double syj)
-{
-  const double d = -(sxi + syj);
-  return 1.0 / (1.0 - J.gp * d);
-}
 
 template <bool PER>
 __global__ __launch_bounds__(kBlock) void k_pcg_dirap_pairs(int64_t nx, int64_t ny, const double* __restrict__ fx,
@@ -555,8 +570,8 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
       } else {
         hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
                            pold, q[it & 1], dap, (int)g, pb, pc, pa, st, it);
-        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
-                           dap, dx, dr, dz, dinv, w, pmode, pb);
+        hipLaunchKernelGGL(k_pcg_update<false>, dim3(g), dim3(kBlock), 0, s, n, nx, (int)g, pa, 0.0, st, it,
+                           q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
       }
       // after the bulk, and every kPcgChunk iterations within it, the host
       // waits for the residual test (chunks keep streams that share a
@@ -618,8 +633,12 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
       } else {
         hipLaunchKernelGGL(k_pcg_dirap, dim3(g), dim3(kBlock), 0, s, nx, ny, P->d_fx, P->d_fy, per, gamma, dz,
                            pold, q[it & 1], dap, (int)g, pb, nullptr, pa, st, it);
-        hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, (int)g, pa, 0.0, st, it, q[it & 1],
-                           dap, dx, dr, dz, dinv, w, pmode, pb);
+        if (jac)
+          hipLaunchKernelGGL(k_pcg_update<true>, dim3(g), dim3(kBlock), 0, s, n, nx, (int)g, pa, 0.0, st, it,
+                             q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
+        else
+          hipLaunchKernelGGL(k_pcg_update<false>, dim3(g), dim3(kBlock), 0, s, n, nx, (int)g, pa, 0.0, st, it,
+                             q[it & 1], dap, dx, dr, dz, dinv, J, w, pmode, pb);
       }
     }
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, (int)g, pb, nullptr, st, last + 1);
