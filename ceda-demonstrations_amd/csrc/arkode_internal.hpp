@@ -135,8 +135,8 @@ void lsrk_after(ArkMem* A, double dsm);
 // stage table of one attempt for passes of up to kp stages (kp = 1: one
 // kernel per stage)
 void fused_fill(ArkMem* A, const StagePlan& SP, StageEntry* T, int kp);
-// PCG in three fused kernels per iteration (pcg_fused.hip); pmode 0 none,
-// 1 Jacobi (P->diag), 2 the preconditioner solve ps
+// PCG in fused kernels with the scalars on the device (pcg_fused.hip);
+// pmode 0 none, 1 Jacobi (P->diag), 2 the preconditioner solve ps
 int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKLsPrecSolveFn ps, void* ud,
               double t, double gamma, N_Vector b, N_Vector x);
 // true (after a message on stderr) when a Krylov scalar is NaN or infinite:
