@@ -505,34 +505,54 @@ def c5_workload(pk, args):
     # network shape per learned method: (dilations, symmetric); learned-sym-pcg
     # is PCG with the symmetric network (a fixed SPD preconditioner)
     shapes = {"learned": ((1, 1, 1, 1), False), "learned-sym": (sdil, True), "learned-sym-pcg": (sdil, True)}
+    # training operators: with --c5-classes K the problems are split by their
+    # anisotropy r_x/r_y (r = gamma*dt*k/dx^2 along each axis) into K classes of
+    # equal count and one network is trained per class, on the class's
+    # geometric-mean (r_x, r_y); otherwise one isotropic network for all
+    rxy = [(gamma_a * dt * p["kx"] * (p["nx"] + 1) ** 2, gamma_a * dt * p["ky"] * (p["ny"] + 1) ** 2)
+           for p in probs]
+    if args.c5_classes > 0:
+        order = sorted(range(len(probs)), key=lambda k: rxy[k][0] / rxy[k][1])
+        K = min(args.c5_classes, len(probs))
+        groups = [order[c * len(order) // K:(c + 1) * len(order) // K] for c in range(K)]
+        cls_of = {k: c for c, g in enumerate(groups) for k in g}
+        targets = [tuple(float(np.exp(np.mean([np.log(rxy[k][a]) for k in g]))) for a in (0, 1)) for g in groups]
+    else:
+        ratio = args.c5_train_ratio if args.c5_train_ratio > 0 else float(np.median(ratios))
+        cls_of = {k: 0 for k in range(len(probs))}
+        targets = [(ratio, ratio)]
     shared = {}
     train = {}
     for kind in methods:
         if kind not in shapes or (kind == "learned-sym-pcg" and "learned-sym" in shared):
             continue
         dil, sym = shapes[kind]
-        ctx = pk.Context()
-        ratio = args.c5_train_ratio if args.c5_train_ratio > 0 else float(np.median(ratios))
         tn = args.sym_train_n if sym else args.train_n
         train_steps = args.sym_train_steps if sym else args.train_steps
         lr = args.sym_train_lr if sym else args.train_lr
         nsamp = args.sym_nsamp if sym else args.nsamp
-        t0 = time.perf_counter()
-        R, E, iters, rc = pk.training_set(ctx, nsamp, tn, ratio, reltol=1e-12, seed=1234)
-        assert rc == 0
-        net = pk.LearnedPrec(ctx, dil, symmetric=sym)
-        net.init_identity(8e-4, 0)
-        net.set_output_scale(1.0 / (1.0 + 4.0 * ratio))
-        hist = net.train(nsamp, tn, ratio, R, E, train_steps, args.train_batch, lr, 0)
-        hip_sync()
-        shared[kind] = net.params()
+        shared[kind], train[kind] = [], []
+        for (rx, ry) in targets:
+            ctx = pk.Context()
+            t0 = time.perf_counter()
+            R, E, iters, rc = pk.training_set(ctx, nsamp, tn, rx, reltol=1e-12, seed=1234, ratio_y=ry)
+            assert rc == 0
+            net = pk.LearnedPrec(ctx, dil, symmetric=sym)
+            net.init_identity(8e-4, 0)
+            net.set_output_scale(1.0 / (1.0 + 2.0 * rx + 2.0 * ry))
+            hist = net.train(nsamp, tn, rx, R, E, train_steps, args.train_batch, lr, 0, ratio_y=ry)
+            hip_sync()
+            shared[kind].append(net.params())
+            rec = dict(ratio_x=rx, ratio_y=ry, pairs=nsamp, grid=tn, steps=train_steps, lr=lr,
+                       dilations=list(dil), symmetric=sym, seconds=time.perf_counter() - t0,
+                       loss_first20=float(hist[:20].mean()), loss_last50=float(hist[-50:].mean()))
+            if len(targets) > 1:
+                rec["problems"] = sum(1 for c in cls_of.values() if c == len(train[kind]))
+            train[kind].append(rec)
+            log(f"{kind} network trained at ratios ({rx:.0f}, {ry:.0f}): {rec}")
+            ctx.close()
         if kind in ("learned-sym", "learned-sym-pcg"):  # one symmetric network for both
             shared["learned-sym"] = shared["learned-sym-pcg"] = shared[kind]
-        train[kind] = dict(ratio=ratio, pairs=nsamp, grid=tn, steps=train_steps, lr=lr, dilations=list(dil),
-                           symmetric=sym, seconds=time.perf_counter() - t0,
-                           loss_first20=float(hist[:20].mean()), loss_last50=float(hist[-50:].mean()))
-        log(f"shared {kind} network trained at ratio {ratio:.0f}: {train[kind]}")
-        ctx.close()
 
     def solve_one(ctx, net, k, kind, t_end):
         p = probs[k]
@@ -574,17 +594,18 @@ def c5_workload(pk, args):
 
         def worker():
             ctx = pk.Context()
-            net = None
+            nets = []
             if kind in shapes:
-                net = pk.LearnedPrec(ctx, shapes[kind][0], symmetric=shapes[kind][1])
-                net.set_params(shared[kind])
+                for params in shared[kind]:
+                    nets.append(pk.LearnedPrec(ctx, shapes[kind][0], symmetric=shapes[kind][1]))
+                    nets[-1].set_params(params)
             try:
                 while True:
                     with lock:
                         if not todo or errors:
                             break
                         k = todo.pop(0)
-                    recs[k] = solve_one(ctx, net, k, kind, t_end)
+                    recs[k] = solve_one(ctx, nets[cls_of[k]] if nets else None, k, kind, t_end)
                     with lock:
                         done = sum(r is not None for r in recs)
                     if done % 8 == 0 or workers == 1:
@@ -613,8 +634,9 @@ def c5_workload(pk, args):
         out = dict(seconds=el, workers=workers, solves_per_s=len(probs) / el,
                    krylov_iters_total=int(its.sum()), krylov_iters_max=int(its.max()),
                    fallbacks=int(sum(r["fallbacks"] for r in recs)),
-                   l2_err_max=float(errs.max()), l2_err_median=float(np.median(errs)))
-        log(f"{kind} x{workers}: {out}")
+                   l2_err_max=float(errs.max()), l2_err_median=float(np.median(errs)),
+                   krylov_iters_per_problem=[int(x) for x in its])
+        log(f"{kind} x{workers}: { {k: v for k, v in out.items() if k != 'krylov_iters_per_problem'} }")
         return out, errs
 
     res = {}
@@ -628,8 +650,11 @@ def c5_workload(pk, args):
            "config": {"workload": f"{len(probs)} mms-dirichlet problems, side U[256,1024], kx,ky logU[0.1,10] "
                                   f"(seed 20260102), {cells} cells, DIRK3 fixed dt=1e-3, {steps} steps, "
                                   "LS reltol 1e-8, analytic J v",
+                      "learned_networks": (f"{len(targets)}, one per anisotropy class" if args.c5_classes > 0
+                                           else "1, isotropic, shared by all problems"),
                       "concurrency": f"{args.c5_workers} host threads, one SUNContext/stream each"},
-           "results": res, "training": train}
+           "results": res, "training": train,
+           "problems": [[p["nx"], round(p["kx"], 4), round(p["ky"], 4)] for p in probs]}
     if "jacobi" in res:
         for kind in res:
             if kind != "jacobi":
@@ -738,6 +763,9 @@ def parse_args(argv=None):
     ap.add_argument("--c5-methods", default="jacobi,mg")
     ap.add_argument("--c5-train-ratio", type=float, default=0.0,
                     help="c5: gamma*dt*k/dx^2 the shared network is trained at (0: the batch median)")
+    ap.add_argument("--c5-classes", type=int, default=0,
+                    help="c5: learned methods train one network per class of problem anisotropy (this many "
+                         "classes of equal count); 0: one isotropic network shared by every problem")
     ap.add_argument("--tout", type=float, default=0.5)
     ap.add_argument("--conv-kernel", type=int, default=1, choices=[0, 1, 2],
                     help="learned network's convolutions: 1 LDS row chains, 2 LDS row chains and two layers per "

@@ -150,6 +150,9 @@ _SIGS = {
     "LearnedPrec_LossGrad": (i32, [vp, i32, i32, i32, d, vp, vp, dp, C.POINTER(C.c_float)]),
     "LearnedPrec_MakeTrainingSet": (i32, [i32, i32, d, d, C.c_ulong, vp, vp, C.POINTER(i32), vp]),
     "LearnedPrec_Train": (i32, [vp, i32, i32, d, vp, vp, i32, i32, d, C.c_ulong, dp]),
+    "LearnedPrec_LossGradXY": (i32, [vp, i32, i32, i32, d, d, vp, vp, dp, C.POINTER(C.c_float)]),
+    "LearnedPrec_MakeTrainingSetXY": (i32, [i32, i32, d, d, d, C.c_ulong, vp, vp, C.POINTER(i32), vp]),
+    "LearnedPrec_TrainXY": (i32, [vp, i32, i32, d, d, vp, vp, i32, i32, d, C.c_ulong, dp]),
     "DiffusionProblem_SetLearnedPrec": (i32, [vp, vp]),
     "ARKodeSetLSRelativeTolerance": (i32, [vp, d]),
     "SUNLinSol_PCG": (vp, [vp, i32, i32, vp]),
@@ -634,20 +637,26 @@ class LearnedPrec:
     def apply(self, nb, nx, ny, vin, vout, addto=0):
         return lib().LearnedPrec_Apply(self.h, nb, nx, ny, dptr_of(vin), dptr_of(vout), addto)
 
-    def loss_grad(self, nb, nx, ny, ratio, r, e, grad=True):
+    def loss_grad(self, nb, nx, ny, ratio, r, e, grad=True, ratio_y=None):
+        """Training loss and gradient on (I - ratio Dxx - ratio_y Dyy) e = r
+        (ratio_y defaults to ratio: the isotropic operator)."""
         import numpy as np
         loss = C.c_double()
         g = np.zeros(self.nparams, dtype=np.float32) if grad else None
-        rc = lib().LearnedPrec_LossGrad(self.h, nb, nx, ny, ratio, dptr_of(r), dptr_of(e), C.byref(loss),
-                                        g.ctypes.data_as(C.POINTER(C.c_float)) if grad else None)
+        ry = ratio if ratio_y is None else ratio_y
+        rc = lib().LearnedPrec_LossGradXY(self.h, nb, nx, ny, ratio, ry, dptr_of(r), dptr_of(e), C.byref(loss),
+                                          g.ctypes.data_as(C.POINTER(C.c_float)) if grad else None)
         assert rc == 0
         return loss.value, g
 
-    def train(self, nsamp, n, ratio, r, e, steps, batch, lr, seed=0):
+    def train(self, nsamp, n, ratio, r, e, steps, batch, lr, seed=0, ratio_y=None):
+        """Adam on the pairs for (I - ratio Dxx - ratio_y Dyy) (ratio_y
+        defaults to ratio); returns the loss of every step."""
         import numpy as np
         hist = np.zeros(steps)
-        rc = lib().LearnedPrec_Train(self.h, nsamp, n, ratio, dptr_of(r), dptr_of(e), steps, batch, lr, seed,
-                                     hist.ctypes.data_as(C.POINTER(C.c_double)))
+        ry = ratio if ratio_y is None else ratio_y
+        rc = lib().LearnedPrec_TrainXY(self.h, nsamp, n, ratio, ry, dptr_of(r), dptr_of(e), steps, batch, lr,
+                                       seed, hist.ctypes.data_as(C.POINTER(C.c_double)))
         assert rc == 0
         return hist
 
@@ -666,12 +675,14 @@ class LearnedPrec:
             pass
 
 
-def training_set(ctx, nsamp, n, ratio, reltol=1e-12, seed=1234):
-    """(r, e) pairs on the GPU: two Vectors of nsamp*n*n values and the largest
-    Jacobi-PCG iteration count."""
+def training_set(ctx, nsamp, n, ratio, reltol=1e-12, seed=1234, ratio_y=None):
+    """(r, e) pairs on the GPU for (I - ratio Dxx - ratio_y Dyy) e = r
+    (ratio_y defaults to ratio): two Vectors of nsamp*n*n values and the
+    largest Jacobi-PCG iteration count."""
     r = Vector(ctx, n=nsamp * n * n)
     e = Vector(ctx, n=nsamp * n * n)
     it = i32()
-    rc = lib().LearnedPrec_MakeTrainingSet(nsamp, n, ratio, reltol, seed, dptr_of(r), dptr_of(e),
-                                          C.byref(it), ctx.h)
+    ry = ratio if ratio_y is None else ratio_y
+    rc = lib().LearnedPrec_MakeTrainingSetXY(nsamp, n, ratio, ry, reltol, seed, dptr_of(r), dptr_of(e),
+                                            C.byref(it), ctx.h)
     return r, e, it.value, rc

@@ -1012,8 +1012,15 @@ __global__ __launch_bounds__(kBlock) void k_rms_final(int nb, int P, int64_t npi
   inv[b] = 1.0 / v;
 }
 
-// y = (I - ratio*L) x on nb stacked nx*ny Dirichlet grids (unit spacing)
-__global__ __launch_bounds__(kBlock) void k_applyA(int nb, int nx, int ny, double ratio,
+// (I - rx Dxx - ry Dyy) at one cell of a unit-spacing grid from its five
+// values; rx == ry takes the isotropic association c - r((((w + e) + s) + n) - 4c)
+__device__ inline double op5(double c, double w, double e, double s, double no, double rx, double ry)
+{
+  if (rx == ry) return c - rx * ((((w + e) + s) + no) - 4.0 * c);
+  return c - (rx * ((w + e) - 2.0 * c) + ry * ((s + no) - 2.0 * c));
+}
+// y = (I - rx Dxx - ry Dyy) x on nb stacked nx*ny Dirichlet grids (unit spacing)
+__global__ __launch_bounds__(kBlock) void k_applyA(int nb, int nx, int ny, double rx, double ry,
                                                    const double* __restrict__ x, double* __restrict__ y)
 {
   const int64_t n = (int64_t)nb * nx * ny;
@@ -1023,7 +1030,7 @@ __global__ __launch_bounds__(kBlock) void k_applyA(int nb, int nx, int ny, doubl
     const double c = x[k];
     const double w = i > 0 ? x[k - 1] : 0.0, e = i < nx - 1 ? x[k + 1] : 0.0;
     const double s = j > 0 ? x[k - nx] : 0.0, no = j < ny - 1 ? x[k + nx] : 0.0;
-    y[k] = c - ratio * ((((w + e) + s) + no) - 4.0 * c);
+    y[k] = op5(c, w, e, s, no, rx, ry);
   }
 }
 
@@ -1149,9 +1156,10 @@ __global__ __launch_bounds__(kBlock) void k_coeffs(int nsamp, int n, uint64_t se
     c[q] = v;
   }
 }
-// out[i][y][x] = sum_k S[x][k] in[i][k][y'] ... two passes of the separable
-// sine synthesis: pass 0 t[i][k][x] = sum_l c[i][k][l] S[x][l];
-// pass 1 r[i][y][x] = sum_k S[y][k] t[i][k][x]
+// the two passes of the separable sine synthesis: pass 0
+// t[i][k][x] = sum_l c[i][k][l] S[x][l], pass 1 r[i][y][x] = sum_k S[y][k] t[i][k][x].
+// S is symmetric (its entries are sin(pi (k+1)(x+1)/(n+1)), the same double
+// either way round), so pass 0 reads S[l][x], consecutive across the threads
 __global__ __launch_bounds__(kBlock) void k_synth(int nsamp, int n, int pass, const double* __restrict__ S,
                                                   const double* __restrict__ in, double* __restrict__ out)
 {
@@ -1162,7 +1170,7 @@ __global__ __launch_bounds__(kBlock) void k_synth(int nsamp, int n, int pass, co
     const double* base = in + i * n * n;
     double s = 0.0;
     if (pass == 0)
-      for (int l = 0; l < n; l++) s += base[(int64_t)a * n + l] * S[(int64_t)bcol * n + l];
+      for (int l = 0; l < n; l++) s += base[(int64_t)a * n + l] * S[(int64_t)l * n + bcol];
     else
       for (int k = 0; k < n; k++) s += S[(int64_t)a * n + k] * base[(int64_t)k * n + bcol];
     out[q] = s;
@@ -1187,68 +1195,104 @@ __device__ inline double block_sum(double v, double* sh)
   __syncthreads();
   return ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
+// (a, b) summed over the workgroup, each exactly as block_sum sums it
+__device__ inline void block_sum2(double& a, double& b, double* sh)
+{
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, kWave);
+    b += __shfl_xor(b, off, kWave);
+  }
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    sh[threadIdx.x / kWave] = a;
+    sh[kWavesPB + threadIdx.x / kWave] = b;
+  }
+  __syncthreads();
+  a = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+  b = ((sh[kWavesPB] + sh[kWavesPB + 1]) + sh[kWavesPB + 2]) + sh[kWavesPB + 3];
+}
 struct CGState {
-  double rz, bb;
+  double rz, bb, beta;  // r.z, b.b, the next direction's beta
   int done, iters;
 };
+// x = 0, r = b, z = b / dg, and the direction before the first one zero (so
+// that the first direction r / dg + beta * 0 is z itself)
 __global__ __launch_bounds__(kBlock) void k_cg_init(int n, double dg, const double* __restrict__ bvec,
                                                     double* __restrict__ x, double* __restrict__ r,
                                                     double* __restrict__ p, CGState* __restrict__ st)
 {
   const int64_t o = (int64_t)blockIdx.x * n * n;
-  __shared__ double sh[kWavesPB];
+  __shared__ double sh[2 * kWavesPB];
   double rz = 0.0, bb = 0.0;
   for (int k = threadIdx.x; k < n * n; k += kBlock) {
     const double b = bvec[o + k];
     x[o + k] = 0.0;
     r[o + k] = b;
-    p[o + k] = b / dg;
+    p[o + k] = 0.0;
     rz += b * (b / dg);
     bb += b * b;
   }
-  rz = block_sum(rz, sh);
-  bb = block_sum(bb, sh);
-  if (threadIdx.x == 0) st[blockIdx.x] = CGState{rz, bb, bb == 0.0 ? 1 : 0, 0};
+  block_sum2(rz, bb, sh);
+  if (threadIdx.x == 0) st[blockIdx.x] = CGState{rz, bb, 0.0, bb == 0.0 ? 1 : 0, 0};
 }
-__global__ __launch_bounds__(kBlock) void k_cg_iter(int n, double ratio, double dg, double tol2,
+// One Jacobi-PCG iteration on (I - rx Dxx - ry Dyy) of one pair per
+// workgroup, in two sweeps over the grid (constant diagonal dg, so z = r / dg):
+//   sweep 1  p = z + beta p_old written to the other direction buffer, and
+//            p.Ap with Ap formed from the new directions of the 5 points
+//            (recomputed from r and p_old, so Ap is never stored);
+//   sweep 2  Ap again from the stored p, x += alpha p, r -= alpha Ap, and
+//            r.r and r.z.
+// The iterates are those of the textbook loop (ap = A p; alpha; x, r; beta;
+// p = z + beta p): every value is formed by the same expression from the same
+// operands.  Eight vector sweeps of HBM per iteration instead of eleven.
+__global__ __launch_bounds__(kBlock) void k_cg_iter(int n, double rx, double ry, double dg, double tol2,
                                                     double* __restrict__ x, double* __restrict__ r,
-                                                    double* __restrict__ p, double* __restrict__ ap,
+                                                    double* __restrict__ pa, double* __restrict__ pb,
                                                     CGState* __restrict__ st)
 {
-  if (st[blockIdx.x].done) return;
+  const CGState S = st[blockIdx.x];
+  if (S.done) return;
   const int64_t o = (int64_t)blockIdx.x * n * n;
-  __shared__ double sh[kWavesPB];
+  double* __restrict__ xs = x + o;
+  double* __restrict__ rs = r + o;
+  const double* __restrict__ pin = ((S.iters & 1) ? pb : pa) + o;
+  double* __restrict__ pout = ((S.iters & 1) ? pa : pb) + o;
+  const int nn = n * n, di = kBlock % n, dj = kBlock / n, i0 = threadIdx.x % n, j0 = threadIdx.x / n;
+  const double beta = S.beta;
+  __shared__ double sh[2 * kWavesPB];
+  auto dir = [&](int q) { return rs[q] / dg + beta * pin[q]; };
   double pap = 0.0;
-  for (int k = threadIdx.x; k < n * n; k += kBlock) {
-    const int i = k % n, j = k / n;
-    const double c = p[o + k];
-    const double w = i > 0 ? p[o + k - 1] : 0.0, e = i < n - 1 ? p[o + k + 1] : 0.0;
-    const double s = j > 0 ? p[o + k - n] : 0.0, no = j < n - 1 ? p[o + k + n] : 0.0;
-    const double v = c - ratio * ((((w + e) + s) + no) - 4.0 * c);
-    ap[o + k] = v;
-    pap += c * v;
+  for (int k = threadIdx.x, i = i0, j = j0; k < nn; k += kBlock) {
+    const double c = dir(k);
+    const double w = i > 0 ? dir(k - 1) : 0.0, e = i < n - 1 ? dir(k + 1) : 0.0;
+    const double s = j > 0 ? dir(k - n) : 0.0, no = j < n - 1 ? dir(k + n) : 0.0;
+    pout[k] = c;
+    pap += c * op5(c, w, e, s, no, rx, ry);
+    i += di;
+    j += dj;
+    if (i >= n) i -= n, j++;
   }
+  // the barriers of the sum also make every sweep-1 store of pout visible to
+  // the workgroup before sweep 2 reads its neighbours
   pap = block_sum(pap, sh);
-  const double alpha = st[blockIdx.x].rz / pap;
+  const double alpha = S.rz / pap;
   double rr = 0.0, rz = 0.0;
-  for (int k = threadIdx.x; k < n * n; k += kBlock) {
-    x[o + k] += alpha * p[o + k];
-    const double rk = r[o + k] - alpha * ap[o + k];
-    r[o + k] = rk;
+  for (int k = threadIdx.x, i = i0, j = j0; k < nn; k += kBlock) {
+    const double c = pout[k];
+    const double w = i > 0 ? pout[k - 1] : 0.0, e = i < n - 1 ? pout[k + 1] : 0.0;
+    const double s = j > 0 ? pout[k - n] : 0.0, no = j < n - 1 ? pout[k + n] : 0.0;
+    const double v = op5(c, w, e, s, no, rx, ry);
+    xs[k] += alpha * c;
+    const double rk = rs[k] - alpha * v;
+    rs[k] = rk;
     rr += rk * rk;
     rz += rk * (rk / dg);
+    i += di;
+    j += dj;
+    if (i >= n) i -= n, j++;
   }
-  rr = block_sum(rr, sh);
-  rz = block_sum(rz, sh);
-  const double beta = rz / st[blockIdx.x].rz;
-  for (int k = threadIdx.x; k < n * n; k += kBlock) p[o + k] = r[o + k] / dg + beta * p[o + k];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    CGState& S = st[blockIdx.x];
-    S.rz = rz;
-    S.iters++;
-    if (rr <= tol2 * S.bb) S.done = 1;
-  }
+  block_sum2(rr, rz, sh);
+  if (threadIdx.x == 0) st[blockIdx.x] = CGState{rz, S.bb, rz / S.rz, rr <= tol2 * S.bb ? 1 : 0, S.iters + 1};
 }
 
 unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 8192); }
@@ -1757,7 +1801,7 @@ void lp_backward_sym(LearnedPrec* L, int nb, int nx, int ny, const double* r)
 }
 
 // loss (device scalar *dloss) and parameter gradient (L->d_g) of a batch
-void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const double* r,
+void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double rx, double ry, const double* r,
                  const double* e, double* dloss)
 {
   hipStream_t s = L->ex->stream;
@@ -1767,7 +1811,7 @@ void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const dou
   lp_forward(L, nb, nx, ny, r, L->u, 0, true);
   const unsigned gg = grid_for(tot);
   hipLaunchKernelGGL(k_diff, dim3(gg), dim3(kBlock), 0, s, npix, nb, L->u, e, L->isc, L->dd);
-  hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nb, nx, ny, ratio, L->dd, L->ad);
+  hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nb, nx, ny, rx, ry, L->dd, L->ad);
   const int P = rms_parts(npix);
   hipLaunchKernelGGL(k_dot2, dim3(P, nb), dim3(kBlock), 0, s, npix, P, L->dd, L->ad, e, r, L->part);
   hipLaunchKernelGGL(k_loss_final, dim3((nb + 63) / 64), dim3(64), 0, s, nb, P, L->part, L->isc,
@@ -2005,13 +2049,14 @@ static int lp_apply_scaled(LearnedPrec* L, int nx, int ny, const double* in, dou
   return r;
 }
 
-int LearnedPrec_LossGrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const double* r,
-                         const double* e, double* loss, float* grad)
+int LearnedPrec_LossGradXY(LearnedPrec* L, int nb, int nx, int ny, double ratio_x, double ratio_y,
+                           const double* r, const double* e, double* loss, float* grad)
 {
-  if (!L || nb < 1 || nx < 1 || ny < 1 || !r || !e || !loss) return -1;
+  if (!L || nb < 1 || nx < 1 || ny < 1 || !(ratio_x >= 0.0) || !(ratio_y >= 0.0) || !r || !e || !loss)
+    return -1;
   double* dl = nullptr;
   CEDA_HIP_CHECK(ceda::dev_malloc(&dl, sizeof(double)));
-  lp_lossgrad(L, nb, nx, ny, ratio, r, e, dl);
+  lp_lossgrad(L, nb, nx, ny, ratio_x, ratio_y, r, e, dl);
   CEDA_HIP_CHECK(hipMemcpyAsync(loss, dl, sizeof(double), hipMemcpyDeviceToHost, L->ex->stream));
   if (grad)
     CEDA_HIP_CHECK(hipMemcpyAsync(grad, L->d_g, sizeof(float) * L->np, hipMemcpyDeviceToHost, L->ex->stream));
@@ -2020,33 +2065,42 @@ int LearnedPrec_LossGrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, c
   return 0;
 }
 
-int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, unsigned long seed,
-                                double* r, double* e, int* max_iters, SUNContext ctx)
+int LearnedPrec_LossGrad(LearnedPrec* L, int nb, int nx, int ny, double ratio, const double* r,
+                         const double* e, double* loss, float* grad)
 {
-  if (nsamp < 1 || n < 2 || n > 1024 || !(ratio >= 0.0) || !(reltol > 0.0) || !r || !e || !ctx) return -1;
+  return LearnedPrec_LossGradXY(L, nb, nx, ny, ratio, ratio, r, e, loss, grad);
+}
+
+int LearnedPrec_MakeTrainingSetXY(int nsamp, int n, double ratio_x, double ratio_y, double reltol,
+                                  unsigned long seed, double* r, double* e, int* max_iters, SUNContext ctx)
+{
+  if (nsamp < 1 || n < 2 || n > 1024 || !(ratio_x >= 0.0) || !(ratio_y >= 0.0) || !(reltol > 0.0) || !r ||
+      !e || !ctx)
+    return -1;
   hipStream_t s = ceda::context_exec(ctx)->stream;
   const int64_t tot = (int64_t)nsamp * n * n;
   std::vector<double> S((size_t)n * n);
   for (int x = 0; x < n; x++)
-    for (int k = 0; k < n; k++) S[(size_t)x * n + k] = sin(M_PI * (double)(k + 1) * (double)(x + 1) / (double)(n + 1));
-  double *dS = nullptr, *tmp = nullptr, *p = nullptr, *ap = nullptr;
+    for (int k = 0; k < n; k++)  // (k+1)(x+1) exact: S[x][k] == S[k][x] bitwise
+      S[(size_t)x * n + k] = sin(M_PI * ((double)(k + 1) * (double)(x + 1)) / (double)(n + 1));
+  double *dS = nullptr, *tmp = nullptr, *p = nullptr, *p2 = nullptr;
   CGState* st = nullptr;
   CEDA_HIP_CHECK(ceda::dev_malloc(&dS, sizeof(double) * n * n));
   CEDA_HIP_CHECK(ceda::dev_malloc(&tmp, sizeof(double) * tot));
   CEDA_HIP_CHECK(ceda::dev_malloc(&p, sizeof(double) * tot));
-  CEDA_HIP_CHECK(ceda::dev_malloc(&ap, sizeof(double) * tot));
+  CEDA_HIP_CHECK(ceda::dev_malloc(&p2, sizeof(double) * tot));
   CEDA_HIP_CHECK(ceda::dev_malloc(&st, sizeof(CGState) * nsamp));
   CEDA_HIP_CHECK(hipMemcpyAsync(dS, S.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
   const unsigned gg = grid_for(tot);
-  // fields: even samples smooth sine sums, odd samples (I - ratio L) applied to
+  // fields: even samples smooth sine sums, odd samples (I - rx Dxx - ry Dyy) applied to
   // one (rough: the high-frequency content a Krylov residual carries)
   hipLaunchKernelGGL(k_coeffs, dim3(gg), dim3(kBlock), 0, s, nsamp, n, (uint64_t)seed, p);
   hipLaunchKernelGGL(k_synth, dim3(gg), dim3(kBlock), 0, s, nsamp, n, 0, dS, p, tmp);
   hipLaunchKernelGGL(k_synth, dim3(gg), dim3(kBlock), 0, s, nsamp, n, 1, dS, tmp, r);
-  hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nsamp, n, n, ratio, r, tmp);
+  hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nsamp, n, n, ratio_x, ratio_y, r, tmp);
   hipLaunchKernelGGL(k_take_odd, dim3(gg), dim3(kBlock), 0, s, (int64_t)n * n, nsamp, tmp, r);
   // Jacobi-PCG on every pair
-  const double dg = 1.0 + 4.0 * ratio;
+  const double dg = 1.0 + (2.0 * ratio_x + 2.0 * ratio_y);  // 1 + 4 ratio exactly when isotropic
   hipLaunchKernelGGL(k_cg_init, dim3(nsamp), dim3(kBlock), 0, s, n, dg, r, e, tmp, p, st);
   std::vector<CGState> hst(nsamp);
   const int maxit = 20000;
@@ -2054,8 +2108,9 @@ int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, u
   bool all = false;
   while (!all && it < maxit) {
     for (int k = 0; k < 50; k++, it++)
-      hipLaunchKernelGGL(k_cg_iter, dim3(nsamp), dim3(kBlock), 0, s, n, ratio, dg, reltol * reltol, e, tmp, p,
-                         ap, st);
+      hipLaunchKernelGGL(k_cg_iter, dim3(nsamp), dim3(kBlock), 0, s, n, ratio_x, ratio_y, dg, reltol * reltol, e,
+                         tmp, p,
+                         p2, st);
     CEDA_HIP_CHECK(hipMemcpyAsync(hst.data(), st, sizeof(CGState) * nsamp, hipMemcpyDeviceToHost, s));
     CEDA_HIP_CHECK(hipStreamSynchronize(s));
     all = true;
@@ -2066,15 +2121,23 @@ int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, u
   if (max_iters) *max_iters = mx;
   CEDA_HIP_CHECK(hipGetLastError());
   CEDA_HIP_CHECK(hipStreamSynchronize(s));
-  void* ps[] = {dS, tmp, p, ap, st};
+  void* ps[] = {dS, tmp, p, p2, st};
   for (void* q : ps) CEDA_HIP_CHECK(ceda::dev_free(q));
   return all ? 0 : 1;
 }
 
-int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const double* r, const double* e,
-                      int steps, int batch, double lr0, unsigned long seed, double* loss_hist)
+int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol, unsigned long seed,
+                                double* r, double* e, int* max_iters, SUNContext ctx)
 {
-  if (!L || nsamp < 1 || n < 2 || steps < 1 || batch < 1 || !r || !e || !(lr0 > 0.0)) return -1;
+  return LearnedPrec_MakeTrainingSetXY(nsamp, n, ratio, ratio, reltol, seed, r, e, max_iters, ctx);
+}
+
+int LearnedPrec_TrainXY(LearnedPrec* L, int nsamp, int n, double ratio_x, double ratio_y, const double* r,
+                        const double* e, int steps, int batch, double lr0, unsigned long seed, double* loss_hist)
+{
+  if (!L || nsamp < 1 || n < 2 || steps < 1 || batch < 1 || !(ratio_x >= 0.0) || !(ratio_y >= 0.0) || !r ||
+      !e || !(lr0 > 0.0))
+    return -1;
   hipStream_t s = L->ex->stream;
   const int64_t npix = (int64_t)n * n;
   double *rb = nullptr, *eb = nullptr, *dloss = nullptr;
@@ -2091,7 +2154,7 @@ int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const doub
   const unsigned gg = grid_for(npix * batch);
   for (int it = 0; it < steps; it++) {
     hipLaunchKernelGGL(k_gather, dim3(gg), dim3(kBlock), 0, s, npix, batch, didx + (int64_t)it * batch, r, e, rb, eb);
-    lp_lossgrad(L, batch, n, n, ratio, rb, eb, dloss + it);
+    lp_lossgrad(L, batch, n, n, ratio_x, ratio_y, rb, eb, dloss + it);
     const double lr = lr0 * 0.5 * (1.0 + cos(M_PI * (double)it / (double)steps));
     adam_step(L, lr);
   }
@@ -2101,6 +2164,12 @@ int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const doub
   void* ps[] = {rb, eb, dloss, didx};
   for (void* q : ps) CEDA_HIP_CHECK(ceda::dev_free(q));
   return 0;
+}
+
+int LearnedPrec_Train(LearnedPrec* L, int nsamp, int n, double ratio, const double* r, const double* e,
+                      int steps, int batch, double lr0, unsigned long seed, double* loss_hist)
+{
+  return LearnedPrec_TrainXY(L, nsamp, n, ratio, ratio, r, e, steps, batch, lr0, seed, loss_hist);
 }
 
 int DiffusionProblem_SetLearnedPrec(DiffusionProblem* P, LearnedPrec* lp)

@@ -108,6 +108,13 @@ int LearnedPrec_ApplyNoAlloc(LearnedPrec* lp, int nb, int nx, int ny, const doub
    parameters (host FP32, NumParams values).  r, e: FP64 device arrays. */
 int LearnedPrec_LossGrad(LearnedPrec* lp, int nb, int nx, int ny, double ratio, const double* r,
                          const double* e, double* loss, float* grad);
+/* the same objective for the anisotropic operator A = I - ratio_x Dxx -
+   ratio_y Dyy (Dxx, Dyy the unit-spacing second differences along x and y);
+   ratio_x == ratio_y is LearnedPrec_LossGrad bit for bit.  Training a
+   network per anisotropy class: a problem with diffusion coefficients kx, ky
+   has ratio_x / ratio_y = (kx/dx^2) / (ky/dy^2). */
+int LearnedPrec_LossGradXY(LearnedPrec* lp, int nb, int nx, int ny, double ratio_x, double ratio_y,
+                           const double* r, const double* e, double* loss, float* grad);
 
 /* training pairs on the GPU: nsamp smooth random fields r_i (sum of sine
    modes with amplitude ~ N(0,1)/|k| up to a random cut-off, seed + i) on n*n
@@ -117,6 +124,12 @@ int LearnedPrec_LossGrad(LearnedPrec* lp, int nb, int nx, int ny, double ratio, 
 int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol,
                                 unsigned long seed, double* r, double* e, int* max_iters,
                                 SUNContext ctx);
+/* the same pairs for (I - ratio_x Dxx - ratio_y Dyy) e = r (Jacobi diagonal
+   1 + 2 ratio_x + 2 ratio_y); ratio_x == ratio_y is LearnedPrec_MakeTrainingSet
+   bit for bit */
+int LearnedPrec_MakeTrainingSetXY(int nsamp, int n, double ratio_x, double ratio_y, double reltol,
+                                  unsigned long seed, double* r, double* e, int* max_iters,
+                                  SUNContext ctx);
 
 /* Adam (beta 0.9/0.999) on minibatches of `batch` pairs drawn (seeded) from
    the training set, learning rate lr0 with cosine decay over `steps`;
@@ -125,6 +138,10 @@ int LearnedPrec_MakeTrainingSet(int nsamp, int n, double ratio, double reltol,
 int LearnedPrec_Train(LearnedPrec* lp, int nsamp, int n, double ratio, const double* r,
                       const double* e, int steps, int batch, double lr0, unsigned long seed,
                       double* loss_hist);
+/* LearnedPrec_Train with the objective of LearnedPrec_LossGradXY */
+int LearnedPrec_TrainXY(LearnedPrec* lp, int nsamp, int n, double ratio_x, double ratio_y,
+                        const double* r, const double* e, int steps, int batch, double lr0,
+                        unsigned long seed, double* loss_hist);
 
 /* attach a trained network to a problem (NULL detaches); the problem does
    not own it.  The network must have been created on the problem's

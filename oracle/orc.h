@@ -139,13 +139,13 @@ int orc_mg_apply_opts(const orc_problem* p, double gamma, int relax, int nu, con
 /* learned preconditioner network (orc_learned.c) */
 int orc_lp_num_params(int nl, const int* dil);
 double orc_lp_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,
-                       int ny, double ratio, const double* r, const double* e, int emu, double* grad,
-                       double* u_out);
+                       int ny, double ratio_x, double ratio_y, const double* r, const double* e, int emu,
+                       double* grad, double* u_out);
 /* the symmetric network N = K^T K (linear layers, no biases) */
 int orc_lp_sym_num_params(int nl, const int* dil);
 double orc_lp_sym_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,
-                           int ny, double ratio, const double* r, const double* e, int emu, double* grad,
-                           double* u_out);
+                           int ny, double ratio_x, double ratio_y, const double* r, const double* e, int emu,
+                           double* grad, double* u_out);
 
 #ifdef __cplusplus
 }

@@ -5,7 +5,8 @@
  * training objective:
  *   s = RMS(r); x_0 = bf16(r/s); x_{l+1} = bf16(relu(W_l * x_l + b_l));
  *   u = out_scale * (W_L * x_L + b_L);  d = u - e/s;
- *   loss = mean_b d^T A d / ((e.r)/s^2),  A = I - ratio*L (unit 5-point, zero BC)
+ *   loss = mean_b d^T A d / ((e.r)/s^2),  A = I - rx Dxx - ry Dyy (unit-spacing second
+ *   differences, zero BC; rx == ry evaluated as I - rx*L with the 5-point L)
  * and the gradient of the loss with respect to every parameter by reverse
  * accumulation.  With emu = 1 the BF16 roundings of the GPU path are emulated
  * at the same points (weights; network input; every stored activation; the
@@ -140,7 +141,7 @@ static void wgrad_acc(int nb, int nx, int ny, int cin, int cout, int d, const do
         }
 }
 
-static void apply_A(int nb, int nx, int ny, double ratio, const double* x, double* y)
+static void apply_A(int nb, int nx, int ny, double rx, double ry, const double* x, double* y)
 {
   for (int b = 0; b < nb; b++)
     for (int j = 0; j < ny; j++)
@@ -149,15 +150,16 @@ static void apply_A(int nb, int nx, int ny, double ratio, const double* x, doubl
         const double c = x[k];
         const double w = i > 0 ? x[k - 1] : 0.0, e = i < nx - 1 ? x[k + 1] : 0.0;
         const double s = j > 0 ? x[k - nx] : 0.0, no = j < ny - 1 ? x[k + nx] : 0.0;
-        y[k] = c - ratio * ((((w + e) + s) + no) - 4.0 * c);
+        y[k] = rx == ry ? c - rx * ((((w + e) + s) + no) - 4.0 * c)
+                        : c - (rx * ((w + e) - 2.0 * c) + ry * ((s + no) - 2.0 * c));
       }
 }
 
 /* loss of a batch and (grad != NULL) its parameter gradient; u (optional)
    receives the network output in normalised units */
 double orc_lp_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,
-                       int ny, double ratio, const double* r, const double* e, int emu, double* grad,
-                       double* u_out)
+                       int ny, double ratio_x, double ratio_y, const double* r, const double* e, int emu,
+                       double* grad, double* u_out)
 {
   lpnet N;
   lp_layout(&N, nl, dil, 0);
@@ -196,7 +198,7 @@ double orc_lp_lossgrad(int nl, const int* dil, const float* params, double out_s
   double* dd = malloc(sizeof(double) * tot);
   double* ad = malloc(sizeof(double) * tot);
   for (int64_t k = 0; k < tot; k++) dd[k] = u[k] - e[k] * inv[k / npix];
-  apply_A(nb, nx, ny, ratio, dd, ad);
+  apply_A(nb, nx, ny, ratio_x, ratio_y, dd, ad);
   double loss = 0.0;
   double* coef = malloc(sizeof(double) * nb);
   for (int b = 0; b < nb; b++) {
@@ -266,8 +268,8 @@ double orc_lp_lossgrad(int nl, const int* dil, const float* params, double out_s
 
 /* the symmetric network: loss, gradient (grad != NULL) and normalised output */
 double orc_lp_sym_lossgrad(int nl, const int* dil, const float* params, double out_scale, int nb, int nx,
-                           int ny, double ratio, const double* r, const double* e, int emu, double* grad,
-                           double* u_out)
+                           int ny, double ratio_x, double ratio_y, const double* r, const double* e, int emu,
+                           double* grad, double* u_out)
 {
   lpnet N;
   lp_layout(&N, nl, dil, 1);
@@ -305,7 +307,7 @@ double orc_lp_sym_lossgrad(int nl, const int* dil, const float* params, double o
   double* dd = malloc(sizeof(double) * tot);
   double* ad = malloc(sizeof(double) * tot);
   for (int64_t k = 0; k < tot; k++) dd[k] = z[k] - e[k] * inv[k / npix];
-  apply_A(nb, nx, ny, ratio, dd, ad);
+  apply_A(nb, nx, ny, ratio_x, ratio_y, dd, ad);
   double loss = 0.0;
   double* coef = malloc(sizeof(double) * nb);
   for (int b = 0; b < nb; b++) {

@@ -231,10 +231,11 @@ def lp_num_params_sym(dil):
     return f(len(d), d.ctypes.data_as(C.POINTER(C.c_int)))
 
 
-def lp_lossgrad(dil, params, out_scale, nb, nx, ny, ratio, r, e, emu=1, want_grad=True, sym=False):
+def lp_lossgrad(dil, params, out_scale, nb, nx, ny, ratio, r, e, emu=1, want_grad=True, sym=False, ratio_y=None):
     """Loss, parameter gradient and normalised network output of the learned
     preconditioner restated on the CPU (emu=1: BF16 roundings of the GPU path
-    emulated; emu=0: exact FP64; sym: the symmetric network K^T K)."""
+    emulated; emu=0: exact FP64; sym: the symmetric network K^T K), for the
+    operator I - ratio Dxx - ratio_y Dyy (ratio_y defaults to ratio)."""
     d = np.ascontiguousarray(dil, dtype=np.int32)
     p = np.ascontiguousarray(params, dtype=np.float32)
     r = np.ascontiguousarray(r, dtype=np.float64)
@@ -244,8 +245,9 @@ def lp_lossgrad(dil, params, out_scale, nb, nx, ny, ratio, r, e, emu=1, want_gra
     f = lib().orc_lp_sym_lossgrad if sym else lib().orc_lp_lossgrad
     f.restype = C.c_double
     f.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_double, C.c_int, C.c_int,
-                  C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                  C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    ry = ratio if ratio_y is None else ratio_y
     loss = f(len(d), d.ctypes.data_as(C.POINTER(C.c_int)), p.ctypes.data_as(C.POINTER(C.c_float)),
-             out_scale, nb, nx, ny, ratio, ptr(r), ptr(e), emu, ptr(g) if g is not None else None, ptr(u))
+             out_scale, nb, nx, ny, ratio, ry, ptr(r), ptr(e), emu, ptr(g) if g is not None else None, ptr(u))
     return loss, g, u

@@ -41,6 +41,7 @@ class _Stub:
     def __init__(self):
         self.evolve_targets = []
         self.train_steps = []
+        self.train_ratios = []
         stub = self
 
         class Context:
@@ -77,7 +78,7 @@ class _Stub:
             def set_output_scale(self, s):
                 pass
 
-            def train(self, nsamp, tn, ratio, R, E, steps, batch, lr, seed):
+            def train(self, nsamp, tn, ratio, R, E, steps, batch, lr, seed, ratio_y=None):
                 stub.train_steps.append(steps)
                 return np.ones(max(steps, 50))
 
@@ -109,7 +110,11 @@ class _Stub:
                 return 0
 
         self.Context, self.Problem, self.LearnedPrec, self.Integrator = Context, Problem, LearnedPrec, Integrator
-        self.training_set = lambda ctx, nsamp, tn, ratio, reltol, seed: (None, None, 0, 0)
+        def training_set(ctx, nsamp, tn, ratio, reltol, seed, ratio_y=None):
+            stub.train_ratios.append((ratio, ratio if ratio_y is None else ratio_y))
+            return None, None, 0, 0
+
+        self.training_set = training_set
         self.lib = lambda: types.SimpleNamespace(SUNLinSol_SPFGMRSetMaxRestartDim=lambda *a: 0)
 
 
@@ -130,3 +135,28 @@ def test_c5_problems_reach_steps_times_dt_after_training(capsys):
     assert all(t == 3 * 1e-3 for t in stub.evolve_targets), stub.evolve_targets
     out = capsys.readouterr().out
     assert "3 steps" in out and "1000 steps" not in out
+
+
+def test_c5_classes_train_one_network_per_anisotropy_class():
+    """--c5-classes K trains K networks on the anisotropic operators of K
+    equal-count classes of r_x/r_y, ordered by anisotropy; without it one
+    isotropic network."""
+    import numpy as np
+    bench = load_bench()
+    bench.hip_sync = lambda: None
+    probs = bench.c5_problems(6, 20260102)
+    for classes, want in ((0, 1), (3, 3)):
+        stub = _Stub()
+        args = bench.parse_args(["--workload", "c5", "--c5-n", "6", "--c5-workers", "1", "--c5-steps", "1",
+                                 "--c5-methods", "learned-sym,learned-sym-pcg", "--nsamp", "4",
+                                 "--c5-classes", str(classes)])
+        bench.c5_workload(stub, args)
+        assert len(stub.train_ratios) == want  # the PCG method shares the FGMRES method's networks
+        assert len(stub.evolve_targets) == 12
+        if classes == 0:
+            assert stub.train_ratios[0][0] == stub.train_ratios[0][1]
+            continue
+        aniso = sorted(p["kx"] / p["ky"] for p in probs)  # square grids: r_x/r_y = kx/ky
+        got = [rx / ry for rx, ry in stub.train_ratios]
+        for c in range(3):
+            assert np.isclose(got[c], np.exp(np.mean(np.log(aniso[2 * c:2 * c + 2])))), (c, got, aniso)

@@ -426,3 +426,52 @@ def test_conv_kernel_switch_rejects_other_values(pk):
     assert pk.lib().LearnedPrec_SetConvKernel(-1) == -1
     assert pk.lib().LearnedPrec_SetPairTuning(0, 2) == -1
     assert pk.lib().LearnedPrec_SetPairTuning(16, 4) == -1
+
+
+def kron_A(nx, ny, rx, ry):
+    """I - rx Dxx - ry Dyy, x the fast axis, zero boundary, from 1-D
+    second-difference matrices."""
+    import scipy.sparse as sp
+    d2 = lambda n: sp.diags([np.ones(n - 1), -2.0 * np.ones(n), np.ones(n - 1)], [-1, 0, 1])
+    return (sp.identity(nx * ny) - rx * sp.kron(sp.identity(ny), d2(nx))
+            - ry * sp.kron(d2(ny), sp.identity(nx))).tocsr()
+
+
+@pytest.mark.parametrize("rx,ry", [(60.0, 6.0), (2.0, 45.0)])
+def test_anisotropic_training_pairs_solve_their_system(pk, ctx, rx, ry):
+    nsamp, n = 10, 32
+    R, E, iters, rc = pk.training_set(ctx, nsamp, n, rx, reltol=1e-10, seed=99, ratio_y=ry)
+    assert rc == 0 and iters > 5
+    r, e = R.get().reshape(nsamp, -1), E.get().reshape(nsamp, -1)
+    A = kron_A(n, n, rx, ry)
+    for i in range(nsamp):
+        # the recursive residual meets 1e-10; the true one within rounding of it
+        assert np.linalg.norm(A @ e[i] - r[i]) <= 1.05e-10 * np.linalg.norm(r[i]), i
+    R.destroy()
+    E.destroy()
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_anisotropic_loss_and_gradient_match_oracle(pk, ctx, sym):
+    nb, nx, ny, rx, ry = 2, 40, 24, 90.0, 3.0
+    dil = SDIL if sym else DIL
+    net = pk.LearnedPrec(ctx, dil, symmetric=sym)
+    p = random_params(net.nparams, 21, 0.15 if sym else 0.2)
+    net.set_params(p)
+    scale = 1.0 / (1.0 + 2.0 * rx + 2.0 * ry)
+    net.set_output_scale(scale)
+    rng = np.random.default_rng(4)
+    e = rng.standard_normal(nb * nx * ny)
+    A = kron_A(nx, ny, rx, ry)
+    r = np.concatenate([A @ e.reshape(nb, -1)[b] for b in range(nb)])
+    R, E = pk.Vector(ctx, n=r.size), pk.Vector(ctx, n=r.size)
+    R.set(r)
+    E.set(e)
+    loss, g = net.loss_grad(nb, nx, ny, rx, R, E, ratio_y=ry)
+    lref, gref, _ = O.lp_lossgrad(dil, p, scale, nb, nx, ny, rx, r, e, emu=1, sym=sym, ratio_y=ry)
+    assert abs(loss - lref) <= 1e-3 * abs(lref)
+    assert np.linalg.norm(g - gref) <= 1e-2 * np.linalg.norm(gref)
+    # the axes matter: the transposed operator gives another loss
+    lt, _ = net.loss_grad(nb, nx, ny, ry, R, E, grad=False, ratio_y=rx)
+    assert abs(lt - loss) > 1e-2 * abs(loss)
+    net.release()

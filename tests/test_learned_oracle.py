@@ -188,3 +188,52 @@ def test_sym_oracle_bf16_emulation_is_a_small_perturbation():
     rel = np.abs(u1 - u0).max() / np.abs(u0).max()
     assert 0.0 < rel < 0.05
     assert np.linalg.norm(g1 - g0) / np.linalg.norm(g0) < 0.05
+
+
+def kron_A(nx, ny, rx, ry):
+    """I - rx Dxx - ry Dyy on an nx*ny grid (x fastest), zero boundary, built
+    from 1-D second-difference matrices (independently of the oracle's
+    stencil loop)."""
+    import scipy.sparse as sp
+    d2 = lambda n: sp.diags([np.ones(n - 1), -2.0 * np.ones(n), np.ones(n - 1)], [-1, 0, 1])
+    return (sp.identity(nx * ny) - rx * sp.kron(sp.identity(ny), d2(nx))
+            - ry * sp.kron(d2(ny), sp.identity(nx))).tocsr()
+
+
+@pytest.mark.parametrize("sym", [False, True])
+@pytest.mark.parametrize("rx,ry", [(3.0, 3.0), (40.0, 0.7), (0.25, 9.0)])
+def test_oracle_anisotropic_operator_orientation(sym, rx, ry):
+    """With the output scale 0 the network contributes nothing (d = -e/s), so
+    the loss is e^T A e / (e.r) per image: the operator alone, checked
+    against the Kronecker construction with x the fast (contiguous) axis."""
+    nb, nx, ny = 2, 9, 5
+    dil = (1, 2, 1)
+    rng = np.random.default_rng(7)
+    n = O.lp_num_params_sym(dil) if sym else O.lp_num_params(dil)
+    params = (0.3 * rng.standard_normal(n)).astype(np.float32)
+    r = rng.standard_normal(nb * nx * ny)
+    e = rng.standard_normal(nb * nx * ny) + 0.5 * r
+    loss, _, _ = O.lp_lossgrad(dil, params, 0.0, nb, nx, ny, rx, r, e, emu=0, want_grad=False, sym=sym,
+                               ratio_y=ry)
+    A = kron_A(nx, ny, rx, ry)
+    E, R = e.reshape(nb, -1), r.reshape(nb, -1)
+    want = np.mean([E[b] @ (A @ E[b]) / (E[b] @ R[b]) for b in range(nb)])
+    assert abs(loss - want) <= 1e-12 * abs(want), (loss, want)
+
+
+def test_oracle_anisotropic_loss_is_affine_and_meets_the_isotropic_one():
+    """For a fixed network and data the loss is affine in (rx, ry) (A is), and
+    the anisotropic branch tends to the isotropic one on the diagonal."""
+    dil = (1, 2, 1)
+    params, r, e = problem(5, dil)
+    L = lambda rx, ry: O.lp_lossgrad(dil, params, 0.6, 2, 7, 6, rx, r, e, emu=0, want_grad=False,
+                                     ratio_y=ry)[0]
+    a = 4.0
+    assert abs(L(a, a * (1 + 1e-12)) - L(a, a)) <= 1e-9 * abs(L(a, a))
+    for (x0, y0) in ((1.0, 3.0), (7.0, 0.5)):
+        l00, l10, l20, l01, l02 = L(x0, y0), L(2 * x0, y0), L(3 * x0, y0), L(x0, 2 * y0), L(x0, 3 * y0)
+        assert abs((l20 - l10) - (l10 - l00)) <= 1e-9 * abs(l10)
+        assert abs((l02 - l01) - (l01 - l00)) <= 1e-9 * abs(l01)
+        # the x slope and the y slope add up to the isotropic slope
+        sx, sy = (l10 - l00) / x0, (l01 - l00) / y0
+        assert abs(L(0.0, 0.0) + 2.0 * (sx + sy) - L(2.0, 2.0)) <= 1e-9 * abs(L(2.0, 2.0))
