@@ -488,7 +488,9 @@ def c5_workload(pk, args):
     at the batch's median gamma*dt*k/dx^2, outside the timed region; its
     output scaled per problem by the inverse Jacobi diagonal) and PCG with the
     multigrid V-cycle.  Time-to-solution is the wall time of the whole batch,
-    problem setup included; Jacobi-PCG is also run by a single worker."""
+    problem setup included (the L2 check against the exact solution and the
+    teardown follow the timed region; the workers' contexts and the trained
+    networks precede it); Jacobi-PCG is also run by a single worker."""
     import threading
     import numpy as np
 
@@ -507,14 +509,22 @@ def c5_workload(pk, args):
     shapes = {"learned": ((1, 1, 1, 1), False), "learned-sym": (sdil, True), "learned-sym-pcg": (sdil, True)}
     # training operators: with --c5-classes K the problems are split by their
     # anisotropy r_x/r_y (r = gamma*dt*k/dx^2 along each axis) into K classes of
-    # equal count and one network is trained per class, on the class's
-    # geometric-mean (r_x, r_y); otherwise one isotropic network for all
+    # equal count (each split again by r_x r_y into --c5-mag-classes classes)
+    # and one network is trained per class, on the class's geometric-mean
+    # (r_x, r_y); otherwise one isotropic network for all
     rxy = [(gamma_a * dt * p["kx"] * (p["nx"] + 1) ** 2, gamma_a * dt * p["ky"] * (p["ny"] + 1) ** 2)
            for p in probs]
     if args.c5_classes > 0:
         order = sorted(range(len(probs)), key=lambda k: rxy[k][0] / rxy[k][1])
         K = min(args.c5_classes, len(probs))
         groups = [order[c * len(order) // K:(c + 1) * len(order) // K] for c in range(K)]
+        if args.c5_mag_classes > 1:  # each anisotropy class split by r_x r_y into equal-count classes
+            sub = []
+            for g in groups:
+                g = sorted(g, key=lambda k: rxy[k][0] * rxy[k][1])
+                m = min(args.c5_mag_classes, len(g))
+                sub += [g[c * len(g) // m:(c + 1) * len(g) // m] for c in range(m)]
+            groups = sub
         cls_of = {k: c for c, g in enumerate(groups) for k in g}
         targets = [tuple(float(np.exp(np.mean([np.log(rxy[k][a]) for k in g]))) for a in (0, 1)) for g in groups]
     else:
@@ -523,8 +533,20 @@ def c5_workload(pk, args):
         targets = [(ratio, ratio)]
     shared = {}
     train = {}
+    saved = {}
+    if args.c5_nets and os.path.exists(args.c5_nets):
+        with np.load(args.c5_nets) as z:
+            saved = {k: z[k] for k in z.files}
     for kind in methods:
         if kind not in shapes or (kind == "learned-sym-pcg" and "learned-sym" in shared):
+            continue
+        key = "sym" if shapes[kind][1] else "relu"
+        if key + "_targets" in saved and np.allclose(saved[key + "_targets"], targets, rtol=1e-12, atol=0):
+            shared[kind] = list(saved[key + "_params"])
+            train[kind] = [{"loaded_from": args.c5_nets}]
+            log(f"{kind}: {len(targets)} trained network(s) loaded from {args.c5_nets}")
+            if kind in ("learned-sym", "learned-sym-pcg"):
+                shared["learned-sym"] = shared["learned-sym-pcg"] = shared[kind]
             continue
         dil, sym = shapes[kind]
         tn = args.sym_train_n if sym else args.train_n
@@ -553,9 +575,14 @@ def c5_workload(pk, args):
             ctx.close()
         if kind in ("learned-sym", "learned-sym-pcg"):  # one symmetric network for both
             shared["learned-sym"] = shared["learned-sym-pcg"] = shared[kind]
+        if args.c5_nets:
+            saved[key + "_targets"] = np.array(targets)
+            saved[key + "_params"] = np.array(shared[kind])
+            np.savez(args.c5_nets, **saved)
 
     def solve_one(ctx, net, k, kind, t_end):
         p = probs[k]
+        t0 = time.perf_counter()
         G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=p["nx"], ny=p["ny"], kx=p["kx"], ky=p["ky"])
         ls, prec, maxl = "cg", True, 20000
         if kind == "mg":
@@ -575,8 +602,16 @@ def c5_workload(pk, args):
         I.ARKodeSetFixedStep(dt)
         I.ARKodeSetLSRelativeTolerance(1e-8)
         I.ARKodeSetMaxNumSteps(1000000)
+        t1 = time.perf_counter()
         flag, t = I.evolve(t_end)
         assert flag == 0 and t == t_end, (kind, k, flag, t, t_end)
+        # host seconds of this worker in the problem's setup and integration
+        return G, I, t, (t1 - t0, time.perf_counter() - t1)
+
+    def check_one(G, I, t, phase):
+        """After the timed region: statistics, L2 error against the exact
+        solution, teardown."""
+        t0 = time.perf_counter()
         st = I.stats()
         y = I.y.get()
         ex = G.exact(t).get()
@@ -584,6 +619,7 @@ def c5_workload(pk, args):
                    l2_err=float(np.sqrt(np.mean((y - ex) ** 2))))
         I.free()
         G.destroy()
+        rec["phase_s"] = phase + (time.perf_counter() - t0,)
         return rec
 
     def run(kind, workers):
@@ -592,42 +628,61 @@ def c5_workload(pk, args):
         recs = [None] * len(probs)
         errors = []
 
+        # the timed region runs from every worker's start to the last solve's
+        # end: each worker's context and trained networks exist before it
+        # (the networks are trained before any batch), the L2 check and the
+        # teardown follow it
+        start = threading.Barrier(workers + 1)
+        solved = threading.Barrier(workers + 1)
+
         def worker():
-            ctx = pk.Context()
-            nets = []
-            if kind in shapes:
-                for params in shared[kind]:
-                    nets.append(pk.LearnedPrec(ctx, shapes[kind][0], symmetric=shapes[kind][1]))
-                    nets[-1].set_params(params)
+            ctx, nets, live = None, [], []
+            try:
+                ctx = pk.Context()
+                if kind in shapes:
+                    for params in shared[kind]:
+                        nets.append(pk.LearnedPrec(ctx, shapes[kind][0], symmetric=shapes[kind][1]))
+                        nets[-1].set_params(params)
+            except Exception as e:  # reported after the join
+                with lock:
+                    errors.append(repr(e))
+            start.wait()
             try:
                 while True:
                     with lock:
                         if not todo or errors:
                             break
                         k = todo.pop(0)
-                    recs[k] = solve_one(ctx, nets[cls_of[k]] if nets else None, k, kind, t_end)
+                    live.append((k, solve_one(ctx, nets[cls_of[k]] if nets else None, k, kind, t_end)))
                     with lock:
-                        done = sum(r is not None for r in recs)
+                        done = len(probs) - len(todo)
                     if done % 8 == 0 or workers == 1:
-                        log(f"  {kind} x{workers}: {done}/{len(probs)} problems")
-            except Exception as e:  # reported after the join
+                        log(f"  {kind} x{workers}: {done}/{len(probs)} problems started")
+            except Exception as e:
+                with lock:
+                    errors.append(repr(e))
+            solved.wait()
+            try:
+                for k, res_k in live:
+                    recs[k] = check_one(*res_k)
+            except Exception as e:
                 with lock:
                     errors.append(repr(e))
             finally:
-                ctx.close()
+                if ctx is not None:
+                    ctx.close()
 
+        th = [threading.Thread(target=worker) for _ in range(workers)]
+        for x in th:
+            x.start()
         hip_sync()
+        start.wait()
         t0 = time.perf_counter()
-        if workers == 1:
-            worker()  # one worker: the calling thread
-        else:
-            th = [threading.Thread(target=worker) for _ in range(workers)]
-            for x in th:
-                x.start()
-            for x in th:
-                x.join()
+        solved.wait()
         hip_sync()
         el = time.perf_counter() - t0
+        for x in th:
+            x.join()
         assert not errors, errors
         errs = np.array([r["l2_err"] for r in recs])
         its = np.array([r["lin_iters"] for r in recs])
@@ -635,7 +690,10 @@ def c5_workload(pk, args):
                    krylov_iters_total=int(its.sum()), krylov_iters_max=int(its.max()),
                    fallbacks=int(sum(r["fallbacks"] for r in recs)),
                    l2_err_max=float(errs.max()), l2_err_median=float(np.median(errs)),
-                   krylov_iters_per_problem=[int(x) for x in its])
+                   krylov_iters_per_problem=[int(x) for x in its],
+                   worker_s_setup_evolve_check=[round(float(sum(r["phase_s"][i] for r in recs)), 3)
+                                                for i in range(3)],
+                   timed="problem setup and integration of every problem; the L2 check and teardown after")
         log(f"{kind} x{workers}: { {k: v for k, v in out.items() if k != 'krylov_iters_per_problem'} }")
         return out, errs
 
@@ -650,7 +708,9 @@ def c5_workload(pk, args):
            "config": {"workload": f"{len(probs)} mms-dirichlet problems, side U[256,1024], kx,ky logU[0.1,10] "
                                   f"(seed 20260102), {cells} cells, DIRK3 fixed dt=1e-3, {steps} steps, "
                                   "LS reltol 1e-8, analytic J v",
-                      "learned_networks": (f"{len(targets)}, one per anisotropy class" if args.c5_classes > 0
+                      "learned_networks": (f"{len(targets)}, one per class of anisotropy"
+                                           + (" and magnitude" if args.c5_mag_classes > 1 else "")
+                                           if args.c5_classes > 0
                                            else "1, isotropic, shared by all problems"),
                       "concurrency": f"{args.c5_workers} host threads, one SUNContext/stream each"},
            "results": res, "training": train,
@@ -763,6 +823,11 @@ def parse_args(argv=None):
     ap.add_argument("--c5-methods", default="jacobi,mg")
     ap.add_argument("--c5-train-ratio", type=float, default=0.0,
                     help="c5: gamma*dt*k/dx^2 the shared network is trained at (0: the batch median)")
+    ap.add_argument("--c5-nets", default="",
+                    help="c5: .npz file of trained networks: loaded when it holds networks for the same training "
+                         "operators, else written after training")
+    ap.add_argument("--c5-mag-classes", type=int, default=1,
+                    help="c5: with --c5-classes, split each anisotropy class into this many classes of r_x r_y")
     ap.add_argument("--c5-classes", type=int, default=0,
                     help="c5: learned methods train one network per class of problem anisotropy (this many "
                          "classes of equal count); 0: one isotropic network shared by every problem")
@@ -771,6 +836,9 @@ def parse_args(argv=None):
                     help="learned network's convolutions: 1 LDS row chains, 2 LDS row chains and two layers per "
                          "kernel in the symmetric network's application, 0 the direct kernel")
     ap.add_argument("--pair-rows", type=int, default=16, help="chain rows per workgroup of the two-layer kernel")
+    ap.add_argument("--chain-tasks", type=int, default=1024,
+                    help="row-chain conv kernels: shorter row segments per workgroup on images that would launch "
+                         "fewer workgroups than this (1: 16 rows everywhere)")
     ap.add_argument("--pair-ahead", type=int, default=2, help="input rows in flight per workgroup, two-layer kernel")
     ap.add_argument("--kpass", type=int, default=0, help="batch: stages per pass (0: the library default)")
     ap.add_argument("--tile-rows", type=int, default=0, help="batch: rows per wavefront tile (0: autotuned)")
@@ -792,6 +860,8 @@ def main():
         assert pk.lib().LearnedPrec_SetConvKernel(args.conv_kernel) == 0
     if (args.pair_rows, args.pair_ahead) != (16, 2):
         assert pk.lib().LearnedPrec_SetPairTuning(args.pair_rows, args.pair_ahead) == 0
+    if args.chain_tasks != 1024:
+        assert pk.lib().LearnedPrec_SetChainTasks(args.chain_tasks) == 0
     if args.kernels_only:
         pin = tuple(int(x) for x in args.pin_pass.split(",")) if args.pin_pass else None
         kernels_only(pk, args.kernels_only, args.sweep, pin)

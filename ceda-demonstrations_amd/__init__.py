@@ -153,6 +153,7 @@ _SIGS = {
     "LearnedPrec_LossGradXY": (i32, [vp, i32, i32, i32, d, d, vp, vp, dp, C.POINTER(C.c_float)]),
     "LearnedPrec_MakeTrainingSetXY": (i32, [i32, i32, d, d, d, C.c_ulong, vp, vp, C.POINTER(i32), vp]),
     "LearnedPrec_TrainXY": (i32, [vp, i32, i32, d, d, vp, vp, i32, i32, d, C.c_ulong, dp]),
+    "LearnedPrec_SetChainTasks": (i32, [i32]),
     "DiffusionProblem_SetLearnedPrec": (i32, [vp, vp]),
     "ARKodeSetLSRelativeTolerance": (i32, [vp, d]),
     "SUNLinSol_PCG": (vp, [vp, i32, i32, vp]),

@@ -91,6 +91,7 @@ struct ConvArgs {
   double out_mul;             // COUT = 1: constant output multiplier
   const unsigned short* mask; // NHWC16: outputs zeroed where the mask is not > 0
   int addto;                  // COUT = 1: out += result
+  int rows;                   // row-chain kernels: output rows per workgroup (set by conv)
 };
 
 template <int CIN, int COUT, bool RELU>
@@ -233,31 +234,41 @@ __global__ __launch_bounds__(kBlock) void k_conv(const ConvArgs a)
 // ------------------------------------- 16-channel layers, LDS row chains
 // A layer of dilation d couples row y only with rows y - d and y + d, so the
 // rows of one residue class mod d form a chain that can be swept on its own
-// (for d = 1 the chain is the whole column).  A workgroup takes kChainR
-// consecutive output rows of one chain in one column strip of kChainW
+// (for d = 1 the chain is the whole column).  A workgroup takes R (kChainR,
+// fewer on small images) consecutive output rows of one chain in one column strip of kChainW
 // pixels and keeps a ring of four input rows of the strip (plus d halo
 // pixels on each side) in LDS: three in use (y - d, y, y + d) and the next
 // one, loaded into registers while the current row is computed and stored
 // into the ring afterwards.  Each input row is read from memory about once
 // per strip (a (kChainW + 2d) / kChainW horizontal halo and two halo rows
-// per kChainR rows), against three times plus the taps' reuse through the
+// per R rows), against three times plus the taps' reuse through the
 // caches for k_conv.  Ring layout: [slot][half h][pixel] of 16-byte chunks
 // (channels 8h..8h+7 of one pixel), so the 16 pixels of a tap-pair operand
 // are 256 contiguous bytes (no bank conflicts).  The operands, the MFMA
 // sequence and the stores are those of k_conv: the same results bit for bit.
 constexpr int kChainW = kWavesPB * kConvU * 16;  // pixels per column strip
-constexpr int kChainR = 16;                      // output rows per workgroup
+constexpr int kChainR = 16;                      // output rows per workgroup (at most)
 constexpr int kChainMaxD = 64;                   // ring rows of at most kChainW + 128 pixels
 constexpr int kChainLoads = (2 * (kChainW + 2 * kChainMaxD) + kBlock - 1) / kBlock;  // chunks per thread
 
 // workgroups of one chain launch: per image, strips x min(d, ny) chains x
-// segments of kChainR rows (the longest chain has ceil(ny / d) rows)
-inline int64_t chain_tasks(int nb, int nx, int ny, int d)
+// segments of R rows (the longest chain has ceil(ny / d) rows)
+inline int64_t chain_tasks(int nb, int nx, int ny, int d, int R)
 {
   const int64_t strips = (nx + kChainW - 1) / kChainW;
   const int64_t chains = std::min(d, ny);
-  const int64_t segs = ((ny + d - 1) / d + kChainR - 1) / kChainR;
+  const int64_t segs = ((ny + d - 1) / d + R - 1) / R;
   return (int64_t)nb * strips * chains * segs;
+}
+// rows per workgroup: kChainR, halved (down to 2) while the launch has fewer
+// than g_chain_tasks workgroups, so that small images still spread over the
+// chip (each halving adds two halo rows per R rows of the strip's reads)
+int g_chain_tasks = 1024;
+inline int chain_rows(int nb, int nx, int ny, int d)
+{
+  int R = kChainR;
+  while (R > 2 && chain_tasks(nb, nx, ny, d, R) < g_chain_tasks) R /= 2;
+  return R;
 }
 
 template <int COUT, bool RELU>
@@ -271,7 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_conv_chain(const ConvArgs a)
   // task: strip fastest, then chain, then segment, then image
   const int strips = (nx + kChainW - 1) / kChainW;
   const int chains = min(d, ny);
-  const int segs = ((ny + d - 1) / d + kChainR - 1) / kChainR;
+  const int R = a.rows;
+  const int segs = ((ny + d - 1) / d + R - 1) / R;
   int64_t t = blockIdx.x;
   const int strip = (int)(t % strips);
   t /= strips;
@@ -281,7 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_conv_chain(const ConvArgs a)
   const int bimg = (int)(t / segs);
   const int64_t img = (int64_t)bimg * ny;
   const int x0 = strip * kChainW;
-  const int y0 = c + seg * kChainR * d;
+  const int y0 = c + seg * R * d;
   if (y0 >= ny) return;  // workgroup-uniform: an empty tail segment of a short chain
 
   bf16x8 A8[5];
@@ -345,10 +357,10 @@ __global__ __launch_bounds__(kBlock) void k_conv_chain(const ConvArgs a)
     const int x = x0 + (wave * kConvU + u) * 16 + n;
     oofs[u] = COUT == kC ? (int)act_off(0, bpr, x, g) : x;
   }
-  for (int m = 0; m < kChainR; m++) {
+  for (int m = 0; m < R; m++) {
     const int y = y0 + m * d;
     if (y >= ny) break;  // workgroup-uniform
-    const bool more = m + 1 < kChainR && y + d < ny;
+    const bool more = m + 1 < R && y + d < ny;
     if (more) fetch(y + 2 * d, v);
     const int rb[3] = {((m + 3) & 3) * 2 * span, (m & 3) * 2 * span, ((m + 1) & 3) * 2 * span};
     int off[5];
@@ -412,7 +424,8 @@ __global__ __launch_bounds__(kBlock) void k_conv1_chain(const ConvArgs a)
   const int bpr = (nx + 15) >> 4;
   const int strips = (nx + kChainW - 1) / kChainW;
   const int chains = min(d, ny);
-  const int segs = ((ny + d - 1) / d + kChainR - 1) / kChainR;
+  const int R = a.rows;
+  const int segs = ((ny + d - 1) / d + R - 1) / R;
   int64_t t = blockIdx.x;
   const int strip = (int)(t % strips);
   t /= strips;
@@ -422,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_conv1_chain(const ConvArgs a)
   const int bimg = (int)(t / segs);
   const int64_t img = (int64_t)bimg * ny;
   const int x0 = strip * kChainW;
-  const int y0 = c + seg * kChainR * d;
+  const int y0 = c + seg * R * d;
   if (y0 >= ny) return;  // workgroup-uniform
 
   const bfx4 A1 = reinterpret_cast<const bfx4*>(a.apack)[lane];
@@ -472,10 +485,10 @@ __global__ __launch_bounds__(kBlock) void k_conv1_chain(const ConvArgs a)
   int oofs[kConvU];
 #pragma unroll
   for (int u = 0; u < kConvU; u++) oofs[u] = (int)act_off(0, bpr, x0 + (wave * kConvU + u) * 16 + n, g);
-  for (int m = 0; m < kChainR; m++) {
+  for (int m = 0; m < R; m++) {
     const int y = y0 + m * d;
     if (y >= ny) break;  // workgroup-uniform
-    const bool more = m + 1 < kChainR && y + d < ny;
+    const bool more = m + 1 < R && y + d < ny;
     if (more) fetch(y + 2 * d, v);
     const int rb[3] = {((m + 3) & 3) * span, (m & 3) * span, ((m + 1) & 3) * span};
     int off[4];
@@ -1389,7 +1402,9 @@ void lp_pack(LearnedPrec* L)
 
 constexpr int kMaxParts = 512;  // workgroups per image of the per-image reductions
 static_assert(kNAMaxParts >= kMaxParts, "the fixed workspace holds every image's RMS partials");
-int rms_parts(int64_t npix) { return (int)std::min<int64_t>(kMaxParts, (npix + 8191) / 8192); }
+// one workgroup per 2048 pixels (8 per thread), at most kMaxParts: enough
+// workgroups in flight that the sum of a small image is not latency-bound
+int rms_parts(int64_t npix) { return (int)std::min<int64_t>(kMaxParts, (npix + 2047) / 2048); }
 
 // activation buffers hold nb*ny rows of whole 16-pixel blocks
 int64_t act_elems(int nb, int nx, int ny) { return (int64_t)nb * ny * ((nx + 15) / 16) * 16 * kC; }
@@ -1465,15 +1480,19 @@ template <int CIN, int COUT, bool RELU>
 void conv(LearnedPrec* L, const ConvArgs& a)
 {
   if (CIN == kC && g_conv_chain && a.dil >= 1 && a.dil <= kChainMaxD) {
-    const int64_t tasks = chain_tasks(a.nb, a.nx, a.ny, a.dil);
+    ConvArgs b = a;
+    b.rows = chain_rows(a.nb, a.nx, a.ny, a.dil);
+    const int64_t tasks = chain_tasks(a.nb, a.nx, a.ny, a.dil, b.rows);
     const size_t lds = sizeof(uint4) * 4 * 2 * (size_t)(kChainW + 2 * a.dil);
-    hipLaunchKernelGGL((k_conv_chain<COUT, RELU>), dim3((unsigned)tasks), dim3(kBlock), lds, L->ex->stream, a);
+    hipLaunchKernelGGL((k_conv_chain<COUT, RELU>), dim3((unsigned)tasks), dim3(kBlock), lds, L->ex->stream, b);
     return;
   }
   if (CIN == 1 && COUT == kC && g_conv_chain && a.dil >= 1 && a.dil <= kChainMaxD) {
-    const int64_t tasks = chain_tasks(a.nb, a.nx, a.ny, a.dil);
+    ConvArgs b = a;
+    b.rows = chain_rows(a.nb, a.nx, a.ny, a.dil);
+    const int64_t tasks = chain_tasks(a.nb, a.nx, a.ny, a.dil, b.rows);
     const size_t lds = sizeof(double) * 4 * (size_t)(kChainW + 2 * a.dil);
-    hipLaunchKernelGGL((k_conv1_chain<RELU>), dim3((unsigned)tasks), dim3(kBlock), lds, L->ex->stream, a);
+    hipLaunchKernelGGL((k_conv1_chain<RELU>), dim3((unsigned)tasks), dim3(kBlock), lds, L->ex->stream, b);
     return;
   }
   // one row segment (kWavesPB * kConvU blocks of 16 pixels) per workgroup
@@ -2005,6 +2024,13 @@ int LearnedPrec_SetPairTuning(int rows, int ahead)
   if (rows < 1 || rows > 1024 || ahead < 1 || ahead > 3) return -1;
   g_pair_rows = rows;
   g_pair_ahead = ahead;
+  return 0;
+}
+
+int LearnedPrec_SetChainTasks(int min_tasks)
+{
+  if (min_tasks < 1 || min_tasks > (1 << 20)) return -1;
+  g_chain_tasks = min_tasks;
   return 0;
 }
 

@@ -53,15 +53,24 @@ inline hipError_t host_free(void* p)
   std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
   return hipHostFree(p);
 }
+// Copies and sets on the null stream, complete on return.  The contexts'
+// streams are non-blocking, so nothing orders their kernels after null-stream
+// work, and hipMemset, like a hipMemcpy from pageable host memory to the
+// device, may return before the device memory is written: without the wait
+// a kernel on a context's stream can read the destination first (a network
+// created while other threads drive the GPU could have its zero fill land
+// after its weights were packed).
 inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
 {
   std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
-  return hipMemcpy(dst, src, bytes, kind);
+  const hipError_t e = hipMemcpy(dst, src, bytes, kind);
+  return e != hipSuccess ? e : hipStreamSynchronize(nullptr);
 }
 inline hipError_t set_sync(void* dst, int value, size_t bytes)
 {
   std::lock_guard<std::recursive_mutex> g(device_sync_mutex());
-  return hipMemset(dst, value, bytes);
+  const hipError_t e = hipMemset(dst, value, bytes);
+  return e != hipSuccess ? e : hipStreamSynchronize(nullptr);
 }
 inline hipError_t graph_exec_destroy(hipGraphExec_t ge)
 {

@@ -80,6 +80,11 @@ int LearnedPrec_SetConvKernel(int kind);
    1..1024) and input rows in flight per workgroup (default 2; 1..3); -1 for
    other values.  The results do not depend on them. */
 int LearnedPrec_SetPairTuning(int rows, int ahead);
+/* the row-chain kernels give each workgroup 16 consecutive rows of a chain,
+   or 8, 4 or 2 on images where 16 would launch fewer than min_tasks
+   workgroups (default 1024; 1 keeps 16 everywhere; 1..2^20, -1 otherwise).
+   The results do not depend on it. */
+int LearnedPrec_SetChainTasks(int min_tasks);
 
 /* out = N(in) for nb images of nx*ny pixels (image-major, x fastest), FP64
    device arrays; addto = 1 adds N(in) to out */

@@ -160,3 +160,44 @@ def test_c5_classes_train_one_network_per_anisotropy_class():
         got = [rx / ry for rx, ry in stub.train_ratios]
         for c in range(3):
             assert np.isclose(got[c], np.exp(np.mean(np.log(aniso[2 * c:2 * c + 2])))), (c, got, aniso)
+
+
+def test_c5_nets_file_is_written_then_reused(tmp_path):
+    """--c5-nets: the first run trains and writes the networks, a second run
+    with the same training operators loads them instead of training; other
+    operators train again."""
+    bench = load_bench()
+    bench.hip_sync = lambda: None
+    path = str(tmp_path / "nets.npz")
+    argv = ["--workload", "c5", "--c5-n", "4", "--c5-workers", "1", "--c5-steps", "1",
+            "--c5-methods", "learned-sym-pcg", "--nsamp", "4", "--c5-nets", path]
+    counts = []
+    for extra in ([], [], ["--c5-classes", "2"]):
+        stub = _Stub()
+        bench.c5_workload(stub, bench.parse_args(argv + extra))
+        counts.append(len(stub.train_ratios))
+    assert counts == [1, 0, 2]
+
+
+def test_c5_magnitude_classes_split_each_anisotropy_class():
+    """--c5-mag-classes M splits each of the K anisotropy classes into M
+    classes by r_x r_y: K*M networks, each trained at its class's
+    geometric-mean operator."""
+    import numpy as np
+    bench = load_bench()
+    bench.hip_sync = lambda: None
+    stub = _Stub()
+    args = bench.parse_args(["--workload", "c5", "--c5-n", "8", "--c5-workers", "1", "--c5-steps", "1",
+                             "--c5-methods", "learned-sym-pcg", "--nsamp", "4", "--c5-classes", "2",
+                             "--c5-mag-classes", "2"])
+    bench.c5_workload(stub, args)
+    assert len(stub.train_ratios) == 4
+    probs = bench.c5_problems(8, 20260102)
+    g = 1767732205903.0 / 4055673282236.0 * 1e-3
+    rxy = [(g * p["kx"] * (p["nx"] + 1) ** 2, g * p["ky"] * (p["ny"] + 1) ** 2) for p in probs]
+    order = sorted(range(8), key=lambda k: rxy[k][0] / rxy[k][1])
+    for c, half in enumerate((order[:4], order[4:])):
+        half = sorted(half, key=lambda k: rxy[k][0] * rxy[k][1])
+        for m, grp in enumerate((half[:2], half[2:])):
+            want = [np.exp(np.mean([np.log(rxy[k][a]) for k in grp])) for a in (0, 1)]
+            assert np.allclose(stub.train_ratios[2 * c + m], want, rtol=1e-12)

@@ -421,7 +421,40 @@ def test_pair_kernel_tuning_does_not_change_the_result(pk, ctx, rows, ahead):
     net.release()
 
 
+@pytest.mark.parametrize("sym", [False, True])
+def test_chain_rows_per_workgroup_do_not_change_the_result(pk, ctx, sym):
+    """Small images get shorter row segments per workgroup (more
+    workgroups); the segment length splits the work only: 16 rows everywhere
+    (min tasks 1), the default, and 2 rows everywhere (min tasks 2^20) give
+    the same bits, forward and in the training gradient."""
+    dil = BENCH_DIL if sym else (1, 2, 5, 1)
+    net = pk.LearnedPrec(ctx, dil, symmetric=sym)
+    net.set_params(random_params(net.nparams, 8, 0.15))
+    nb, nx, ny = 2, 300, 141
+    rng = np.random.default_rng(3)
+    e = rng.standard_normal(nb * nx * ny)
+    r = apply_A(e, nb, nx, ny, 30.0)
+    R, E, Z = pk.Vector(ctx, n=r.size), pk.Vector(ctx, n=r.size), pk.Vector(ctx, n=r.size)
+    R.set(r)
+    E.set(e)
+    out = []
+    try:
+        for m in (1, 1024, 1 << 20):
+            assert pk.lib().LearnedPrec_SetChainTasks(m) == 0
+            assert net.apply(nb, nx, ny, R, Z) == 0
+            loss, g = net.loss_grad(nb, nx, ny, 30.0, R, E)
+            out.append((Z.get(), loss, g))
+    finally:
+        assert pk.lib().LearnedPrec_SetChainTasks(1024) == 0
+    for k in (1, 2):
+        assert np.array_equal(out[0][0].view(np.uint64), out[k][0].view(np.uint64)), k
+        assert out[0][1] == out[k][1] and np.array_equal(out[0][2], out[k][2]), k
+    net.release()
+
+
 def test_conv_kernel_switch_rejects_other_values(pk):
+    assert pk.lib().LearnedPrec_SetChainTasks(0) == -1
+    assert pk.lib().LearnedPrec_SetChainTasks((1 << 20) + 1) == -1
     assert pk.lib().LearnedPrec_SetConvKernel(3) == -1
     assert pk.lib().LearnedPrec_SetConvKernel(-1) == -1
     assert pk.lib().LearnedPrec_SetPairTuning(0, 2) == -1
