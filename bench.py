@@ -627,6 +627,7 @@ def c5_workload(pk, args):
         lock = threading.Lock()
         recs = [None] * len(probs)
         errors = []
+        solved_count = [0]
 
         # the timed region runs from every worker's start to the last solve's
         # end: each worker's context and trained networks exist before it
@@ -655,9 +656,10 @@ def c5_workload(pk, args):
                         k = todo.pop(0)
                     live.append((k, solve_one(ctx, nets[cls_of[k]] if nets else None, k, kind, t_end)))
                     with lock:
-                        done = len(probs) - len(todo)
+                        solved_count[0] += 1
+                        done = solved_count[0]
                     if done % 8 == 0 or workers == 1:
-                        log(f"  {kind} x{workers}: {done}/{len(probs)} problems started")
+                        log(f"  {kind} x{workers}: {done}/{len(probs)} problems solved")
             except Exception as e:
                 with lock:
                     errors.append(repr(e))

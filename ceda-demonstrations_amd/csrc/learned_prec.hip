@@ -1228,9 +1228,10 @@ struct CGState {
   double rz, bb, beta;  // r.z, b.b, the next direction's beta
   int done, iters;
 };
-// x = 0, r = b, z = b / dg, and the direction before the first one zero (so
-// that the first direction r / dg + beta * 0 is z itself)
-__global__ __launch_bounds__(kBlock) void k_cg_init(int n, double dg, const double* __restrict__ bvec,
+// x = 0, r = b, z = b * idg (idg = 1 / dg, the constant inverse Jacobi
+// diagonal), and the direction before the first one zero (so that the first
+// direction r * idg + beta * 0 is z itself)
+__global__ __launch_bounds__(kBlock) void k_cg_init(int n, double idg, const double* __restrict__ bvec,
                                                     double* __restrict__ x, double* __restrict__ r,
                                                     double* __restrict__ p, CGState* __restrict__ st)
 {
@@ -1242,23 +1243,30 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int n, double dg, const doub
     x[o + k] = 0.0;
     r[o + k] = b;
     p[o + k] = 0.0;
-    rz += b * (b / dg);
+    rz += b * (b * idg);
     bb += b * b;
   }
   block_sum2(rz, bb, sh);
   if (threadIdx.x == 0) st[blockIdx.x] = CGState{rz, bb, 0.0, bb == 0.0 ? 1 : 0, 0};
 }
 // One Jacobi-PCG iteration on (I - rx Dxx - ry Dyy) of one pair per
-// workgroup, in two sweeps over the grid (constant diagonal dg, so z = r / dg):
+// workgroup, in two sweeps over the grid (constant diagonal, z = r * idg):
 //   sweep 1  p = z + beta p_old written to the other direction buffer, and
 //            p.Ap with Ap formed from the new directions of the 5 points
 //            (recomputed from r and p_old, so Ap is never stored);
 //   sweep 2  Ap again from the stored p, x += alpha p, r -= alpha Ap, and
 //            r.r and r.z.
-// The iterates are those of the textbook loop (ap = A p; alpha; x, r; beta;
-// p = z + beta p): every value is formed by the same expression from the same
-// operands.  Eight vector sweeps of HBM per iteration instead of eleven.
-__global__ __launch_bounds__(kBlock) void k_cg_iter(int n, double rx, double ry, double dg, double tol2,
+// Eight vector sweeps of HBM per iteration instead of the textbook loop's
+// eleven (ap = A p; alpha; x, r; beta; p = z + beta p).  PAIRS (even n): a
+// thread takes two cells of a row with 16-byte accesses, and the cells west
+// and east of the pair with 8-byte ones.
+__device__ inline double2 cg_ld2(const double* __restrict__ a, int q) { return *reinterpret_cast<const double2*>(a + q); }
+__device__ inline void cg_st2(double* __restrict__ a, int q, double u, double v)
+{
+  *reinterpret_cast<double2*>(a + q) = double2{u, v};
+}
+template <bool PAIRS>
+__global__ __launch_bounds__(kBlock) void k_cg_iter(int n, double rx, double ry, double idg, double tol2,
                                                     double* __restrict__ x, double* __restrict__ r,
                                                     double* __restrict__ pa, double* __restrict__ pb,
                                                     CGState* __restrict__ st)
@@ -1270,39 +1278,89 @@ __global__ __launch_bounds__(kBlock) void k_cg_iter(int n, double rx, double ry,
   double* __restrict__ rs = r + o;
   const double* __restrict__ pin = ((S.iters & 1) ? pb : pa) + o;
   double* __restrict__ pout = ((S.iters & 1) ? pa : pb) + o;
-  const int nn = n * n, di = kBlock % n, dj = kBlock / n, i0 = threadIdx.x % n, j0 = threadIdx.x / n;
   const double beta = S.beta;
   __shared__ double sh[2 * kWavesPB];
-  auto dir = [&](int q) { return rs[q] / dg + beta * pin[q]; };
-  double pap = 0.0;
-  for (int k = threadIdx.x, i = i0, j = j0; k < nn; k += kBlock) {
-    const double c = dir(k);
-    const double w = i > 0 ? dir(k - 1) : 0.0, e = i < n - 1 ? dir(k + 1) : 0.0;
-    const double s = j > 0 ? dir(k - n) : 0.0, no = j < n - 1 ? dir(k + n) : 0.0;
-    pout[k] = c;
-    pap += c * op5(c, w, e, s, no, rx, ry);
-    i += di;
-    j += dj;
-    if (i >= n) i -= n, j++;
-  }
-  // the barriers of the sum also make every sweep-1 store of pout visible to
-  // the workgroup before sweep 2 reads its neighbours
-  pap = block_sum(pap, sh);
-  const double alpha = S.rz / pap;
-  double rr = 0.0, rz = 0.0;
-  for (int k = threadIdx.x, i = i0, j = j0; k < nn; k += kBlock) {
-    const double c = pout[k];
-    const double w = i > 0 ? pout[k - 1] : 0.0, e = i < n - 1 ? pout[k + 1] : 0.0;
-    const double s = j > 0 ? pout[k - n] : 0.0, no = j < n - 1 ? pout[k + n] : 0.0;
-    const double v = op5(c, w, e, s, no, rx, ry);
-    xs[k] += alpha * c;
-    const double rk = rs[k] - alpha * v;
-    rs[k] = rk;
-    rr += rk * rk;
-    rz += rk * (rk / dg);
-    i += di;
-    j += dj;
-    if (i >= n) i -= n, j++;
+  auto dir = [&](int q) { return rs[q] * idg + beta * pin[q]; };
+  double pap = 0.0, rr = 0.0, rz = 0.0, alpha;
+  if (PAIRS) {
+    // cells (2m, 2m + 1); m walks the n/2 pairs of each row
+    const int h = n / 2, npair = h * n, di = kBlock % h, dj = kBlock / h, i0 = threadIdx.x % h,
+              j0 = threadIdx.x / h;
+    for (int m = threadIdx.x, i = i0, j = j0; m < npair; m += kBlock) {
+      const int q = 2 * m, c = 2 * i;
+      const double2 rc = cg_ld2(rs, q), pc = cg_ld2(pin, q);
+      const double c0 = rc.x * idg + beta * pc.x, c1 = rc.y * idg + beta * pc.y;
+      const double w = c > 0 ? dir(q - 1) : 0.0, e = c + 2 < n ? dir(q + 2) : 0.0;
+      double s0 = 0.0, s1 = 0.0, n0 = 0.0, n1 = 0.0;
+      if (j > 0) {
+        const double2 ra = cg_ld2(rs, q - n), pv = cg_ld2(pin, q - n);
+        s0 = ra.x * idg + beta * pv.x;
+        s1 = ra.y * idg + beta * pv.y;
+      }
+      if (j < n - 1) {
+        const double2 ra = cg_ld2(rs, q + n), pv = cg_ld2(pin, q + n);
+        n0 = ra.x * idg + beta * pv.x;
+        n1 = ra.y * idg + beta * pv.y;
+      }
+      cg_st2(pout, q, c0, c1);
+      pap += c0 * op5(c0, w, c1, s0, n0, rx, ry);
+      pap += c1 * op5(c1, c0, e, s1, n1, rx, ry);
+      i += di;
+      j += dj;
+      if (i >= h) i -= h, j++;
+    }
+    // the barriers of the sum also make every sweep-1 store of pout visible
+    // to the workgroup before sweep 2 reads its neighbours
+    pap = block_sum(pap, sh);
+    alpha = S.rz / pap;
+    for (int m = threadIdx.x, i = i0, j = j0; m < npair; m += kBlock) {
+      const int q = 2 * m, c = 2 * i;
+      const double2 pc = cg_ld2(pout, q);
+      const double w = c > 0 ? pout[q - 1] : 0.0, e = c + 2 < n ? pout[q + 2] : 0.0;
+      const double2 ps = j > 0 ? cg_ld2(pout, q - n) : double2{0.0, 0.0};
+      const double2 pn = j < n - 1 ? cg_ld2(pout, q + n) : double2{0.0, 0.0};
+      const double v0 = op5(pc.x, w, pc.y, ps.x, pn.x, rx, ry);
+      const double v1 = op5(pc.y, pc.x, e, ps.y, pn.y, rx, ry);
+      const double2 xc = cg_ld2(xs, q), rc = cg_ld2(rs, q);
+      cg_st2(xs, q, xc.x + alpha * pc.x, xc.y + alpha * pc.y);
+      const double r0 = rc.x - alpha * v0, r1 = rc.y - alpha * v1;
+      cg_st2(rs, q, r0, r1);
+      rr += r0 * r0;
+      rr += r1 * r1;
+      rz += r0 * (r0 * idg);
+      rz += r1 * (r1 * idg);
+      i += di;
+      j += dj;
+      if (i >= h) i -= h, j++;
+    }
+  } else {
+    const int nn = n * n, di = kBlock % n, dj = kBlock / n, i0 = threadIdx.x % n, j0 = threadIdx.x / n;
+    for (int k = threadIdx.x, i = i0, j = j0; k < nn; k += kBlock) {
+      const double c = dir(k);
+      const double w = i > 0 ? dir(k - 1) : 0.0, e = i < n - 1 ? dir(k + 1) : 0.0;
+      const double s = j > 0 ? dir(k - n) : 0.0, no = j < n - 1 ? dir(k + n) : 0.0;
+      pout[k] = c;
+      pap += c * op5(c, w, e, s, no, rx, ry);
+      i += di;
+      j += dj;
+      if (i >= n) i -= n, j++;
+    }
+    pap = block_sum(pap, sh);  // its barriers order the pout stores before sweep 2
+    alpha = S.rz / pap;
+    for (int k = threadIdx.x, i = i0, j = j0; k < nn; k += kBlock) {
+      const double c = pout[k];
+      const double w = i > 0 ? pout[k - 1] : 0.0, e = i < n - 1 ? pout[k + 1] : 0.0;
+      const double s = j > 0 ? pout[k - n] : 0.0, no = j < n - 1 ? pout[k + n] : 0.0;
+      const double v = op5(c, w, e, s, no, rx, ry);
+      xs[k] += alpha * c;
+      const double rk = rs[k] - alpha * v;
+      rs[k] = rk;
+      rr += rk * rk;
+      rz += rk * (rk * idg);
+      i += di;
+      j += dj;
+      if (i >= n) i -= n, j++;
+    }
   }
   block_sum2(rr, rz, sh);
   if (threadIdx.x == 0) st[blockIdx.x] = CGState{rz, S.bb, rz / S.rz, rr <= tol2 * S.bb ? 1 : 0, S.iters + 1};
@@ -2126,17 +2184,20 @@ int LearnedPrec_MakeTrainingSetXY(int nsamp, int n, double ratio_x, double ratio
   hipLaunchKernelGGL(k_applyA, dim3(gg), dim3(kBlock), 0, s, nsamp, n, n, ratio_x, ratio_y, r, tmp);
   hipLaunchKernelGGL(k_take_odd, dim3(gg), dim3(kBlock), 0, s, (int64_t)n * n, nsamp, tmp, r);
   // Jacobi-PCG on every pair
-  const double dg = 1.0 + (2.0 * ratio_x + 2.0 * ratio_y);  // 1 + 4 ratio exactly when isotropic
-  hipLaunchKernelGGL(k_cg_init, dim3(nsamp), dim3(kBlock), 0, s, n, dg, r, e, tmp, p, st);
+  const double idg = 1.0 / (1.0 + (2.0 * ratio_x + 2.0 * ratio_y));  // inverse Jacobi diagonal
+  hipLaunchKernelGGL(k_cg_init, dim3(nsamp), dim3(kBlock), 0, s, n, idg, r, e, tmp, p, st);
   std::vector<CGState> hst(nsamp);
   const int maxit = 20000;
   int it = 0;
   bool all = false;
   while (!all && it < maxit) {
     for (int k = 0; k < 50; k++, it++)
-      hipLaunchKernelGGL(k_cg_iter, dim3(nsamp), dim3(kBlock), 0, s, n, ratio_x, ratio_y, dg, reltol * reltol, e,
-                         tmp, p,
-                         p2, st);
+      if (n % 2 == 0)
+        hipLaunchKernelGGL(k_cg_iter<true>, dim3(nsamp), dim3(kBlock), 0, s, n, ratio_x, ratio_y, idg,
+                           reltol * reltol, e, tmp, p, p2, st);
+      else
+        hipLaunchKernelGGL(k_cg_iter<false>, dim3(nsamp), dim3(kBlock), 0, s, n, ratio_x, ratio_y, idg,
+                           reltol * reltol, e, tmp, p, p2, st);
     CEDA_HIP_CHECK(hipMemcpyAsync(hst.data(), st, sizeof(CGState) * nsamp, hipMemcpyDeviceToHost, s));
     CEDA_HIP_CHECK(hipStreamSynchronize(s));
     all = true;

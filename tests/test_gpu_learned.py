@@ -89,8 +89,9 @@ def test_loss_and_gradient_match_oracle(pk, ctx, seed):
     net.release()
 
 
-def test_training_pairs_solve_the_system(pk, ctx):
-    nsamp, n, ratio = 24, 32, 60.0
+@pytest.mark.parametrize("n", [32, 31])  # even sides: the paired kernel, odd: cell by cell
+def test_training_pairs_solve_the_system(pk, ctx, n):
+    nsamp, ratio = 24, 60.0
     R, E, iters, rc = pk.training_set(ctx, nsamp, n, ratio, reltol=1e-10, seed=1234)
     assert rc == 0 and iters > 5
     r, e = R.get().reshape(nsamp, -1), E.get().reshape(nsamp, -1)
@@ -470,9 +471,9 @@ def kron_A(nx, ny, rx, ry):
             - ry * sp.kron(d2(ny), sp.identity(nx))).tocsr()
 
 
-@pytest.mark.parametrize("rx,ry", [(60.0, 6.0), (2.0, 45.0)])
-def test_anisotropic_training_pairs_solve_their_system(pk, ctx, rx, ry):
-    nsamp, n = 10, 32
+@pytest.mark.parametrize("rx,ry,n", [(60.0, 6.0, 32), (2.0, 45.0, 32), (30.0, 3.0, 27)])
+def test_anisotropic_training_pairs_solve_their_system(pk, ctx, rx, ry, n):
+    nsamp = 10
     R, E, iters, rc = pk.training_set(ctx, nsamp, n, rx, reltol=1e-10, seed=99, ratio_y=ry)
     assert rc == 0 and iters > 5
     r, e = R.get().reshape(nsamp, -1), E.get().reshape(nsamp, -1)
