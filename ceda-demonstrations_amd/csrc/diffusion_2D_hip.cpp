@@ -173,13 +173,14 @@ int main(int argc, char** argv)
   }
 
   // learned preconditioner: the symmetric network K^T K (4 layers, dilations
-  // 1, 3, 9, 27), trained for this run's gamma*h*k/dx^2 (h = --fixedstep, or
-  // tf/nout), on pairs from Jacobi-PCG solves
+  // 1, 3, 9, 27), trained on pairs from Jacobi-PCG solves with this run's
+  // operator I - rx Dxx - ry Dyy, r = gamma*h*k/d^2 along each axis (h =
+  // --fixedstep, or tf/nout)
   LearnedPrec* net = nullptr;
   if (learned) {
     const double gam = (order == 3) ? 1767732205903.0 / 4055673282236.0 : 1.0;  // DIRK diagonal
     const double h = (hfixed > 0.0) ? hfixed : prm.tf / nout;
-    const double ratio = gam * h * 0.5 * (prm.kx / (dx * dx) + prm.ky / (dy * dy));
+    const double rx = gam * h * (prm.kx / (dx * dx)), ry = gam * h * (prm.ky / (dy * dy));
     const int dil[4] = {1, 3, 9, 27};
     const double tt = wall();
     net = LearnedPrec_CreateSymmetric(4, dil, ctx);
@@ -187,18 +188,18 @@ int main(int argc, char** argv)
     const size_t bytes = sizeof(double) * (size_t)learned_pairs * learned_grid * learned_grid;
     int its = 0;
     bool ok = net && hipMalloc(&r, bytes) == hipSuccess && hipMalloc(&e, bytes) == hipSuccess;
-    ok = ok && LearnedPrec_MakeTrainingSet(learned_pairs, learned_grid, ratio, 1e-12, 1234, r, e, &its, ctx) == 0;
+    ok = ok && LearnedPrec_MakeTrainingSetXY(learned_pairs, learned_grid, rx, ry, 1e-12, 1234, r, e, &its, ctx) == 0;
     ok = ok && LearnedPrec_InitIdentity(net, 8e-4, 0) == 0;
-    ok = ok && LearnedPrec_SetOutputScale(net, 1.0 / (1.0 + 4.0 * ratio)) == 0;
-    ok = ok && LearnedPrec_Train(net, learned_pairs, learned_grid, ratio, r, e, 1000, 16, 3e-3, 0, nullptr) == 0;
+    ok = ok && LearnedPrec_SetOutputScale(net, 1.0 / (1.0 + 2.0 * rx + 2.0 * ry)) == 0;
+    ok = ok && LearnedPrec_TrainXY(net, learned_pairs, learned_grid, rx, ry, r, e, 1000, 16, 3e-3, 0, nullptr) == 0;
     if (r) (void)hipFree(r);
     if (e) (void)hipFree(e);
     if (!ok || DiffusionProblem_SetLearnedPrec(udata, net)) {
       fprintf(stderr, "ERROR: learned preconditioner setup\n");
       return 1;
     }
-    printf("  learned preconditioner: symmetric network trained in %.2f s (gamma*h*k/dx^2 = %g, %d pairs of "
-           "%d^2)\n", wall() - tt, ratio, learned_pairs, learned_grid);
+    printf("  learned preconditioner: symmetric network trained in %.2f s (gamma*h*k/d^2 = %g, %g, %d pairs of "
+           "%d^2)\n", wall() - tt, rx, ry, learned_pairs, learned_grid);
   }
 
   void* arkode_mem = impl ? ARKStepCreate(nullptr, diffusion, 0.0, u, ctx)
