@@ -194,7 +194,7 @@ def c4_problems(n=256, seed=20260101):
     return out
 
 
-def batch_workload(pk, nprob, tout, batch_only=False, kpass=0, tile_rows=0):
+def batch_workload(pk, nprob, tout, batch_only=False, kpass=0, tile_rows=0, thin=None):
     """Batched RKC (one step group per step: every problem plans its attempt,
     then one launch per phase over all problems still in their stage loop)
     against the same problems solved one after another with the
@@ -212,6 +212,8 @@ def batch_workload(pk, nprob, tout, batch_only=False, kpass=0, tile_rows=0):
         assert B.SetStagesPerPass(kpass) == 0
     if tile_rows:
         assert pk.lib().DiffusionBatch_SetTileRows(B.h, tile_rows) == 0
+    if thin is not None:
+        assert pk.lib().DiffusionBatch_SetThinPhases(B.h, thin[0], thin[1]) == 0
     hip_sync()
     log(f"batched: {nprob} problems, {sum(p['nx'] * p['ny'] for p in plist)} cells, "
         f"{kpass or 'default'} stages per pass")
@@ -844,6 +846,8 @@ def parse_args(argv=None):
     ap.add_argument("--pair-ahead", type=int, default=2, help="input rows in flight per workgroup, two-layer kernel")
     ap.add_argument("--kpass", type=int, default=0, help="batch: stages per pass (0: the library default)")
     ap.add_argument("--tile-rows", type=int, default=0, help="batch: rows per wavefront tile (0: autotuned)")
+    ap.add_argument("--thin-phases", default="",
+                    help="batch: ROWS,MIN_TILES of the fine tiles of thin phases (ROWS 0: never; default 12,2048)")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch: the batched solve alone, without the one-after-another comparison")
     return ap.parse_args(argv)
@@ -869,7 +873,8 @@ def main():
         kernels_only(pk, args.kernels_only, args.sweep, pin)
         return
     if args.workload == "batch":
-        batch_workload(pk, args.nprob, args.tout, args.batch_only, args.kpass, args.tile_rows)
+        thin = tuple(int(v) for v in args.thin_phases.split(",")) if args.thin_phases else None
+        batch_workload(pk, args.nprob, args.tout, args.batch_only, args.kpass, args.tile_rows, thin)
         return
     if args.workload == "implicit":
         implicit_workload(pk, args)

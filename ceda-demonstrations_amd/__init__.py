@@ -178,6 +178,7 @@ _SIGS = {
     "DiffusionBatch_SetMaxNumSteps": (i32, [vp, C.c_long]),
     "DiffusionBatch_SetFixedStep": (i32, [vp, d]),
     "DiffusionBatch_SetTileRows": (i32, [vp, i32]),
+    "DiffusionBatch_SetThinPhases": (i32, [vp, i32, i32]),
     "DiffusionBatch_SetStagesPerPass": (i32, [vp, i32]),
     "DiffusionBatch_Evolve": (i32, [vp, d]),
     "DiffusionBatch_GetSolution": (i32, [vp, i32, vp]),

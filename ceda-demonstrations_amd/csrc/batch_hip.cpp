@@ -44,10 +44,12 @@ struct BProb {
   bool done = false, new_step = true;
   int err = 0;
   int rows = 0, cpl = 1, strips = 0, rbs = 0;
+  int rows_f = 0, rbs_f = 0;  // the fine tiles of thin phases
   StagePlan SP;
 };
 
 int tiles_of(const BProb& b) { return b.strips * b.rbs; }
+int fine_tiles_of(const BProb& b) { return b.strips * b.rbs_f; }
 
 }  // namespace
 
@@ -58,6 +60,12 @@ struct DiffusionBatch {
   int method = M_RKC;
   int smax = 200, stride = 202;
   int tile_rows = 0;
+  // thin phases (fewer than thin_tiles wavefront tiles, late in a step group
+  // where only the problems with the most stages are left) run on tiles of
+  // thin_rows rows: more, shorter column marches, each re-computing the same
+  // 2(K-1) halo rows, so the phase ends sooner on an underfilled GPU; the
+  // cells' values do not depend on the tiling.  thin_rows = 0: coarse tiles only
+  int thin_rows = 12, thin_tiles = 2048;
   int kpass = ceda::kBatchMaxK;
   int layout = ceda::kLayoutAny;  // shared column layout and boundary of the problems
   bool tuned = false;
@@ -69,12 +77,14 @@ struct DiffusionBatch {
   double alg_bytes = 0.0;
   // device/pinned staging (grown on demand)
   BatchDesc* d_desc = nullptr;
+  BatchDesc* d_desc_f = nullptr;  // the same problems with the fine tiles' geometry
   StageEntry *d_ent = nullptr, *h_ent = nullptr;
   BatchTile *d_tiles = nullptr, *h_tiles = nullptr;
+  BatchTile *d_tiles_f = nullptr, *h_tiles_f = nullptr;
   ceda::StsErrItem *d_err = nullptr, *h_err = nullptr;  // per slot: its error-norm reduction
   int *d_s = nullptr, *h_s = nullptr;
   double *d_part = nullptr, *d_dsq = nullptr, *h_dsq = nullptr;
-  int cap_slots = 0, cap_tiles = 0, cap_vblocks = 0;
+  int cap_slots = 0, cap_tiles = 0, cap_vblocks = 0, cap_tiles_f = 0;
 };
 
 namespace {
@@ -92,28 +102,33 @@ void host_alloc(T** h, size_t n)
 
 void free_staging(DiffusionBatch* B)
 {
-  void* ds[] = {B->d_ent, B->d_tiles, B->d_err, B->d_s, B->d_part, B->d_dsq};
+  void* ds[] = {B->d_ent, B->d_tiles, B->d_tiles_f, B->d_err, B->d_s, B->d_part, B->d_dsq};
   for (void* p : ds)
     if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
-  void* hs[] = {B->h_ent, B->h_tiles, B->h_err, B->h_s, B->h_dsq};
+  void* hs[] = {B->h_ent, B->h_tiles, B->h_tiles_f, B->h_err, B->h_s, B->h_dsq};
   for (void* p : hs)
     if (p) CEDA_HIP_CHECK(ceda::host_free(p));
   B->d_ent = B->h_ent = nullptr;
   B->d_tiles = B->h_tiles = nullptr;
+  B->d_tiles_f = B->h_tiles_f = nullptr;
   B->d_err = B->h_err = nullptr;
   B->d_s = B->h_s = nullptr;
   B->d_part = B->d_dsq = B->h_dsq = nullptr;
 }
 
-void ensure_staging(DiffusionBatch* B, int slots, int tiles, int vblocks = 0)
+void ensure_staging(DiffusionBatch* B, int slots, int tiles, int vblocks = 0, int tiles_f = 0)
 {
-  if (slots <= B->cap_slots && tiles <= B->cap_tiles && vblocks <= B->cap_vblocks) return;
+  if (slots <= B->cap_slots && tiles <= B->cap_tiles && vblocks <= B->cap_vblocks && tiles_f <= B->cap_tiles_f)
+    return;
   CEDA_HIP_CHECK(hipStreamSynchronize(B->ex->stream));
   free_staging(B);
   B->cap_slots = std::max(slots, B->cap_slots);
   B->cap_tiles = std::max(tiles, B->cap_tiles);
   B->cap_vblocks = std::max(vblocks, B->cap_vblocks);
+  B->cap_tiles_f = std::max(tiles_f, B->cap_tiles_f);
   const size_t ns = B->cap_slots, nt = B->cap_tiles;
+  dev_alloc(&B->d_tiles_f, B->cap_tiles_f);
+  host_alloc(&B->h_tiles_f, B->cap_tiles_f);
   dev_alloc(&B->d_ent, ns * B->stride);
   host_alloc(&B->h_ent, ns * B->stride);
   memset(B->h_ent, 0, sizeof(StageEntry) * ns * B->stride);
@@ -144,18 +159,31 @@ void list_tiles(const BProb& b, int slot, int prob, BatchTile* out, int& t)
   for (int rb = 0; rb < b.rbs; rb++)
     for (int x = 0; x < b.strips; x++) out[t++] = BatchTile{slot, prob, x, rb};
 }
+void list_fine_tiles(const BProb& b, int slot, int prob, BatchTile* out, int& t)
+{
+  for (int rb = 0; rb < b.rbs_f; rb++)
+    for (int x = 0; x < b.strips; x++) out[t++] = BatchTile{slot, prob, x, rb};
+}
 
+// the problems' descriptors with the coarse tiles' geometry and with the
+// fine one (at most thin_rows rows, never more than the coarse tiles')
 void upload_desc(DiffusionBatch* B)
 {
-  std::vector<BatchDesc> h(B->probs.size());
+  std::vector<BatchDesc> h(B->probs.size()), hf(B->probs.size());
   for (size_t k = 0; k < B->probs.size(); k++) {
-    const BProb& b = B->probs[k];
+    BProb& b = B->probs[k];
     const DiffusionProblem* P = b.P;
+    b.rows_f = B->thin_rows > 0 ? std::min(B->thin_rows, b.rows) : b.rows;
+    b.rbs_f = (int)((P->p.ny + b.rows_f - 1) / b.rows_f);
     h[k] = BatchDesc{P->d_fx, P->d_fy, P->d_sx, P->d_sy, P->d_cx, P->d_cy, P->p.nx, P->p.ny,
                      P->p.kind == DIFFUSION_CEDA_PERIODIC ? 1 : 0, b.cpl, b.rows, b.strips};
+    hf[k] = h[k];
+    hf[k].rows = b.rows_f;
   }
   if (!B->d_desc) dev_alloc(&B->d_desc, h.size());
+  if (!B->d_desc_f) dev_alloc(&B->d_desc_f, h.size());
   CEDA_HIP_CHECK(ceda::copy_sync(B->d_desc, h.data(), sizeof(BatchDesc) * h.size(), hipMemcpyHostToDevice));
+  CEDA_HIP_CHECK(ceda::copy_sync(B->d_desc_f, hf.data(), sizeof(BatchDesc) * hf.size(), hipMemcpyHostToDevice));
 }
 
 // the tile autotuner: for every size class (nx, ny) time a full-length pass
@@ -265,15 +293,17 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
     return B->probs[a].SP.s > B->probs[c].SP.s;
   });
   const bool adaptive = !B->probs[planned[0]].A->fixedstep;
-  int ntiles = 0, nvblocks = 0;
+  const bool thin = B->thin_rows > 0;
+  int ntiles = 0, nvblocks = 0, ntiles_f = 0;
   for (int k : planned) {
     ntiles += tiles_of(B->probs[k]);
+    if (thin) ntiles_f += fine_tiles_of(B->probs[k]);
     nvblocks += ceda::sts_err_blocks(B->probs[k].P->n);
   }
   const int nslots = (int)planned.size();
-  ensure_staging(B, nslots, ntiles, nvblocks);
-  int t = 0, vb = 0;
-  std::vector<int> tile_end(nslots);
+  ensure_staging(B, nslots, ntiles, nvblocks, ntiles_f);
+  int t = 0, vb = 0, tf = 0;
+  std::vector<int> tile_end(nslots), tile_end_f(nslots);
   for (int q = 0; q < nslots; q++) {
     BProb& b = B->probs[planned[q]];
     ArkMem* A = b.A;
@@ -282,6 +312,8 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
     B->h_s[q] = b.SP.s;
     list_tiles(b, q, planned[q], B->h_tiles, t);
     tile_end[q] = t;
+    if (thin) list_fine_tiles(b, q, planned[q], B->h_tiles_f, tf);
+    tile_end_f[q] = tf;
     // the single-problem path's reduction (raw_sts_error_wsqr), bit for bit
     const int g = ceda::sts_err_blocks(A->n);
     B->h_err[q] = ceda::StsErrItem{N_VGetDeviceArrayPointer(A->yn), N_VGetDeviceArrayPointer(A->ycur),
@@ -294,6 +326,9 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
                                 hipMemcpyHostToDevice, s));
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tiles, B->h_tiles, sizeof(BatchTile) * ntiles,
                                 hipMemcpyHostToDevice, s));
+  if (thin)
+    CEDA_HIP_CHECK(hipMemcpyAsync(B->d_tiles_f, B->h_tiles_f, sizeof(BatchTile) * ntiles_f,
+                                  hipMemcpyHostToDevice, s));
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_s, B->h_s, sizeof(int) * nslots, hipMemcpyHostToDevice, s));
   CEDA_HIP_CHECK(hipMemcpyAsync(B->d_err, B->h_err, sizeof(ceda::StsErrItem) * nslots, hipMemcpyHostToDevice, s));
   // phase 0: stage 1 of every slot; phase q >= 1 starts at stage
@@ -309,8 +344,12 @@ int step_group(DiffusionBatch* B, double tout, bool* progress)
     const int need = adaptive ? j - kp : j;
     while (active > 0 && B->h_s[active - 1] < need) active--;
     if (active == 0) break;
-    ceda::launch_batch_phase(B->d_desc, B->d_tiles, tile_end[active - 1], B->d_ent, B->stride,
-                             B->d_s, q, kp, adaptive ? 1 : 0, B->layout, s);
+    if (thin && tile_end[active - 1] < B->thin_tiles)
+      ceda::launch_batch_phase(B->d_desc_f, B->d_tiles_f, tile_end_f[active - 1], B->d_ent, B->stride,
+                               B->d_s, q, kp, adaptive ? 1 : 0, B->layout, s);
+    else
+      ceda::launch_batch_phase(B->d_desc, B->d_tiles, tile_end[active - 1], B->d_ent, B->stride,
+                               B->d_s, q, kp, adaptive ? 1 : 0, B->layout, s);
     B->nlaunches++;
   }
   if (adaptive) {
@@ -404,6 +443,7 @@ void DiffusionBatch_Destroy(DiffusionBatch* B)
   CEDA_HIP_CHECK(hipStreamSynchronize(B->ex->stream));
   free_staging(B);
   if (B->d_desc) CEDA_HIP_CHECK(ceda::dev_free(B->d_desc));
+  if (B->d_desc_f) CEDA_HIP_CHECK(ceda::dev_free(B->d_desc_f));
   for (BProb& b : B->probs) {
     if (b.A) {
       void* m = b.A;
@@ -470,6 +510,15 @@ int DiffusionBatch_SetTileRows(DiffusionBatch* B, int rows)
     upload_desc(B);
     B->tuned = true;
   } else B->tuned = false;
+  return 0;
+}
+
+int DiffusionBatch_SetThinPhases(DiffusionBatch* B, int rows, int min_tiles)
+{
+  if (!B || rows < 0 || rows > 256 || min_tiles < 1) return ARK_ILL_INPUT;
+  B->thin_rows = rows;
+  B->thin_tiles = min_tiles;
+  if (B->tuned) upload_desc(B);
   return 0;
 }
 

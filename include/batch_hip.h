@@ -38,6 +38,12 @@ int DiffusionBatch_SetFixedStep(DiffusionBatch* B, sunrealtype h);
    124-column strip (62 for odd nx); rows 1..256, 0 = choose per size class by
    the tile autotuner */
 int DiffusionBatch_SetTileRows(DiffusionBatch* B, int rows);
+/* thin phases: a phase launch that would have fewer than min_tiles wavefront
+   tiles (late in a step group, when only the problems with the most stages
+   are left) runs on tiles of at most `rows` rows instead (shorter column
+   marches over more wavefronts); rows 1..256, 0 = never; defaults 12 and
+   2048.  The results do not depend on it. */
+int DiffusionBatch_SetThinPhases(DiffusionBatch* B, int rows, int min_tiles);
 /* stages per pass of the batched multi-stage kernel (1..5, default 5): each
    phase launch advances every active problem by up to this many stages */
 int DiffusionBatch_SetStagesPerPass(DiffusionBatch* B, int kpass);

@@ -181,3 +181,32 @@ def test_mixed_batch_equals_single_problems(pk, ctx, method):
         I.free()
         G.destroy()
     B.release()
+
+
+@pytest.mark.parametrize("thin", [(0, 1), (4, 1 << 30), (12, 2048), (1, 64)])
+def test_thin_phase_tiles_do_not_change_the_result(pk, ctx, thin):
+    """Thin phases may run on finer tiles (shorter column marches): never
+    (rows 0), always with 4 rows, the default, or 1 row below 64 tiles.  The
+    tiling does not enter the cells' values: every setting gives the
+    coarse-only batch's states bit for bit, with the same counts."""
+    plist = problems(23, 16, [16, 33, 48, 64]) + problems(24, 8, [24, 31], kind=0)
+    out = []
+    for rows, tiles in ((0, 1), thin):
+        B = pk.Batch(ctx, plist, "rkc")
+        B.SetTolerances(1e-5, 1e-10)
+        B.SetMaxNumSteps(100000)
+        assert pk.lib().DiffusionBatch_SetThinPhases(B.h, rows, tiles) == 0
+        assert B.evolve(0.04) == 0
+        out.append(([B.solution(k) for k in range(len(plist))], [B.stats(k)["nfe"] for k in range(len(plist))]))
+        B.release()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert out[0][1] == out[1][1]
+
+
+def test_thin_phase_setting_rejects_bad_values(pk, ctx):
+    B = pk.Batch(ctx, problems(25, 2, [16]), "rkc")
+    assert pk.lib().DiffusionBatch_SetThinPhases(B.h, -1, 10) != 0
+    assert pk.lib().DiffusionBatch_SetThinPhases(B.h, 257, 10) != 0
+    assert pk.lib().DiffusionBatch_SetThinPhases(B.h, 8, 0) != 0
+    B.release()
