@@ -31,6 +31,7 @@ struct SUNLinearSolver_ {
   int mrestart = 30;
   N_Vector r = nullptr, p = nullptr, z = nullptr, Ap = nullptr;
   std::vector<N_Vector> V, Z;
+  double* vz_pool = nullptr;  // one allocation holding every V and Z vector's data
   // fused PCG: per-workgroup partial sums and the solve state (device), and
   // the state's host copy (pinned); allocated on the first fused solve
   double* d_ws = nullptr;

@@ -1571,14 +1571,34 @@ static SUNLinearSolver make_ls(int type, N_Vector y, int pretype, int maxl)
   return S;
 }
 
-static void gmres_alloc(SUNLinearSolver S, N_Vector y)
+static void gmres_free(SUNLinearSolver S)
 {
   for (N_Vector v : S->V) N_VDestroy(v);
   for (N_Vector v : S->Z) N_VDestroy(v);
   S->V.clear();
   S->Z.clear();
-  for (int i = 0; i <= S->mrestart; i++) S->V.push_back(N_VClone(y));
-  for (int i = 0; i < S->mrestart; i++) S->Z.push_back(N_VClone(y));
+  if (S->vz_pool) {  // no kernel may still use it (S->r shares the vectors' stream)
+    CEDA_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(N_VGetStream_HipNative(S->r))));
+    CEDA_HIP_CHECK(ceda::dev_free(S->vz_pool));
+    S->vz_pool = nullptr;
+  }
+}
+
+// the Krylov basis V_0..V_m and the preconditioned Z_0..Z_{m-1} in one
+// device allocation (vectors that do not own their data, each starting on a
+// 256-byte boundary), instead of 2m + 1 allocations: creating many solvers
+// from concurrent host threads then costs one allocator call each
+static void gmres_alloc(SUNLinearSolver S, N_Vector y)
+{
+  gmres_free(S);
+  const sunindextype n = N_VGetLength(y);
+  const size_t stride = ((size_t)n + 31) / 32 * 32;
+  const int nv = 2 * S->mrestart + 1;
+  CEDA_HIP_CHECK(ceda::dev_malloc(&S->vz_pool, sizeof(double) * stride * (size_t)nv));
+  for (int i = 0; i < nv; i++) {
+    N_Vector v = N_VMake_HipNative(n, S->vz_pool + stride * (size_t)i, y->sunctx);
+    (i <= S->mrestart ? S->V : S->Z).push_back(v);
+  }
 }
 
 SUNLinearSolver SUNLinSol_PCG(N_Vector y, int pretype, int maxl, SUNContext)
@@ -1620,11 +1640,10 @@ int SUNLinSol_SPFGMRSetMaxRestartDim(SUNLinearSolver S, int mrestart)
 int SUNLinSolFree(SUNLinearSolver S)
 {
   if (!S) return 0;
+  gmres_free(S);  // before S->r, whose stream it waits on
   N_Vector vs[] = {S->r, S->p, S->z, S->Ap};
   for (N_Vector v : vs)
     if (v) N_VDestroy(v);
-  for (N_Vector v : S->V) N_VDestroy(v);
-  for (N_Vector v : S->Z) N_VDestroy(v);
   if (S->d_ws) CEDA_HIP_CHECK(ceda::dev_free(S->d_ws));
   if (S->h_ws) CEDA_HIP_CHECK(ceda::host_free(S->h_ws));
   if (S->d_p2) CEDA_HIP_CHECK(ceda::dev_free(S->d_p2));
