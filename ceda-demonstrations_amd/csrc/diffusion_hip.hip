@@ -22,6 +22,7 @@
 //   t = -((Dw+De)+(Ds+Dn))*uc + Dw*uw + De*ue + Ds*us + Dn*un ; f = 0 + t
 // and the library is built without FMA contraction, which makes the result
 // bitwise identical to a serial CPU evaluation.
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -391,13 +392,26 @@ double coeff(double k, int inhomogeneous, double s)
   return k;
 }
 
-double* upload(const std::vector<double>& h)
+// the problem's 1-D tables (face coefficients, forcing and initial-condition
+// factors) in one device allocation, each table starting on a 256-byte
+// boundary: one allocation and one copy per problem, not one of each per
+// table (problems created from several host threads contend for both)
+void upload_tables(DiffusionProblem* P)
 {
-  double* d = nullptr;
-  CEDA_HIP_CHECK(ceda::dev_malloc(&d, sizeof(double) * (h.empty() ? 1 : h.size())));
-  if (!h.empty())
-    CEDA_HIP_CHECK(ceda::copy_sync(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-  return d;
+  const std::vector<double>* hs[] = {&P->fx, &P->fy, &P->px, &P->py, &P->sx,
+                                     &P->sy, &P->cx, &P->cy, &P->ax, &P->by};
+  double** ds[] = {&P->d_fx, &P->d_fy, &P->d_px, &P->d_py, &P->d_sx, &P->d_sy, &P->d_cx, &P->d_cy, &P->d_ax, &P->d_by};
+  constexpr int nt = sizeof(hs) / sizeof(hs[0]);
+  size_t off[nt], tot = 0;
+  for (int i = 0; i < nt; i++) {
+    off[i] = tot;
+    tot += (std::max<size_t>(hs[i]->size(), 1) + 31) / 32 * 32;
+  }
+  std::vector<double> h(tot, 0.0);
+  for (int i = 0; i < nt; i++) std::copy(hs[i]->begin(), hs[i]->end(), h.begin() + (ptrdiff_t)off[i]);
+  CEDA_HIP_CHECK(ceda::dev_malloc(&P->d_tables, sizeof(double) * tot));
+  CEDA_HIP_CHECK(ceda::copy_sync(P->d_tables, h.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+  for (int i = 0; i < nt; i++) *ds[i] = P->d_tables + off[i];
 }
 
 // autotuner results, per (nx, ny, kind) for the life of the process
@@ -555,11 +569,7 @@ DiffusionProblem* DiffusionProblem_Create(const DiffusionParams* prm, SUNContext
       P->cy[j] = prm->ky * (-tpp * cos(2.0 * kPi * y));
     }
   }
-  P->d_fx = upload(P->fx); P->d_fy = upload(P->fy);
-  P->d_px = upload(P->px); P->d_py = upload(P->py);
-  P->d_sx = upload(P->sx); P->d_sy = upload(P->sy);
-  P->d_cx = upload(P->cx); P->d_cy = upload(P->cy);
-  P->d_ax = upload(P->ax); P->d_by = upload(P->by);
+  upload_tables(P);
   CEDA_HIP_CHECK(ceda::dev_malloc(&P->d_tf, sizeof(double) * 8));
   CEDA_HIP_CHECK(ceda::host_malloc(&P->h_tf, sizeof(double) * 8, hipHostMallocDefault));
   P->diag = N_VNew_HipNative(P->n, ctx);
@@ -576,8 +586,7 @@ void DiffusionProblem_Destroy(DiffusionProblem* P)
   CEDA_HIP_CHECK(hipStreamSynchronize(P->exec->stream));
   N_VDestroy(P->diag);
   ceda::mg_destroy(P->mg);
-  double* ds[] = {P->d_fx, P->d_fy, P->d_px, P->d_py, P->d_sx, P->d_sy, P->d_cx, P->d_cy,
-                  P->d_ax, P->d_by, P->d_tf};
+  double* ds[] = {P->d_tables, P->d_tf};
   for (double* d : ds)
     if (d) CEDA_HIP_CHECK(ceda::dev_free(d));
   if (P->h_tf) CEDA_HIP_CHECK(ceda::host_free(P->h_tf));

@@ -32,6 +32,7 @@ struct DiffusionProblem {
   double *d_fx = nullptr, *d_fy = nullptr, *d_px = nullptr, *d_py = nullptr;
   double *d_sx = nullptr, *d_sy = nullptr, *d_cx = nullptr, *d_cy = nullptr;
   double *d_ax = nullptr, *d_by = nullptr;
+  double* d_tables = nullptr;  // the one allocation the d_ tables above point into
   double* d_tf = nullptr;   // device time factors (T1, T2) for the plain RHS
   double* h_tf = nullptr;   // pinned staging for d_tf
   N_Vector diag = nullptr;  // Jacobi inverse diagonal (udata->diag, main.cpp:223)
