@@ -1435,7 +1435,7 @@ struct LearnedPrec {
   unsigned short *d_afw = nullptr, *d_abw = nullptr;  // [layer][9*256]
   int adam_t = 0;
   // workspace
-  int64_t cap_pix = 0, cap_act = 0;
+  int64_t cap_pix = 0, cap_act = 0, cap_g = 0;
   int cap_nb = 0;
   unsigned short* act[kMaxL] = {};  // act[l] = input of layer l (l >= 1), NHWC16; sym: act[nl] = K x
   unsigned short* uact[kMaxL] = {}; // sym: uact[l] = input of the adjoint of layer l - 1
@@ -1467,28 +1467,39 @@ int rms_parts(int64_t npix) { return (int)std::min<int64_t>(kMaxParts, (npix + 2
 // activation buffers hold nb*ny rows of whole 16-pixel blocks
 int64_t act_elems(int nb, int nx, int ny) { return (int64_t)nb * ny * ((nx + 15) / 16) * 16 * kC; }
 
-void lp_workspace(LearnedPrec* L, int nb, int nx, int ny)
+// device workspace for nb images of nx*ny pixels, grown on demand: the two
+// activation ping-pong buffers and the per-image scale and partial sums that
+// an application needs, and with train the buffers only training needs (the
+// activations kept for the backward pass, the FP64 residual/error images)
+void lp_workspace(LearnedPrec* L, int nb, int nx, int ny, bool train)
 {
   const int64_t npix = (int64_t)nx * ny;
   const int64_t tot = npix * nb;
   const int64_t ae = act_elems(nb, nx, ny);
   const int P = rms_parts(npix);
-  if (tot > L->cap_pix || ae > L->cap_act) {
+  if (ae > L->cap_g) {
+    CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
+    void* ps[] = {L->gA, L->gB};
+    for (void* p : ps)
+      if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gA, sizeof(unsigned short) * ae));
+    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gB, sizeof(unsigned short) * ae));
+    L->cap_g = ae;
+  }
+  if (train && (tot > L->cap_pix || ae > L->cap_act)) {
     CEDA_HIP_CHECK(hipStreamSynchronize(L->ex->stream));
     for (int l = 0; l < kMaxL; l++) {
       if (L->act[l]) CEDA_HIP_CHECK(ceda::dev_free(L->act[l]));
       if (L->uact[l]) CEDA_HIP_CHECK(ceda::dev_free(L->uact[l]));
       L->act[l] = L->uact[l] = nullptr;
     }
-    void* ps[] = {L->gA, L->gB, L->u, L->dd, L->ad, L->gy};
+    void* ps[] = {L->u, L->dd, L->ad, L->gy};
     for (void* p : ps)
       if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
     for (int l = 1; l < L->nl + (L->sym ? 1 : 0); l++)
       CEDA_HIP_CHECK(ceda::dev_malloc(&L->act[l], sizeof(unsigned short) * ae));
     if (L->sym)
       for (int l = 1; l < L->nl; l++) CEDA_HIP_CHECK(ceda::dev_malloc(&L->uact[l], sizeof(unsigned short) * ae));
-    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gA, sizeof(unsigned short) * ae));
-    CEDA_HIP_CHECK(ceda::dev_malloc(&L->gB, sizeof(unsigned short) * ae));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->u, sizeof(double) * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->dd, sizeof(double) * tot));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->ad, sizeof(double) * tot));
@@ -1502,7 +1513,9 @@ void lp_workspace(LearnedPrec* L, int nb, int nx, int ny)
     for (void* p : ps)
       if (p) CEDA_HIP_CHECK(ceda::dev_free(p));
     const int cnb = std::max(nb, L->cap_nb);
-    const int pc = std::max(cnb * P * 2, L->part_cap);
+    // room for every image's largest partial count: images that grow later
+    // need no new allocation
+    const int pc = std::max(cnb * kMaxParts * 2, L->part_cap);
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->sc, sizeof(double) * cnb));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->isc, sizeof(double) * cnb));
     CEDA_HIP_CHECK(ceda::dev_malloc(&L->part, sizeof(double) * pc));
@@ -1511,7 +1524,7 @@ void lp_workspace(LearnedPrec* L, int nb, int nx, int ny)
     L->cap_nb = cnb;
     L->part_cap = pc;
   }
-  if (!L->wpart) CEDA_HIP_CHECK(ceda::dev_malloc(&L->wpart, sizeof(float) * kPW * kWGradWaves));
+  if (train && !L->wpart) CEDA_HIP_CHECK(ceda::dev_malloc(&L->wpart, sizeof(float) * kPW * kWGradWaves));
 }
 
 
@@ -1883,7 +1896,7 @@ void lp_lossgrad(LearnedPrec* L, int nb, int nx, int ny, double rx, double ry, c
 {
   hipStream_t s = L->ex->stream;
   const int64_t npix = (int64_t)nx * ny, tot = npix * nb;
-  lp_workspace(L, nb, nx, ny);
+  lp_workspace(L, nb, nx, ny, true);
   lp_scale(L, nb, npix, r);
   lp_forward(L, nb, nx, ny, r, L->u, 0, true);
   const unsigned gg = grid_for(tot);
@@ -2095,9 +2108,9 @@ int LearnedPrec_SetChainTasks(int min_tasks)
 int LearnedPrec_Apply(LearnedPrec* L, int nb, int nx, int ny, const double* in, double* out, int addto)
 {
   if (!L || nb < 1 || nx < 1 || ny < 1 || !in || !out) return -1;
-  lp_workspace(L, nb, nx, ny);
-  // the application keeps no activations: two of the training buffers serve
-  // as the ping-pong pair
+  lp_workspace(L, nb, nx, ny, false);
+  // the application keeps no activations: the two gradient buffers of
+  // training serve as the ping-pong pair
   lp_apply_in(L, nb, nx, ny, in, out, addto, L->gA, L->gB, L->sc, L->isc, L->part);
   return 0;
 }
