@@ -600,6 +600,8 @@ def c5_workload(pk, args):
         I.analytic_jtimes()
         if args.fused_pcg:
             I.fused_pcg()
+            if args.pcg_bulk != 0.75:
+                assert pk.lib().ARKodeSetFusedPCGBulk(I.h, args.pcg_bulk) == 0
         I.ARKodeSetOrder(3)
         I.ARKodeSetFixedStep(dt)
         I.ARKodeSetLSRelativeTolerance(1e-8)
@@ -827,6 +829,9 @@ def parse_args(argv=None):
     ap.add_argument("--c5-methods", default="jacobi,mg")
     ap.add_argument("--c5-train-ratio", type=float, default=0.0,
                     help="c5: gamma*dt*k/dx^2 the shared network is trained at (0: the batch median)")
+    ap.add_argument("--pcg-bulk", type=float, default=0.75,
+                    help="c5: fraction of the previous solve's iterations the fused PCG runs before testing "
+                         "the residual before each preconditioner call")
     ap.add_argument("--c5-nets", default="",
                     help="c5: .npz file of trained networks: loaded when it holds networks for the same training "
                          "operators, else written after training")

@@ -111,6 +111,7 @@ _SIGS = {
     "LSRKStepSetNumDomEigEstPreprocessIters": (i32, [vp, i32]),
     "LSRKStepSetDiffusionFastPath": (i32, [vp, vp, i32]),
     "ARKodeSetDiffusionFastPCG": (i32, [vp, vp]),
+    "ARKodeSetFusedPCGBulk": (i32, [vp, d]),
     "LSRKStepGetSpectralRadius": (i32, [vp, dp]),
     "SUNDomEigEstimator_Power": (vp, [vp, C.c_long, d, vp]),
     "SUNDomEigEstimator_Destroy": (i32, [C.POINTER(vp)]),

@@ -85,6 +85,7 @@ struct ArkMem {
   // fused path
   DiffusionProblem* fast = nullptr;
   DiffusionProblem* fast_pcg = nullptr;  // fused PCG (ARKodeSetDiffusionFastPCG)
+  double pcg_bulk = 0.75;                // ARKodeSetFusedPCGBulk
   int use_graphs = 0;
   StageEntry* h_tab = nullptr;
   StageEntry* d_tab = nullptr;

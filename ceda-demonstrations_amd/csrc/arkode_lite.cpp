@@ -1217,6 +1217,14 @@ int LSRKStepGetSpectralRadius(void* m, sunrealtype* rho)
   return 0;
 }
 
+int ARKodeSetFusedPCGBulk(void* m, sunrealtype fraction)
+{
+  if (!m) return ARK_MEM_NULL;
+  if (!(fraction >= 0.0 && fraction <= 1.0)) return ARK_ILL_INPUT;
+  AM(m)->pcg_bulk = fraction;
+  return 0;
+}
+
 int ARKodeSetDiffusionFastPCG(void* m, DiffusionProblem* prob)
 {
   if (!m) return ARK_MEM_NULL;

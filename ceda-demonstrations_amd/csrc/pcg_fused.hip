@@ -542,9 +542,10 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
   // iteration is the stencil/direction kernel, the update, the callback's
   // z = M r (enqueued on the same stream) and r.z (k_pcg_dot).  Every kernel
   // returns at once after convergence, but the callback's own work would
-  // still be spent, so the host enqueues the first 3/4 of the previous
-  // solve's iteration count without waiting and then tests each residual
-  // (k_pcg_rtest, one wait) before it calls the preconditioner.
+  // still be spent, so the host enqueues a fraction (A->pcg_bulk, 3/4 by
+  // default) of the previous solve's iteration count without waiting and
+  // then tests each residual (k_pcg_rtest, one wait) before it calls the
+  // preconditioner.
   double* q[2] = {S->d_p2, dp};
   // pairs of cells with 16-byte accesses: an even row length (every pair in
   // one row, every pair's first cell 16-byte aligned)
@@ -555,7 +556,7 @@ int pcg_fused(ArkMem* A, SUNLinearSolver S, DiffusionProblem* P, int pmode, ARKL
   const JacobiDiag J{per ? P->d_px : P->d_fx, per ? P->d_py : P->d_fy, P->diag_gamma};
   hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, st, rz, delta);
   if (pmode == 2) {
-    const long bulk = (S->last_iters > 4) ? S->last_iters * 3 / 4 : 0;
+    const long bulk = (S->last_iters > 4) ? (long)(A->pcg_bulk * (double)S->last_iters) : 0;
     for (int it = 1; it <= S->maxl; it++) {
       const double* pold = it == 1 ? q[1] : q[(it - 1) & 1];
       if (pairs) {

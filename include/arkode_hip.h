@@ -114,6 +114,13 @@ int LSRKStepSetDiffusionFastPath(void* arkode_mem, DiffusionProblem* prob, int u
    the N_Vector PCG, not bitwise equal to it (the dot products are summed in
    another order).  prob = NULL switches it off. */
 int ARKodeSetDiffusionFastPCG(void* arkode_mem, DiffusionProblem* prob);
+/* [ext] with a preconditioner callback, the fused PCG enqueues this fraction
+   of the previous solve's iteration count without testing the residual
+   (the host waits only every few iterations), then tests it before every
+   callback (default 0.75; 0..1, ARK_ILL_INPUT otherwise).  A larger
+   fraction waits less but may run the callback for iterations past
+   convergence. */
+int ARKodeSetFusedPCGBulk(void* arkode_mem, sunrealtype fraction);
 
 SUNDomEigEstimator SUNDomEigEstimator_Power(N_Vector q, long max_iters, sunrealtype rel_tol,
                                             SUNContext sunctx);

@@ -27,7 +27,7 @@ def ctx(pk):
     c.close()
 
 
-def dirk(pk, ctx, kind, side, dt, tout, precond, fused, reltol=1e-8, jtimes=True, ny=None):
+def dirk(pk, ctx, kind, side, dt, tout, precond, fused, reltol=1e-8, jtimes=True, ny=None, bulk=None):
     G = pk.Problem(ctx, kind, nx=side, ny=ny or side, kx=1.0, ky=0.7)
     I = pk.Integrator(G, "dirk", G.initial(), ls="cg", maxl=20000, precond=precond)
     I.ARKodeSetOrder(3)
@@ -41,6 +41,8 @@ def dirk(pk, ctx, kind, side, dt, tout, precond, fused, reltol=1e-8, jtimes=True
         I.analytic_jtimes()
     if fused:
         assert I.fused_pcg() == 0
+    if bulk is not None:
+        assert pk.lib().ARKodeSetFusedPCGBulk(I.h, bulk) == 0
     flag, t = I.evolve(tout)
     out = dict(flag=flag, t=t, y=I.y.get(), stats=I.stats())
     I.free()
@@ -115,3 +117,26 @@ def test_fused_pcg_off_with_difference_quotient(pk, ctx):
     b = dirk(pk, ctx, pk.MMS_DIRICHLET, 48, 2e-3, 6e-3, True, True, jtimes=False)
     assert a["stats"] == b["stats"]
     np.testing.assert_array_equal(a["y"], b["y"])
+
+
+@pytest.mark.parametrize("bulk", [0.0, 1.0])
+def test_fused_pcg_bulk_fraction_changes_only_the_waits(pk, ctx, bulk):
+    """With a preconditioner callback (here the multigrid V-cycle) the host
+    enqueues a fraction of the previous solve's iterations before it tests
+    the residual; the device stops each solve at the same iterate whatever
+    the fraction, so the states, the counts and every bit agree."""
+    ref = dirk(pk, ctx, pk.MMS_DIRICHLET, 96, 2e-3, 8e-3, "mg", True)
+    got = dirk(pk, ctx, pk.MMS_DIRICHLET, 96, 2e-3, 8e-3, "mg", True, bulk=bulk)
+    assert got["flag"] == ref["flag"] == 0
+    for f in ("nst", "nni", "nli"):
+        assert got["stats"][f] == ref["stats"][f], f
+    assert np.array_equal(got["y"], ref["y"])
+
+
+def test_fused_pcg_bulk_fraction_rejects_other_values(pk, ctx):
+    G = pk.Problem(ctx, pk.MMS_DIRICHLET, nx=16, ny=16, kx=1.0, ky=1.0)
+    I = pk.Integrator(G, "dirk", G.initial(), ls="cg", maxl=100, precond=True)
+    assert pk.lib().ARKodeSetFusedPCGBulk(I.h, -0.1) != 0
+    assert pk.lib().ARKodeSetFusedPCGBulk(I.h, 1.5) != 0
+    assert pk.lib().ARKodeSetFusedPCGBulk(I.h, float("nan")) != 0
+    I.free()
