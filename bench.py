@@ -485,10 +485,12 @@ def c5_workload(pk, args):
     integrated to t = steps*dt.  The problems run as independent integrators
     driven concurrently by `--c5-workers` host threads, each with its own
     SUNContext (HIP stream); a worker integrates its problems one after
-    another.  Methods: Jacobi-PCG (the reference's configuration), FGMRES
-    with one learned network shared by every problem (trained once on the GPU
-    at the batch's median gamma*dt*k/dx^2, outside the timed region; its
-    output scaled per problem by the inverse Jacobi diagonal) and PCG with the
+    another.  Methods: Jacobi-PCG (the reference's configuration), PCG and
+    FGMRES with learned symmetric networks (one per class of problem
+    anisotropy and magnitude, --c5-classes x --c5-mag-classes, or with
+    --c5-classes 0 one isotropic network at the batch's median
+    gamma*dt*k/dx^2; trained on the GPU outside the timed region; the output
+    scaled per problem by its inverse Jacobi diagonal) and PCG with the
     multigrid V-cycle.  Time-to-solution is the wall time of the whole batch,
     problem setup included (the L2 check against the exact solution and the
     teardown follow the timed region; the workers' contexts and the trained
@@ -826,7 +828,9 @@ def parse_args(argv=None):
     ap.add_argument("--c5-n", type=int, default=64, help="c5: number of problems")
     ap.add_argument("--c5-steps", type=int, default=3, help="c5: DIRK3 steps per problem")
     ap.add_argument("--c5-workers", type=int, default=8, help="c5: concurrent host threads")
-    ap.add_argument("--c5-methods", default="jacobi,mg")
+    ap.add_argument("--c5-methods", default="jacobi,mg,learned-sym-pcg,learned-sym",
+                    help="c5: methods, from jacobi, mg, learned-sym-pcg (PCG + symmetric network), learned-sym "
+                         "(FGMRES + symmetric network), learned (FGMRES + ReLU network)")
     ap.add_argument("--c5-train-ratio", type=float, default=0.0,
                     help="c5: gamma*dt*k/dx^2 the shared network is trained at (0: the batch median)")
     ap.add_argument("--pcg-bulk", type=float, default=0.75,
@@ -835,9 +839,9 @@ def parse_args(argv=None):
     ap.add_argument("--c5-nets", default="",
                     help="c5: .npz file of trained networks: loaded when it holds networks for the same training "
                          "operators, else written after training")
-    ap.add_argument("--c5-mag-classes", type=int, default=1,
+    ap.add_argument("--c5-mag-classes", type=int, default=3,
                     help="c5: with --c5-classes, split each anisotropy class into this many classes of r_x r_y")
-    ap.add_argument("--c5-classes", type=int, default=0,
+    ap.add_argument("--c5-classes", type=int, default=4,
                     help="c5: learned methods train one network per class of problem anisotropy (this many "
                          "classes of equal count); 0: one isotropic network shared by every problem")
     ap.add_argument("--tout", type=float, default=0.5)

@@ -126,7 +126,7 @@ def test_c5_problems_reach_steps_times_dt_after_training(capsys):
     bench.hip_sync = lambda: None
     stub = _Stub()
     args = bench.parse_args(["--workload", "c5", "--c5-n", "3", "--c5-workers", "1", "--c5-steps", "3",
-                             "--c5-methods", "jacobi,learned-sym,mg", "--nsamp", "4",
+                             "--c5-methods", "jacobi,learned-sym,mg", "--nsamp", "4", "--c5-classes", "0",
                              "--sym-train-steps", "1000"])
     bench.c5_workload(stub, args)
     assert stub.train_steps == [1000]
@@ -149,7 +149,7 @@ def test_c5_classes_train_one_network_per_anisotropy_class():
         stub = _Stub()
         args = bench.parse_args(["--workload", "c5", "--c5-n", "6", "--c5-workers", "1", "--c5-steps", "1",
                                  "--c5-methods", "learned-sym,learned-sym-pcg", "--nsamp", "4",
-                                 "--c5-classes", str(classes)])
+                                 "--c5-classes", str(classes), "--c5-mag-classes", "1"])
         bench.c5_workload(stub, args)
         assert len(stub.train_ratios) == want  # the PCG method shares the FGMRES method's networks
         assert len(stub.evolve_targets) == 12
@@ -170,9 +170,9 @@ def test_c5_nets_file_is_written_then_reused(tmp_path):
     bench.hip_sync = lambda: None
     path = str(tmp_path / "nets.npz")
     argv = ["--workload", "c5", "--c5-n", "4", "--c5-workers", "1", "--c5-steps", "1",
-            "--c5-methods", "learned-sym-pcg", "--nsamp", "4", "--c5-nets", path]
+            "--c5-methods", "learned-sym-pcg", "--nsamp", "4", "--c5-nets", path, "--c5-mag-classes", "1"]
     counts = []
-    for extra in ([], [], ["--c5-classes", "2"]):
+    for extra in (["--c5-classes", "0"], ["--c5-classes", "0"], ["--c5-classes", "2"]):
         stub = _Stub()
         bench.c5_workload(stub, bench.parse_args(argv + extra))
         counts.append(len(stub.train_ratios))
@@ -201,3 +201,10 @@ def test_c5_magnitude_classes_split_each_anisotropy_class():
         for m, grp in enumerate((half[:2], half[2:])):
             want = [np.exp(np.mean([np.log(rxy[k][a]) for k in grp])) for a in (0, 1)]
             assert np.allclose(stub.train_ratios[2 * c + m], want, rtol=1e-12)
+
+
+def test_c5_defaults_are_twelve_class_networks_and_every_method():
+    bench = load_bench()
+    args = bench.parse_args(["--workload", "c5"])
+    assert (args.c5_classes, args.c5_mag_classes) == (4, 3)
+    assert set(args.c5_methods.split(",")) == {"jacobi", "mg", "learned-sym-pcg", "learned-sym"}
