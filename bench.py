@@ -711,8 +711,8 @@ def c5_workload(pk, args):
         res[kind], errs[kind] = run(kind, args.c5_workers)
     if "jacobi" in res and args.c5_workers != 1:
         res["jacobi_one_worker"], _ = run("jacobi", 1)
-    out = {"metric": "time-to-solution at fixed L2 err, batched implicit: shared learned preconditioner "
-                     "(FGMRES) and multigrid (PCG) vs Jacobi-PCG",
+    out = {"metric": "time-to-solution at fixed L2 err, batched implicit: learned preconditioners (PCG, "
+                     "FGMRES) and multigrid (PCG) vs Jacobi-PCG",
            "config": {"workload": f"{len(probs)} mms-dirichlet problems, side U[256,1024], kx,ky logU[0.1,10] "
                                   f"(seed 20260102), {cells} cells, DIRK3 fixed dt=1e-3, {steps} steps, "
                                   "LS reltol 1e-8, analytic J v",
